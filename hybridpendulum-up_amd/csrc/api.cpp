@@ -1,0 +1,521 @@
+// C ABI of libhpup: the plan (device-resident) API and the gateway
+// equivalents of the reference's MEX entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <strings.h>
+
+#include "drift.h"
+#include "hpup_internal.h"
+#include "stats.h"
+
+namespace hpup {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& m) { g_last_error = m; }
+
+using cd = std::complex<double>;
+static const double kPi = 3.14159265358979323846;
+
+// --------------------------------------------------------- table upload --
+
+template <class T> std::vector<T> cast_vec(const std::vector<double>& v) { return std::vector<T>(v.begin(), v.end()); }
+
+// Transform tables from a Wigner-d table d (M,M,B,2B) and weights w (2B).
+template <class T> struct XformTables {
+    DevBuf Wf, Wi, twB, dftN, idftN, twNf, twNi;
+    XformDev<T> view{};
+    void build(int B, int dim, int N, const double* d, const double* w) {
+        const int M = 2 * B - 1, n2 = 2 * B;
+        std::vector<double> wf((size_t)M * M * B * n2, 0.0), wi((size_t)M * M * B * n2, 0.0);
+        for (int mn = 0; mn < M * M; mn++)
+            for (int l = 0; l < B; l++)
+                for (int j = 0; j < n2; j++) {
+                    const int mi = mn % M, ni = mn / M;
+                    const double dv = d[mi + (size_t)M * (ni + (size_t)M * (l + (size_t)B * j))];
+                    wf[((size_t)mn * B + l) * n2 + j] = w[j] * dv;
+                    wi[((size_t)mn * n2 + j) * B + l] = (2.0 * l + 1.0) * dv;
+                }
+        Wf = upload(cast_vec<T>(wf));
+        Wi = upload(cast_vec<T>(wi));
+        std::vector<T> tb(2 * n2), dn, idn, tf, ti;
+        for (int t = 0; t < n2; t++) { tb[2 * t] = (T)std::cos(kPi * t / B); tb[2 * t + 1] = (T)std::sin(kPi * t / B); }
+        twB = upload(tb);
+        const int Nn = std::max(N, 1);
+        dn.resize(2 * (size_t)Nn * Nn); idn.resize(2 * (size_t)Nn * Nn); tf.resize(2 * Nn); ti.resize(2 * Nn);
+        for (int k = 0; k < Nn; k++) {
+            for (int a = 0; a < Nn; a++) {
+                const double ang = 2.0 * kPi * (double)((int64_t)k * a % Nn) / Nn;
+                dn[2 * ((size_t)k * Nn + a)] = (T)std::cos(ang);
+                dn[2 * ((size_t)k * Nn + a) + 1] = (T)-std::sin(ang);
+                idn[2 * ((size_t)k * Nn + a)] = (T)(std::cos(ang) / Nn);
+                idn[2 * ((size_t)k * Nn + a) + 1] = (T)(std::sin(ang) / Nn);
+            }
+            const double ang = 2.0 * kPi * k / Nn;
+            tf[2 * k] = (T)std::cos(ang); tf[2 * k + 1] = (T)-std::sin(ang);
+            ti[2 * k] = (T)std::cos(ang); ti[2 * k + 1] = (T)std::sin(ang);
+        }
+        dftN = upload(dn); idftN = upload(idn); twNf = upload(tf); twNi = upload(ti);
+        view.B = B; view.dim = dim; view.N = Nn;
+        view.nx = ipow64(Nn, dim); view.nrc = nrc_of(B);
+        view.Wf = Wf.as<T>(); view.Wi = Wi.as<T>(); view.twB = twB.as<cT<T>>();
+        view.dftN = dftN.as<cT<T>>(); view.idftN = idftN.as<cT<T>>();
+        view.twNf = twNf.as<cT<T>>(); view.twNi = twNi.as<cT<T>>();
+    }
+};
+
+template <class T> struct DriftTables {
+    DevBuf cg_src, cg_coef, u_src, u_coef, e3, kap, kc, kd, M3;
+    DriftDev<T> view{};
+    void build(const DriftSpec& s) {
+        const int N = s.N;
+        cg_src = upload(s.cg_src);
+        cg_coef = upload(cast_vec<T>(s.cg_coef));
+        u_src = upload(s.u_src);
+        u_coef = upload(cast_vec<T>(s.u_coef));
+        e3 = upload(cast_vec<T>(s.e3));
+        kap = upload(cast_vec<T>(s.kappa));
+        std::vector<double> kcv(N), kdv(N);
+        const int Bx = N / 2;
+        for (int k = 0; k < N; k++) {
+            const double kk = k < Bx ? k : k - N;
+            const double kp = k < Bx ? k : (k == Bx ? 0 : k - N);
+            kcv[k] = 2.0 * kPi * kp / s.L;
+            kdv[k] = -(2.0 * kPi * kk / s.L) * (2.0 * kPi * kk / s.L);
+        }
+        kc = upload(cast_vec<T>(kcv));
+        kd = upload(cast_vec<T>(kdv));
+        if (s.dim == 3) {
+            std::vector<double> m3(2 * (size_t)N * N);
+            for (int k = 0; k < N; k++)
+                for (int a = 0; a < N; a++) {
+                    const int j = ((k - a) % N + N) % N;
+                    m3[2 * ((size_t)k * N + a)] = s.kappa[2 * (2 * N + j)];
+                    m3[2 * ((size_t)k * N + a) + 1] = s.kappa[2 * (2 * N + j) + 1];
+                }
+            M3 = upload(cast_vec<T>(m3));
+        }
+        DriftDev<T>& v = view;
+        v.B = s.B; v.dim = s.dim; v.N = N;
+        v.nx = ipow64(N, s.dim); v.nrc = nrc_of(s.B);
+        v.TG = s.TG; v.TU = s.TU;
+        v.cg_src = cg_src.as<int32_t>(); v.cg_coef = cg_coef.as<cT<T>>();
+        v.u_src = u_src.as<int32_t>(); v.u_coef = u_coef.as<cT<T>>();
+        v.e3 = e3.as<cT<T>>(); v.kap = kap.as<cT<T>>();
+        v.kc = kc.as<T>(); v.kd = kd.as<T>();
+        for (int i = 0; i < 9; i++) v.G[i] = (T)s.G[i];
+        for (int i = 0; i < 3; i++) { v.b[i] = (T)s.b[i]; v.gyro[i] = (T)s.gyro[i]; }
+        v.has_gyro = s.has_gyro ? 1 : 0;
+    }
+};
+
+// ---------------------------------------------------------------- plan --
+
+struct PlanBase {
+    hpup_model model{};
+    int64_t nx = 1, nrc = 0, ncoef = 0, ngrid = 0;
+    std::vector<double> w;  // beta weights (FP64)
+    DevBuf w_dev;
+    virtual ~PlanBase() = default;
+    virtual void forward(const void* f, void* F, hipStream_t st) = 0;
+    virtual void backward(const void* F, void* f, hipStream_t st) = 0;
+    virtual void step(void* F, int nsteps, hipStream_t st) = 0;
+    virtual void derivative(const void* F, void* dF, hipStream_t st) = 0;
+    virtual void full_to_compact(const void* a, void* b, hipStream_t st) = 0;
+    virtual void compact_to_full(const void* a, void* b, hipStream_t st) = 0;
+    virtual void stat(const void* f, double* ER, double* Ex, double* Varx, double* fx, hipStream_t st) = 0;
+};
+
+template <class T> struct PlanT : PlanBase {
+    XformTables<T> xf;
+    DriftTables<T> dr;
+    bool has_drift = false;
+    DevBuf G, Wb, V1, V2, Z3, Fs, ACC, tmp, fxb, fRb;
+
+    void init_xform(const double* d, const double* wv) {
+        const hpup_model& m = model;
+        xf.build(m.B, m.dim, m.N, d, wv);
+    }
+    void init_drift(const DriftSpec& s) {
+        dr.build(s);
+        has_drift = true;
+    }
+    void ensure(DevBuf& b, size_t bytes) {
+        if (b.bytes < bytes) b.alloc(bytes);
+    }
+    size_t cbytes() const { return sizeof(cT<T>) * (size_t)ncoef; }
+
+    void forward(const void* f, void* F, hipStream_t st) override {
+        const int M = 2 * model.B - 1;
+        ensure(G, sizeof(cT<T>) * (size_t)M * M * 2 * model.B * nx);
+        launch_fwd_ag<T>(xf.view, f, G.p, st);
+        launch_beta_fwd<T>(xf.view, G.p, F, st);
+        launch_omega_dft<T>(xf.view, F, nullptr, false, st);
+        HPUP_HIP(hipGetLastError());
+    }
+    void backward(const void* F, void* f, hipStream_t st) override {
+        const int M = 2 * model.B - 1;
+        ensure(G, sizeof(cT<T>) * (size_t)M * M * 2 * model.B * nx);
+        ensure(tmp, cbytes());
+        HPUP_HIP(hipMemcpyAsync(tmp.p, F, cbytes(), hipMemcpyDeviceToDevice, st));
+        launch_omega_dft<T>(xf.view, tmp.p, nullptr, true, st);
+        launch_beta_inv<T>(xf.view, tmp.p, G.p, st);
+        launch_inv_ag<T>(xf.view, G.p, f, st);
+        HPUP_HIP(hipGetLastError());
+    }
+    void stage(const void* Fin, const void* F0, void* Fout, const StageCoef<T>& sc, hipStream_t st) {
+        const DriftDev<T>& D = dr.view;
+        launch_rotmix<T>(D, Fin, Wb.p, V1.p, V2.p, st);
+        if (D.dim == 3) launch_axis_outer<T>(Fin, Z3.p, dr.M3.p, D.N, (int64_t)D.N * D.N, D.nrc, T(1), st);
+        launch_drift_slab<T>(D, Fin, Z3.p, Wb.p, V1.p, V2.p, F0, Fout, ACC.p, sc, st);
+    }
+    void ensure_drift_buffers(bool rk) {
+        HPUP_REQUIRE(has_drift, HPUP_ERR_ARG, "plan has no drift operator (dim must be 2 or 3)");
+        ensure(Wb, cbytes()); ensure(V1, cbytes()); ensure(V2, cbytes());
+        if (model.dim == 3) ensure(Z3, cbytes());
+        if (rk) { ensure(Fs, cbytes()); ensure(ACC, cbytes()); }
+    }
+    void step(void* Fv, int nsteps, hipStream_t st) override {
+        const int meth = model.method;
+        ensure_drift_buffers(meth != HPUP_EULER);
+        const T dt = (T)model.dt;
+        cT<T>* F = (cT<T>*)Fv;
+        for (int it = 0; it < nsteps; it++) {
+            switch (meth) {
+            case HPUP_EULER:
+                stage(F, nullptr, F, StageCoef<T>{dt, 0, 1, 0, 0, 0, 0, 0, 0}, st);
+                break;
+            case HPUP_MIDPOINT:
+                stage(F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 0, 0, 0, 0, 0}, st);
+                stage(Fs.p, F, F, StageCoef<T>{dt, 0, 1, 0, 0, 1, 0, 0, 0}, st);
+                break;
+            case HPUP_RK2:
+                stage(F, nullptr, Fs.p, StageCoef<T>{dt, 0, 1, 0, 1, 0, 0, 1, 0}, st);
+                stage(Fs.p, F, F, StageCoef<T>{dt / 2, 1, 1, 0, 0, 1, 1, 0, 0}, st);
+                break;
+            case HPUP_RK4:
+                stage(F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 1, 0, 0, 1, 0}, st);
+                stage(Fs.p, F, Fs.p, StageCoef<T>{dt / 2, 0, 1, 1, 2, 1, 1, 1, 0}, st);
+                stage(Fs.p, F, Fs.p, StageCoef<T>{dt, 0, 1, 1, 2, 1, 1, 1, 0}, st);
+                stage(Fs.p, F, F, StageCoef<T>{dt / 6, 1, 1, 0, 0, 1, 1, 0, 0}, st);
+                break;
+            default:
+                throw Error{HPUP_ERR_METHOD, "unknown integrator"};
+            }
+        }
+        HPUP_HIP(hipGetLastError());
+    }
+    void derivative(const void* F, void* dF, hipStream_t st) override {
+        ensure_drift_buffers(false);
+        stage(F, nullptr, dF, StageCoef<T>{1, 0, 1, 0, 0, 0, 0, 0, 1}, st);
+        HPUP_HIP(hipGetLastError());
+    }
+    void full_to_compact(const void* a, void* b, hipStream_t st) override {
+        launch_full_to_compact<T>(model.B, nx, a, b, st);
+        HPUP_HIP(hipGetLastError());
+    }
+    void compact_to_full(const void* a, void* b, hipStream_t st) override {
+        launch_compact_to_full<T>(model.B, nx, a, b, st);
+        HPUP_HIP(hipGetLastError());
+    }
+    void stat(const void* f, double* ER, double* Ex, double* Varx, double* fx, hipStream_t st) override {
+        const int B = model.B, n2 = 2 * B, dim = model.dim, N = std::max(model.N, 1);
+        const int64_t nR = (int64_t)n2 * n2 * n2;
+        ensure(fxb, sizeof(double) * nx);
+        ensure(fRb, sizeof(double) * nR);
+        launch_marginals<T>(f, B, nx, w_dev.as<double>(), fxb.as<double>(), fRb.as<double>(), st);
+        HPUP_HIP(hipGetLastError());
+        std::vector<double> hx(nx), hR(nR);
+        HPUP_HIP(hipMemcpyAsync(hx.data(), fxb.p, sizeof(double) * nx, hipMemcpyDeviceToHost, st));
+        HPUP_HIP(hipMemcpyAsync(hR.data(), fRb.p, sizeof(double) * nR, hipMemcpyDeviceToHost, st));
+        HPUP_HIP(hipStreamSynchronize(st));
+        const double L = model.L;
+        const double dv = dim > 0 ? std::pow(L / N, dim) : 1.0;
+        // E[R] = sum_R R w_j fR dv  (R = Rz(alpha) Ry(beta) Rz(gamma), fftSO3R.m:38-49)
+        double acc[9] = {0};
+        for (int64_t k = 0; k < n2; k++)
+            for (int64_t j = 0; j < n2; j++)
+                for (int64_t i = 0; i < n2; i++) {
+                    const double a = kPi / B * i, b = kPi / (4.0 * B) * (2 * j + 1), g = kPi / B * k;
+                    const double ca = std::cos(a), sa = std::sin(a), cb = std::cos(b), sb = std::sin(b),
+                                 cg = std::cos(g), sg = std::sin(g);
+                    const double R[9] = {ca * cb * cg - sa * sg, sa * cb * cg + ca * sg, -sb * cg,
+                                         -ca * cb * sg - sa * cg, -sa * cb * sg + ca * cg, sb * sg,
+                                         ca * sb, sa * sb, cb};
+                    const double wt = hR[i + n2 * (j + n2 * k)] * w[j] * dv;
+                    for (int q = 0; q < 9; q++) acc[q] += R[q] * wt;
+                }
+        if (ER) std::memcpy(ER, acc, sizeof(acc));
+        if (fx) std::memcpy(fx, hx.data(), sizeof(double) * nx);
+        if (dim > 0) {
+            double E[3] = {0, 0, 0}, S[9] = {0};
+            for (int64_t x = 0; x < nx; x++) {
+                double v[3];
+                for (int p = 0; p < dim; p++) v[p] = -L / 2 + L / N * ((x / ipow64(N, p)) % N);
+                for (int p = 0; p < dim; p++) {
+                    E[p] += v[p] * hx[x];
+                    for (int q = 0; q < dim; q++) S[p + dim * q] += v[p] * v[q] * hx[x];
+                }
+            }
+            for (int p = 0; p < dim; p++) E[p] *= dv;
+            if (Ex) for (int p = 0; p < dim; p++) Ex[p] = E[p];
+            if (Varx)
+                for (int p = 0; p < dim; p++)
+                    for (int q = 0; q < dim; q++) Varx[p + dim * q] = S[p + dim * q] * dv - E[p] * E[q];
+        }
+    }
+};
+
+static PlanBase* make_plan(const hpup_model& m, const double* d, const double* w, const DriftSpec* spec) {
+    HPUP_REQUIRE(m.B >= 1 && m.B <= 128, HPUP_ERR_ARG, "B out of range");
+    HPUP_REQUIRE(m.dim == 0 || m.dim == 2 || m.dim == 3, HPUP_ERR_ARG, "dim must be 0, 2 or 3");
+    HPUP_REQUIRE(m.dim == 0 || (m.N >= 2 && m.N % 2 == 0 && m.N <= 64), HPUP_ERR_ARG, "N must be even, 2..64");
+    std::unique_ptr<PlanBase> p;
+    if (m.precision == HPUP_FP64) {
+        auto* q = new PlanT<double>();
+        p.reset(q);
+    } else if (m.precision == HPUP_FP32) {
+        auto* q = new PlanT<float>();
+        p.reset(q);
+    } else {
+        throw Error{HPUP_ERR_UNSUPPORTED, "precision mode not available"};
+    }
+    p->model = m;
+    if (m.dim == 0) p->model.N = 1;
+    p->nx = m.dim == 0 ? 1 : ipow64(m.N, m.dim);
+    p->nrc = nrc_of(m.B);
+    p->ncoef = p->nrc * p->nx;
+    p->ngrid = 8LL * m.B * m.B * m.B * p->nx;
+    p->w.assign(w, w + 2 * m.B);
+    p->w_dev = upload(p->w);
+    if (m.precision == HPUP_FP64) {
+        auto* q = static_cast<PlanT<double>*>(p.get());
+        q->init_xform(d, w);
+        if (spec) q->init_drift(*spec);
+    } else {
+        auto* q = static_cast<PlanT<float>*>(p.get());
+        q->init_xform(d, w);
+        if (spec) q->init_drift(*spec);
+    }
+    return p.release();
+}
+
+template <class F> int guard(F&& fn) {
+    try {
+        fn();
+        return HPUP_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return HPUP_ERR_ARG;
+    }
+}
+
+}  // namespace hpup
+
+using namespace hpup;
+
+struct hpup_plan {
+    std::unique_ptr<PlanBase> impl;
+};
+
+extern "C" {
+
+const char* hpup_last_error(void) { return g_last_error.c_str(); }
+int hpup_version(void) { return 100; }
+
+int hpup_parse_method(const char* method) {
+    if (!method) return -1;
+    if (!strcasecmp(method, "euler")) return HPUP_EULER;
+    if (!strcasecmp(method, "midpoint")) return HPUP_MIDPOINT;
+    if (!strcasecmp(method, "RK2")) return HPUP_RK2;
+    if (!strcasecmp(method, "RK4")) return HPUP_RK4;
+    return -1;
+}
+
+int hpup_plan_create(const hpup_model* m, hpup_plan** out) {
+    return guard([&] {
+        HPUP_REQUIRE(m && out, HPUP_ERR_ARG, "null argument");
+        *out = nullptr;
+        HPUP_REQUIRE(m->method >= 0 && m->method <= 3, HPUP_ERR_METHOD, "unknown integrator");
+        const int B = m->B, M = 2 * B - 1;
+        std::vector<double> beta, w;
+        host_grid(B, beta, w);
+        std::vector<double> d((size_t)M * M * B * 2 * B);
+        for (int j = 0; j < 2 * B; j++) host_wigner_d(beta[j], B - 1, d.data() + (size_t)j * M * M * B);
+        std::unique_ptr<DriftSpec> spec;
+        if (m->dim >= 2) spec.reset(new DriftSpec(build_drift_spec_from_model(*m)));
+        auto* h = new hpup_plan();
+        h->impl.reset(make_plan(*m, d.data(), w.data(), spec.get()));
+        *out = h;
+    });
+}
+
+int hpup_plan_destroy(hpup_plan* p) {
+    return guard([&] { delete p; });
+}
+
+int64_t hpup_plan_num_coeffs(const hpup_plan* p) { return p ? p->impl->ncoef : -1; }
+int64_t hpup_plan_num_grid(const hpup_plan* p) { return p ? p->impl->ngrid : -1; }
+
+int hpup_plan_forward(hpup_plan* p, const void* f, void* F, void* stream) {
+    return guard([&] { p->impl->forward(f, F, (hipStream_t)stream); });
+}
+int hpup_plan_backward(hpup_plan* p, const void* F, void* f, void* stream) {
+    return guard([&] { p->impl->backward(F, f, (hipStream_t)stream); });
+}
+int hpup_plan_step(hpup_plan* p, void* F, int nsteps, void* stream) {
+    return guard([&] { p->impl->step(F, nsteps, (hipStream_t)stream); });
+}
+int hpup_plan_derivative(hpup_plan* p, const void* F, void* dF, void* stream) {
+    return guard([&] { p->impl->derivative(F, dF, (hipStream_t)stream); });
+}
+int hpup_plan_full_to_compact(hpup_plan* p, const void* a, void* b, void* stream) {
+    return guard([&] { p->impl->full_to_compact(a, b, (hipStream_t)stream); });
+}
+int hpup_plan_compact_to_full(hpup_plan* p, const void* a, void* b, void* stream) {
+    return guard([&] { p->impl->compact_to_full(a, b, (hipStream_t)stream); });
+}
+int hpup_plan_stat_moments(hpup_plan* p, const void* f, double* ER, double* Ex, double* Varx, double* fx,
+                           void* stream) {
+    return guard([&] { p->impl->stat(f, ER, Ex, Varx, fx, (hipStream_t)stream); });
+}
+
+// ------------------------------------------------------- gateways: FFT --
+
+int hpup_fftSO3R_forward(const double* f, const int64_t* dims, int ndims, const double* d, const double* w,
+                         double* F) {
+    return guard([&] {
+        HPUP_REQUIRE(f && dims && d && w && F, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(ndims == 3 || ndims == 5 || ndims == 6, HPUP_ERR_ARG, "f must have rank 3, 5 or 6");
+        HPUP_REQUIRE(dims[0] % 2 == 0 && dims[0] == dims[1] && dims[1] == dims[2], HPUP_ERR_ARG,
+                     "f must be (2B,2B,2B,...)");
+        hpup_model m{};
+        m.B = (int)(dims[0] / 2);
+        m.dim = ndims - 3;
+        m.N = ndims > 3 ? (int)dims[3] : 1;
+        for (int i = 4; i < ndims; i++) HPUP_REQUIRE(dims[i] == dims[3], HPUP_ERR_ARG, "omega axes must be equal");
+        m.precision = HPUP_FP64;
+        std::unique_ptr<PlanBase> p(make_plan(m, d, w, nullptr));
+        const int64_t nfull = (int64_t)(2 * m.B - 1) * (2 * m.B - 1) * m.B * p->nx;
+        DevBuf fd(sizeof(double) * p->ngrid), Fc(sizeof(double) * 2 * p->ncoef), Ff(sizeof(double) * 2 * nfull);
+        HPUP_HIP(hipMemcpy(fd.p, f, fd.bytes, hipMemcpyHostToDevice));
+        p->forward(fd.p, Fc.p, nullptr);
+        p->compact_to_full(Fc.p, Ff.p, nullptr);
+        HPUP_HIP(hipMemcpy(F, Ff.p, Ff.bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int hpup_fftSO3R_backward(const double* F, const int64_t* dims, int ndims, const double* d, double* f) {
+    return guard([&] {
+        HPUP_REQUIRE(F && dims && d && f, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(ndims == 3 || ndims == 5 || ndims == 6, HPUP_ERR_ARG, "F must have rank 3, 5 or 6");
+        const int B = (int)dims[2];
+        HPUP_REQUIRE(dims[0] == 2 * B - 1 && dims[1] == 2 * B - 1, HPUP_ERR_ARG, "F must be (2B-1,2B-1,B,...)");
+        hpup_model m{};
+        m.B = B;
+        m.dim = ndims - 3;
+        m.N = ndims > 3 ? (int)dims[3] : 1;
+        for (int i = 4; i < ndims; i++) HPUP_REQUIRE(dims[i] == dims[3], HPUP_ERR_ARG, "omega axes must be equal");
+        m.precision = HPUP_FP64;
+        std::vector<double> beta, w;
+        host_grid(B, beta, w);  // weights are not used by the inverse
+        std::unique_ptr<PlanBase> p(make_plan(m, d, w.data(), nullptr));
+        const int64_t nfull = (int64_t)(2 * B - 1) * (2 * B - 1) * B * p->nx;
+        DevBuf Ff(sizeof(double) * 2 * nfull), Fc(sizeof(double) * 2 * p->ncoef), fd(sizeof(double) * p->ngrid);
+        HPUP_HIP(hipMemcpy(Ff.p, F, Ff.bytes, hipMemcpyHostToDevice));
+        p->full_to_compact(Ff.p, Fc.p, nullptr);
+        p->backward(Fc.p, fd.p, nullptr);
+        HPUP_HIP(hipMemcpy(f, fd.p, fd.bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ gateways: propagation --
+
+namespace hpup {
+
+template <class T>
+static void propagate_gateway(int dim, const T* F, const int64_t* dims, const T* X, const T* OJO, const T* MR,
+                              const T* bt, const T* G, double dtt, double L, const T* u, const T* const* CG,
+                              const char* method, T* Fnew) {
+    HPUP_REQUIRE(F && dims && X && MR && bt && G && u && CG && method && Fnew, HPUP_ERR_ARG, "null argument");
+    HPUP_REQUIRE(dim == 2 || OJO, HPUP_ERR_ARG, "OJO required for the 3-D model");
+    const int meth = hpup_parse_method(method);
+    HPUP_REQUIRE(meth >= 0, HPUP_ERR_METHOD, std::string("unknown method '") + method + "'");
+    const int B = (int)dims[2], M = 2 * B - 1, N = (int)dims[3];
+    HPUP_REQUIRE(dims[0] == M && dims[1] == M, HPUP_ERR_ARG, "F must be (2B-1,2B-1,B,N,...)");
+    for (int i = 4; i < 3 + dim; i++) HPUP_REQUIRE(dims[i] == N, HPUP_ERR_ARG, "omega axes must be equal");
+    const int64_t nx = ipow64(N, dim), nrot = (int64_t)M * M * B;
+    // promote the gateway arrays to FP64 for the table analysis
+    auto to_d = [](const T* a, int64_t n) { return std::vector<double>(a, a + n); };
+    std::vector<double> Xd = to_d(X, 2 * nx * dim), MRd = to_d(MR, 2 * nrot * dim), ud = to_d(u, 2 * nrot * 3);
+    std::vector<double> Od = dim == 3 ? to_d(OJO, 2 * nx * 3) : std::vector<double>();
+    double btd[3] = {0, 0, 0}, Gd[9] = {0};
+    for (int i = 0; i < dim; i++) btd[i] = bt[i];
+    for (int i = 0; i < dim * dim; i++) Gd[i] = G[i];
+    std::vector<std::vector<double>> cells((size_t)B * B);
+    std::vector<const double*> cptr((size_t)B * B, nullptr);
+    for (int l1 = 0; l1 < B; l1++) {
+        if (B < 2) break;
+        const T* c = CG[l1 + (size_t)B * 1];
+        HPUP_REQUIRE(c != nullptr, HPUP_ERR_ARG, "CG cell missing");
+        const int64_t nc = (int64_t)(2 * l1 + 1) * 3;
+        cells[l1 + (size_t)B] = to_d(c, nc * nc);
+        cptr[l1 + (size_t)B] = cells[l1 + (size_t)B].data();
+    }
+    DriftSpec spec = build_drift_spec_from_arrays(B, dim, N, L, Xd.data(), dim == 3 ? Od.data() : nullptr,
+                                                  MRd.data(), btd, Gd, ud.data(), cptr.data(),
+                                                  sizeof(T) == 8 ? 1e-9 : 1e-4);
+    hpup_model m{};
+    m.B = B; m.dim = dim; m.N = N; m.L = L; m.dt = dtt; m.method = meth;
+    m.precision = sizeof(T) == 8 ? HPUP_FP64 : HPUP_FP32;
+    for (int i = 0; i < dim; i++) m.b[i] = btd[i];
+    for (int i = 0; i < dim * dim; i++) m.G[i] = Gd[i];
+    std::vector<double> beta, w;
+    host_grid(B, beta, w);
+    std::vector<double> d((size_t)M * M * B * 2 * B);
+    for (int j = 0; j < 2 * B; j++) host_wigner_d(beta[j], B - 1, d.data() + (size_t)j * M * M * B);
+    std::unique_ptr<PlanBase> p(make_plan(m, d.data(), w.data(), &spec));
+    const int64_t nfull = nrot * nx;
+    DevBuf Ff(sizeof(T) * 2 * nfull), Fc(sizeof(T) * 2 * p->ncoef);
+    HPUP_HIP(hipMemcpy(Ff.p, F, Ff.bytes, hipMemcpyHostToDevice));
+    p->full_to_compact(Ff.p, Fc.p, nullptr);
+    p->step(Fc.p, 1, nullptr);
+    p->compact_to_full(Fc.p, Ff.p, nullptr);
+    HPUP_HIP(hipMemcpy(Fnew, Ff.p, Ff.bytes, hipMemcpyDeviceToHost));
+}
+
+}  // namespace hpup
+
+extern "C" {
+
+int hpup_pendulum_propagate_reduced(const double* F, const int64_t* dims, const double* X, const double* MR,
+                                    const double* bt, const double* G, double dtt, double L, const double* u,
+                                    const double* const* CG, const char* method, double* Fnew) {
+    return guard([&] { propagate_gateway<double>(2, F, dims, X, nullptr, MR, bt, G, dtt, L, u, CG, method, Fnew); });
+}
+int hpup_pendulum_propagate_reduced32(const float* F, const int64_t* dims, const float* X, const float* MR,
+                                      const float* bt, const float* G, float dtt, float L, const float* u,
+                                      const float* const* CG, const char* method, float* Fnew) {
+    return guard([&] { propagate_gateway<float>(2, F, dims, X, nullptr, MR, bt, G, dtt, L, u, CG, method, Fnew); });
+}
+int hpup_pendulum_propagate(const double* F, const int64_t* dims, const double* X, const double* OJO,
+                            const double* MR, const double* bt, const double* G, double dtt, double L,
+                            const double* u, const double* const* CG, const char* method, double* Fnew) {
+    return guard([&] { propagate_gateway<double>(3, F, dims, X, OJO, MR, bt, G, dtt, L, u, CG, method, Fnew); });
+}
+int hpup_pendulum_propagate32(const float* F, const int64_t* dims, const float* X, const float* OJO,
+                              const float* MR, const float* bt, const float* G, float dtt, float L, const float* u,
+                              const float* const* CG, const char* method, float* Fnew) {
+    return guard([&] { propagate_gateway<float>(3, F, dims, X, OJO, MR, bt, G, dtt, L, u, CG, method, Fnew); });
+}
+
+}  // extern "C"

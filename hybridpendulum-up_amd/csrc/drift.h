@@ -1,0 +1,79 @@
+// Device views of the structured drift operator and the launchers of its kernels.
+#pragma once
+
+#include "hpup_internal.h"
+#include "kernels_common.h"
+
+namespace hpup {
+
+// LDS budget per workgroup used by every kernel launch.
+constexpr size_t kLdsLimit = 64 * 1024;
+
+template <class T> struct DriftDev {
+    int B, dim, N;
+    int64_t nx, nrc;
+    int TG, TU;
+    const int32_t* cg_src;   // nrc*TG
+    const cT<T>* cg_coef;    // nrc*TG*dim
+    const int32_t* u_src;    // 2*nrc*TU
+    const cT<T>* u_coef;     // 2*nrc*TU
+    const cT<T>* e3;         // nrc
+    const cT<T>* kap;        // dim*N circulant kernels
+    const T* kc;             // N: 2 pi k'/L, Nyquist -> 0
+    const T* kd;             // N: -(2 pi k/L)^2
+    T G[9];
+    T b[3];
+    T gyro[3];
+    int has_gyro;
+};
+
+// Integrator epilogue of the slab kernel:
+//   Fout = F0 + dt (ca ACC + cd dF);  ACC <- ea ACC + ed dF
+template <class T> struct StageCoef {
+    T dt, ca, cd, ea, ed;
+    int read_f0, read_acc, write_acc, deriv_only;
+};
+
+template <class T>
+void launch_rotmix(const DriftDev<T>& D, const void* F, void* W, void* V1, void* V2, hipStream_t st);
+template <class T>
+void launch_axis_outer(const void* in, void* out, const void* Mat, int N, int64_t inner, int64_t nlines,
+                       T scale, hipStream_t st);
+template <class T>
+void launch_drift_slab(const DriftDev<T>& D, const void* Fin, const void* Z3, const void* W, const void* V1,
+                       const void* V2, const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc,
+                       hipStream_t st);
+
+// ---------------------------------------------------------- transforms --
+
+template <class T> struct XformDev {
+    int B, dim, N;
+    int64_t nx, nrc;
+    const T* Wf;        // forward beta weights  [mn][l][j] = w_j d^l_mn(beta_j)
+    const T* Wi;        // inverse beta weights  [mn][j][l] = (2l+1) d^l_mn(beta_j)
+    const cT<T>* twB;   // e^{i pi t / B}, t in [0, 2B)
+    const cT<T>* dftN;  // forward DFT matrix N x N (row-major [k][a]) e^{-2 pi i k a/N}
+    const cT<T>* idftN; // inverse DFT matrix (1/N scaled) e^{+2 pi i k a/N}/N
+    const cT<T>* twNf;  // e^{-2 pi i t/N}, t in [0, N)
+    const cT<T>* twNi;  // e^{+2 pi i t/N}
+};
+
+template <class T>
+void launch_fwd_ag(const XformDev<T>& X, const void* f, void* G, hipStream_t st);
+template <class T>
+void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t st);
+template <class T>
+void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t st);
+template <class T>
+void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st);
+// in-place omega DFT of the compact rows (forward sign -1, inverse +1 with 1/nx)
+template <class T>
+void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hipStream_t st);
+
+// layout conversion full (M,M,B,nx) <-> compact [r][x]
+template <class T>
+void launch_full_to_compact(int B, int64_t nx, const void* full, void* compact, hipStream_t st);
+template <class T>
+void launch_compact_to_full(int B, int64_t nx, const void* compact, void* full, hipStream_t st);
+
+}  // namespace hpup

@@ -1,0 +1,135 @@
+// Internal declarations shared by the HIP sources of libhpup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hpup.h"
+
+namespace hpup {
+
+// ------------------------------------------------------------------ errors --
+
+void set_error(const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define HPUP_HIP(call)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::hpup::Error{HPUP_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define HPUP_REQUIRE(cond, code, msg)                  \
+    do {                                               \
+        if (!(cond)) throw ::hpup::Error{(code), (msg)}; \
+    } while (0)
+
+// ------------------------------------------------------------ index maths --
+
+inline int64_t nrc_of(int B) { return (int64_t)B * (2 * B - 1) * (2 * B + 1) / 3; }
+inline int64_t lblock_off(int l) { return (int64_t)l * (2 * l - 1) * (2 * l + 1) / 3; }
+// compact rotation index (reference compact layout, pendulum_propagate_reduced/integrate.cu:141-178)
+inline int64_t ridx(int l, int m, int n) { return lblock_off(l) + (int64_t)(n + l) * (2 * l + 1) + (m + l); }
+inline int64_t ipow64(int64_t a, int e) {
+    int64_t r = 1;
+    while (e-- > 0) r *= a;
+    return r;
+}
+
+// ------------------------------------------------------------ device buf --
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        release();
+        p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0;
+        return *this;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{HPUP_ERR_NOMEM, "hipMalloc(" + std::to_string(n) + " bytes) failed: " + hipGetErrorString(e)};
+        }
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+DevBuf upload(const std::vector<T>& v) {
+    DevBuf b(v.size() * sizeof(T));
+    if (!v.empty()) HPUP_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return b;
+}
+
+// ----------------------------------------------------------- host tables --
+
+// Grid and weights (fftSO3R.m:27-29,63-66), Wigner-d by FP64 recursion (Wigner_d.m:10-64).
+void host_grid(int B, std::vector<double>& beta, std::vector<double>& w);
+void host_wigner_d(double beta, int lmax, double* d);  // (2lmax+1)^2 (lmax+1)
+
+// Structured drift operator, built on the host in FP64 and uploaded per precision.
+struct DriftSpec {
+    int B = 0, dim = 0, N = 0;
+    double L = 1.0;
+    double b[3] = {0, 0, 0};
+    double G[9] = {0};      // dim x dim column-major
+    double gyro[3] = {0, 0, 0};  // ojo_p = gyro_p * x_q * x_r  ({p,q,r} = {1,2,3})
+    bool has_gyro = false;
+    // circulant kernels kappa_p[j] (complex), T_p F[k] = sum_a kappa_p[(k_p - a_p) mod N] F[..a_p..]
+    std::vector<double> kappa;  // 2 * N * dim
+    // Clebsch-Gordan gravity terms per compact output r: up to TG terms (src, coef[dim] complex)
+    int TG = 0;
+    std::vector<int32_t> cg_src;   // nRc * TG, -1 padded
+    std::vector<double> cg_coef;   // nRc * TG * dim * 2
+    // Lie-derivative mixing along n (u_1, u_2): up to TU terms per output
+    int TU = 0;
+    std::vector<int32_t> u_src;    // 2 * nRc * TU (p = 0,1)
+    std::vector<double> u_coef;    // 2 * nRc * TU * 2
+    // diagonal u_3 term: e3[r] = -u_3[n,n]^l (complex), 3-D only
+    std::vector<double> e3;        // nRc * 2
+};
+
+// Builds the structured operator from the gateway arrays (X, OJO, MR, u, CG
+// as the reference's drivers pass them).  Throws HPUP_ERR_UNSUPPORTED if an
+// array falls outside the structure the kernels implement.
+DriftSpec build_drift_spec_from_arrays(int B, int dim, int N, double L, const double* X,
+                                       const double* OJO, const double* MR, const double* bt,
+                                       const double* G, const double* u,
+                                       const double* const* CG, double tol = 1e-9);
+// Builds the same operator from physical parameters (the drivers' recipe).
+DriftSpec build_drift_spec_from_model(const hpup_model& m);
+
+// Clebsch-Gordan cell by the reference's tridiagonal null-vector recipe (clebsch_gordan.m).
+void host_clebsch_gordan(int l1, int l2, std::vector<double>& CG);
+
+// ------------------------------------------------------------------ plan --
+
+struct Plan;
+
+}  // namespace hpup
+
+struct hpup_plan;

@@ -1,0 +1,298 @@
+"""hpup: host-side mirror of the reference's operator interface over libhpup.so.
+
+Two layers, both over the C ABI declared in include/hpup.h:
+
+* gateway functions with the reference MEX names and argument meaning
+  (``fftSO3R_mex``, ``pendulum_propagate_reduced[32]``, ``pendulum_propagate[32]``,
+  ``getLambda_mex``, ``getOmega_mex``, ``getFcL_mex``, ``getIndRule_mex``,
+  ``pendulum_reduced_discrete_propagate``) taking numpy arrays in MATLAB
+  (Fortran) order and returning the array the gateway returns;
+* ``Plan``: device-resident tables and coefficients (torch tensors on the GPU),
+  steps run back to back on the current HIP stream.
+
+The library is required: every entry point raises if libhpup.so is missing or
+no GPU is present.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libhpup.so")
+
+HPUP_FP64, HPUP_FP32, HPUP_BF16X3 = 0, 1, 2
+METHODS = {"euler": 0, "midpoint": 1, "rk2": 2, "rk4": 3}
+PRECISIONS = {"fp64": HPUP_FP64, "fp32": HPUP_FP32, "bf16x3": HPUP_BF16X3}
+
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_I16 = ctypes.POINTER(ctypes.c_int16)
+
+
+class HpupError(RuntimeError):
+    pass
+
+
+class Model(ctypes.Structure):
+    _fields_ = [("B", ctypes.c_int), ("dim", ctypes.c_int), ("N", ctypes.c_int), ("L", ctypes.c_double),
+                ("b", ctypes.c_double * 3), ("G", ctypes.c_double * 9), ("Jt", ctypes.c_double * 9),
+                ("dt", ctypes.c_double), ("method", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+_lib = None
+
+# every symbol of include/hpup.h
+EXPORTS = [
+    "hpup_last_error", "hpup_version", "hpup_parse_method",
+    "hpup_fftSO3R_forward", "hpup_fftSO3R_backward",
+    "hpup_pendulum_propagate_reduced", "hpup_pendulum_propagate_reduced32",
+    "hpup_pendulum_propagate", "hpup_pendulum_propagate32",
+    "hpup_getLambda", "hpup_getOmega", "hpup_getFcL", "hpup_getIndRule",
+    "hpup_discrete_propagate_noise", "hpup_discrete_propagate_nonoise",
+    "hpup_plan_create", "hpup_plan_destroy", "hpup_plan_num_coeffs", "hpup_plan_num_grid",
+    "hpup_plan_forward", "hpup_plan_backward", "hpup_plan_step", "hpup_plan_derivative",
+    "hpup_plan_full_to_compact", "hpup_plan_compact_to_full", "hpup_plan_stat_moments",
+]
+
+
+def lib():
+    """Loads libhpup.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HpupError("libhpup.so is not built at %s (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.hpup_last_error.restype = ctypes.c_char_p
+        L.hpup_plan_num_coeffs.restype = ctypes.c_int64
+        L.hpup_plan_num_grid.restype = ctypes.c_int64
+        L.hpup_plan_create.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(ctypes.c_void_p)]
+        for name in ("hpup_plan_forward", "hpup_plan_backward", "hpup_plan_derivative",
+                     "hpup_plan_full_to_compact", "hpup_plan_compact_to_full"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.hpup_plan_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.hpup_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.hpup_plan_num_coeffs.argtypes = [ctypes.c_void_p]
+        L.hpup_plan_num_grid.argtypes = [ctypes.c_void_p]
+        L.hpup_plan_stat_moments.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _D, _D, _D, _D, ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HpupError("libhpup error %d: %s" % (rc, lib().hpup_last_error().decode()))
+
+
+def _p(a, t=_D):
+    return a.ctypes.data_as(t)
+
+
+def _dims(a):
+    return (ctypes.c_int64 * a.ndim)(*a.shape)
+
+
+def _interleaved(a, dtype):
+    """complex array -> contiguous Fortran-order buffer viewed as interleaved reals."""
+    return np.asfortranarray(a, dtype=dtype)
+
+
+# ------------------------------------------------------------ transforms --
+
+def fftSO3R_mex(direction, *args):
+    """F = fftSO3R_mex('forward', f, d, w) / f = fftSO3R_mex('backward', F, d)."""
+    if direction.lower() == "forward":
+        f, d, w = args
+        f = np.asfortranarray(f, dtype=np.float64)
+        B = f.shape[0] // 2
+        F = np.zeros((2 * B - 1, 2 * B - 1, B) + f.shape[3:], dtype=np.complex128, order="F")
+        _check(lib().hpup_fftSO3R_forward(_p(f), _dims(f), f.ndim, _p(np.asfortranarray(d, dtype=np.float64)),
+                                          _p(np.ascontiguousarray(np.ravel(w), dtype=np.float64)), _p(F)))
+        return F
+    if direction.lower() == "backward":
+        F, d = args
+        F = _interleaved(F, np.complex128)
+        B = F.shape[2]
+        f = np.zeros((2 * B,) * 3 + F.shape[3:], dtype=np.float64, order="F")
+        _check(lib().hpup_fftSO3R_backward(_p(F), _dims(F), F.ndim, _p(np.asfortranarray(d, dtype=np.float64)),
+                                           _p(f)))
+        return f
+    raise ValueError("direction must be 'forward' or 'backward'")
+
+
+def _cg_ptrs(CG, B, dtype, keep):
+    t = _D if dtype == np.float64 else _F
+    arr = (t * (B * B))()
+    for key, C in CG.items():
+        l1, l2 = key
+        a = np.asfortranarray(C, dtype=dtype)
+        keep.append(a)
+        arr[l1 + B * l2] = a.ctypes.data_as(t)
+    return arr
+
+
+def _propagate(fn, dtype, F, X, OJO, MR, bt, G, dtt, L, u, CG, method):
+    cdt = np.complex128 if dtype == np.float64 else np.complex64
+    t = _D if dtype == np.float64 else _F
+    ct = ctypes.c_double if dtype == np.float64 else ctypes.c_float
+    F = _interleaved(F, cdt)
+    B = F.shape[2]
+    keep = []
+    cg = _cg_ptrs(CG, B, dtype, keep)
+    out = np.zeros_like(F, order="F")
+    args = [_p(F, t), _dims(F), _p(_interleaved(X, cdt), t)]
+    if OJO is not None:
+        args.append(_p(_interleaved(OJO, cdt), t))
+    args += [_p(_interleaved(MR, cdt), t), _p(np.ascontiguousarray(np.ravel(bt), dtype=dtype), t),
+             _p(np.asfortranarray(G, dtype=dtype), t), ct(dtt), ct(L), _p(_interleaved(u, cdt), t), cg,
+             method.encode(), _p(out, t)]
+    _check(fn(*args))
+    return out
+
+
+def pendulum_propagate_reduced(F, X, MR, bt, G, dtt, L, u, CG, method):
+    return _propagate(lib().hpup_pendulum_propagate_reduced, np.float64, F, X, None, MR, bt, G, dtt, L, u, CG, method)
+
+
+def pendulum_propagate_reduced32(F, X, MR, bt, G, dtt, L, u, CG, method):
+    return _propagate(lib().hpup_pendulum_propagate_reduced32, np.float32, F, X, None, MR, bt, G, dtt, L, u, CG,
+                      method)
+
+
+def pendulum_propagate(F, X, OJO, MR, bt, G, dtt, L, u, CG, method):
+    return _propagate(lib().hpup_pendulum_propagate, np.float64, F, X, OJO, MR, bt, G, dtt, L, u, CG, method)
+
+
+def pendulum_propagate32(F, X, OJO, MR, bt, G, dtt, L, u, CG, method):
+    return _propagate(lib().hpup_pendulum_propagate32, np.float32, F, X, OJO, MR, bt, G, dtt, L, u, CG, method)
+
+
+# ------------------------------------------------------------------ plan --
+
+class Plan:
+    """Device-resident propagator.  Coefficients are torch tensors on cuda:0
+    in the compact layout [r][k3][k2][k1] (complex128 for fp64, complex64
+    otherwise); the grid is real (2B,2B,2B,N..) in MATLAB order, stored as a
+    flat tensor."""
+
+    def __init__(self, B, dim, N=1, L=1.0, b=(0, 0, 0), G=None, Jt=None, dt=0.0, method="euler",
+                 precision="fp64"):
+        import torch
+        if not torch.cuda.is_available():
+            raise HpupError("Plan needs a GPU")
+        m = Model()
+        m.B, m.dim, m.N, m.L, m.dt = B, dim, N if dim else 1, L, dt
+        m.method = METHODS[method.lower()]
+        m.precision = PRECISIONS[precision.lower()]
+        for i, v in enumerate(b):
+            m.b[i] = v
+        if G is not None:
+            Gm = np.asarray(G, dtype=np.float64)
+            for j in range(dim):
+                for i in range(dim):
+                    m.G[i + dim * j] = Gm[i, j]
+        Jm = np.eye(3) if Jt is None else np.asarray(Jt, dtype=np.float64)
+        for j in range(3):
+            for i in range(3):
+                m.Jt[i + 3 * j] = Jm[i, j]
+        self.model = m
+        self.precision = precision.lower()
+        self.B, self.dim, self.N = B, dim, (N if dim else 1)
+        self.nx = self.N ** dim
+        self.nrc = B * (2 * B - 1) * (2 * B + 1) // 3
+        h = ctypes.c_void_p()
+        _check(lib().hpup_plan_create(ctypes.byref(m), ctypes.byref(h)))
+        self._h = h
+        self.ncoef = lib().hpup_plan_num_coeffs(h)
+        self.ngrid = lib().hpup_plan_num_grid(h)
+        self.cdtype = torch.complex128 if self.precision == "fp64" else torch.complex64
+        self.rdtype = torch.float64 if self.precision == "fp64" else torch.float32
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.hpup_plan_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def _stream():
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def _chk(self, t, n, dtype):
+        import torch
+        if not (t.is_cuda and t.is_contiguous() and t.numel() == n and t.dtype == dtype):
+            raise HpupError("tensor must be a contiguous CUDA %s with %d elements" % (dtype, n))
+        return ctypes.c_void_p(t.data_ptr())
+
+    def empty_coeffs(self):
+        import torch
+        return torch.empty(self.ncoef, dtype=self.cdtype, device="cuda")
+
+    def empty_grid(self):
+        import torch
+        return torch.empty(self.ngrid, dtype=self.rdtype, device="cuda")
+
+    def forward(self, f, F=None):
+        F = self.empty_coeffs() if F is None else F
+        _check(lib().hpup_plan_forward(self._h, self._chk(f, self.ngrid, self.rdtype),
+                                       self._chk(F, self.ncoef, self.cdtype), self._stream()))
+        return F
+
+    def backward(self, F, f=None):
+        f = self.empty_grid() if f is None else f
+        _check(lib().hpup_plan_backward(self._h, self._chk(F, self.ncoef, self.cdtype),
+                                        self._chk(f, self.ngrid, self.rdtype), self._stream()))
+        return f
+
+    def step(self, F, nsteps=1):
+        _check(lib().hpup_plan_step(self._h, self._chk(F, self.ncoef, self.cdtype), int(nsteps), self._stream()))
+        return F
+
+    def derivative(self, F, dF=None):
+        dF = self.empty_coeffs() if dF is None else dF
+        _check(lib().hpup_plan_derivative(self._h, self._chk(F, self.ncoef, self.cdtype),
+                                          self._chk(dF, self.ncoef, self.cdtype), self._stream()))
+        return dF
+
+    @property
+    def nfull(self):
+        return (2 * self.B - 1) ** 2 * self.B * self.nx
+
+    def full_to_compact(self, Ffull, F=None):
+        F = self.empty_coeffs() if F is None else F
+        _check(lib().hpup_plan_full_to_compact(self._h, self._chk(Ffull, self.nfull, self.cdtype),
+                                               self._chk(F, self.ncoef, self.cdtype), self._stream()))
+        return F
+
+    def compact_to_full(self, F, Ffull=None):
+        import torch
+        Ffull = torch.empty(self.nfull, dtype=self.cdtype, device="cuda") if Ffull is None else Ffull
+        _check(lib().hpup_plan_compact_to_full(self._h, self._chk(F, self.ncoef, self.cdtype),
+                                               self._chk(Ffull, self.nfull, self.cdtype), self._stream()))
+        return Ffull
+
+    def stat_moments(self, f):
+        ER = np.zeros(9)
+        Ex = np.zeros(max(self.dim, 1))
+        Varx = np.zeros(max(self.dim, 1) ** 2)
+        fx = np.zeros(self.nx)
+        _check(lib().hpup_plan_stat_moments(self._h, self._chk(f, self.ngrid, self.rdtype), _p(ER), _p(Ex),
+                                            _p(Varx), _p(fx), self._stream()))
+        d = max(self.dim, 1)
+        return (ER.reshape((3, 3), order="F"), Ex[: self.dim], Varx.reshape((d, d), order="F")[: self.dim, : self.dim],
+                fx.reshape((self.N,) * self.dim, order="F") if self.dim else fx)
+
+    # host conveniences (MATLAB full layout numpy arrays)
+    def upload_full(self, Ffull_np):
+        import torch
+        a = np.asfortranarray(Ffull_np).ravel(order="F")
+        t = torch.from_numpy(a.astype(np.complex128 if self.precision == "fp64" else np.complex64)).cuda()
+        return self.full_to_compact(t)
+
+    def download_full(self, F):
+        full = self.compact_to_full(F).cpu().numpy()
+        M = 2 * self.B - 1
+        return full.reshape((M, M, self.B) + (self.N,) * self.dim, order="F")

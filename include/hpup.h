@@ -1,0 +1,208 @@
+/*
+ * hpup.h -- C ABI of the MI355X-native spectral Fokker-Planck propagator on
+ * SO(3) x R^n (libhpup.so).
+ *
+ * Two groups of entry points:
+ *
+ *  1. Gateway equivalents.  One function per MEX gateway of the reference,
+ *     taking the same arrays the gateway's prhs[] carry (MATLAB column-major,
+ *     complex interleaved re,im as mxGetComplexDoubles/Singles returns them),
+ *     plain pointers and explicit dims, writing the array the gateway would
+ *     return in plhs[0] into a caller-allocated buffer.  Host memory in, host
+ *     memory out, synchronous, no device state kept between calls -- exactly
+ *     the reference's calling convention, so a mexFunction shim is a few lines
+ *     (see INTEGRATION.md).  Precision is selected by the name (`...32` takes
+ *     float arrays) as in the reference.
+ *
+ *  2. Device-resident plan API.  Tables are built once, coefficients stay in
+ *     HBM in the compact layout, and any number of steps run back to back on
+ *     a caller-supplied HIP stream.  This is what a time loop should call.
+ *
+ * All functions return HPUP_OK (0) or a negative hpup_status; the message of
+ * the last failure on the calling thread is returned by hpup_last_error().
+ */
+#ifndef HPUP_H
+#define HPUP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    HPUP_OK = 0,
+    HPUP_ERR_ARG = -1,          /* bad shape, rank or value */
+    HPUP_ERR_UNSUPPORTED = -2,  /* operator outside the structured support (see message) */
+    HPUP_ERR_DEVICE = -3,       /* HIP runtime error */
+    HPUP_ERR_NOMEM = -4,        /* device allocation failed */
+    HPUP_ERR_METHOD = -5        /* unknown integrator name */
+} hpup_status;
+
+typedef enum {
+    HPUP_FP64 = 0,    /* complex128 storage, FP64 arithmetic */
+    HPUP_FP32 = 1,    /* complex64 storage, FP32 MFMA drift convolutions */
+    HPUP_BF16X3 = 2   /* complex64 storage, 3xBF16 split-product MFMA convolutions, FP32 accumulate */
+} hpup_precision;
+
+typedef enum { HPUP_EULER = 0, HPUP_MIDPOINT = 1, HPUP_RK2 = 2, HPUP_RK4 = 3 } hpup_method;
+
+const char* hpup_last_error(void);
+int hpup_version(void);
+/* Parses "euler" | "midpoint" | "RK2" | "RK4" (case-insensitive); -1 if unknown. */
+int hpup_parse_method(const char* method);
+
+/* ---------------------------------------------------------------------------
+ * 1. Gateway equivalents (host buffers).
+ * ------------------------------------------------------------------------- */
+
+/* F = fftSO3R_mex('forward', f, d, w)
+ * Replaces propagation/mex/sources/fftSO3R/kernel.cu:11-195 (mexFunction,
+ * 'forward' branch).  f real (2B,2B,2B[,N,N[,N]]), d real (2B-1,2B-1,B,2B),
+ * w real (2B); dims_f/ndims describe f (ndims 3, 5 or 6).  F complex
+ * (2B-1,2B-1,B[,N,...]) interleaved; entries with max(|m|,|n|) > l are set 0. */
+int hpup_fftSO3R_forward(const double* f, const int64_t* dims_f, int ndims, const double* d,
+                         const double* w, double* F);
+
+/* f = fftSO3R_mex('backward', F, d)
+ * Replaces propagation/mex/sources/fftSO3R/kernel.cu:196-391 ('backward'
+ * branch).  dims_F/ndims describe F (ndims 3, 5 or 6). */
+int hpup_fftSO3R_backward(const double* F, const int64_t* dims_F, int ndims, const double* d,
+                          double* f);
+
+/* Fnew = pendulum_propagate_reduced(F, X, MR, bt, G, dtt, L, u, CG, method)
+ * Replaces propagation/mex/sources/pendulum_propagate_reduced/kernel.cu:9-262.
+ * F complex (2B-1,2B-1,B,N,N); X complex (N,N,2); MR complex (2B-1,2B-1,B,2);
+ * bt real(2); G real 2x2; u complex (2B-1,2B-1,B,3); CG: B*B pointers to the
+ * cells CG{l1+1,l2+1} at index l1 + l2*B (only l2 = 1 is read, see DESIGN).
+ * Fnew same shape as F, entries with max(|m|,|n|) > l set 0. */
+int hpup_pendulum_propagate_reduced(const double* F, const int64_t* dims_F, const double* X,
+                                    const double* MR, const double* bt, const double* G,
+                                    double dtt, double L, const double* u,
+                                    const double* const* CG, const char* method, double* Fnew);
+/* pendulum_propagate_reduced32: the same in single precision (FP32 MFMA path). */
+int hpup_pendulum_propagate_reduced32(const float* F, const int64_t* dims_F, const float* X,
+                                      const float* MR, const float* bt, const float* G,
+                                      float dtt, float L, const float* u,
+                                      const float* const* CG, const char* method, float* Fnew);
+
+/* Fnew = pendulum_propagate(F, X, OJO, MR, bt, G, dtt, L, u, CG, method)
+ * Replaces propagation/mex/sources/pendulum_propagate/kernel.cu:9-519 (both
+ * its small and large paths; damping and noise are always applied).
+ * F complex (2B-1,2B-1,B,N,N,N); X, OJO complex (N,N,N,3); MR (...,3); bt(3);
+ * G 3x3. */
+int hpup_pendulum_propagate(const double* F, const int64_t* dims_F, const double* X,
+                            const double* OJO, const double* MR, const double* bt,
+                            const double* G, double dtt, double L, const double* u,
+                            const double* const* CG, const char* method, double* Fnew);
+int hpup_pendulum_propagate32(const float* F, const int64_t* dims_F, const float* X,
+                              const float* OJO, const float* MR, const float* bt,
+                              const float* G, float dtt, float L, const float* u,
+                              const float* const* CG, const char* method, float* Fnew);
+
+/* ---------------------------------------------------------------------------
+ * Hybrid reset gateways (reduced model, 2-D omega grid).  Index arrays are
+ * 0-based as the reference's C code produces and consumes them.
+ * ------------------------------------------------------------------------- */
+
+/* [lambda, lambda_indR, PC] = getLambda_mex(R, x, d, h, r, theta_t, lambda_max, false)
+ * Replaces propagation/discrete/mex/sources/pendulum_reduced_collision/src/getLambda.cu:6-92
+ * (gateway mexLambda.cpp).  R real (3,3,2B,2B,2B); x real (2,N,N).  Outputs:
+ * lambda (numR, N, N), indR (numR) int32, PC real (3, nR).  lambda and indR
+ * must hold nR*N*N and nR entries; *numR receives the collision-region size. */
+int hpup_getLambda(const double* R, int B, const double* x, int N, double d, double h, double r,
+                   double theta_t, double lambda_max, double* lambda, int32_t* indR, int* numR,
+                   double* PC);
+
+/* Omega = getOmega_mex(R, x, indR, epsilon, PC, 'new'|'old')
+ * Replaces .../getOmega.cu:6-137 (gateway mexOmega.cpp).  Omega real (2, numR, N, N). */
+int hpup_getOmega(const double* R, int B, const double* x, int N, const int32_t* indR, int numR,
+                  double epsilon, const double* PC, const char* direction, double* Omega);
+
+/* [fcL, fcL_indx] = getFcL_mex(x, Omega, lambda, Gd, nD)
+ * Replaces .../getFcL.cu:11-166 (gateway mexFcL.cpp).  fcL real, fcL_indx int16,
+ * both (nD, numR, N, N). */
+int hpup_getFcL(const double* x, int N, const double* Omega, const double* lambda, int numR,
+                const double* Gd, int nD, double* fcL, int16_t* fcL_indx);
+
+/* [ind_interp, coeff_interp] = getIndRule_mex(x, Omega_old)
+ * Replaces .../getIndRule.cu:6-90 (gateway mexIndRule.cpp); (4, numR, N, N). */
+int hpup_getIndRule(const double* x, int N, const double* Omega, int numR, int32_t* ind_interp,
+                    double* coeff_interp);
+
+/* df = pendulum_reduced_discrete_propagate(true, f, lambda, lambda_indR, fcL, fcL_indx)
+ * Replaces .../propagate.cu:6-121 (gateway mexPropagate.cpp, noise branch).
+ * f, df real (2B,2B,2B,N,N). */
+int hpup_discrete_propagate_noise(const double* f, int B, int N, const double* lambda,
+                                  const int32_t* indR, int numR, const double* fcL,
+                                  const int16_t* fcL_indx, int nD, double* df);
+
+/* df = pendulum_reduced_discrete_propagate(false, f, lambda, lambda_indR, lambda_indx,
+ *                                          ind_interp, coeff_interp)
+ * Replaces .../propagate.cu:123-229 (gateway mexPropagate.cpp, no-noise branch).
+ * lambda (numR); lambda_indx: numR pointers to int32 lists of lengths lambda_numx. */
+int hpup_discrete_propagate_nonoise(const double* f, int B, int N, const double* lambda,
+                                    const int32_t* indR, int numR,
+                                    const int32_t* const* lambda_indx, const int32_t* lambda_numx,
+                                    const int32_t* ind_interp, const double* coeff_interp,
+                                    double* df);
+
+/* ---------------------------------------------------------------------------
+ * 2. Device-resident plan API.
+ * ------------------------------------------------------------------------- */
+
+typedef struct hpup_plan hpup_plan;
+
+typedef struct {
+    int B;           /* rotation bandlimit */
+    int dim;         /* omega dimension: 0 (pure SO(3)), 2 (reduced) or 3 */
+    int N;           /* omega grid points per axis (2*Bx) */
+    double L;        /* omega box length (periodic on [-L/2, L/2)) */
+    double b[3];     /* damping bt */
+    double G[9];     /* noise matrix, dim x dim column-major (leading dim = dim) */
+    double Jt[9];    /* scaled inertia (3-D), column-major 3x3; ignored for dim 2 */
+    double dt;       /* time step */
+    int method;      /* hpup_method */
+    int precision;   /* hpup_precision */
+} hpup_model;
+
+/* Builds every table (Wigner-d by FP64 recursion, weights, X/OJO/MR operator
+ * structure, Clebsch-Gordan terms) and allocates device work space. */
+int hpup_plan_create(const hpup_model* model, hpup_plan** plan);
+int hpup_plan_destroy(hpup_plan* plan);
+
+/* Number of compact coefficients: nRc * nx, nRc = B(2B-1)(2B+1)/3. */
+int64_t hpup_plan_num_coeffs(const hpup_plan* plan);
+/* Number of grid points: (2B)^3 * nx. */
+int64_t hpup_plan_num_grid(const hpup_plan* plan);
+
+/* Device coefficient layout: compact rotation index r (l-block offset
+ * l(2l-1)(2l+1)/3, then (n+l)(2l+1) + (m+l)) outermost, omega index
+ * k1 + N(k2 + N k3) innermost; complex128 (FP64) or complex64 (FP32, BF16X3).
+ * Grid layout: MATLAB (alpha, beta, gamma, x) real, same precision. */
+
+/* Forward transform: grid f_dev -> coefficients F_dev (device pointers). */
+int hpup_plan_forward(hpup_plan* plan, const void* f_dev, void* F_dev, void* stream);
+/* Inverse transform: coefficients F_dev -> grid f_dev. */
+int hpup_plan_backward(hpup_plan* plan, const void* F_dev, void* f_dev, void* stream);
+/* nsteps integrator steps in place on F_dev. */
+int hpup_plan_step(hpup_plan* plan, void* F_dev, int nsteps, void* stream);
+/* One derivative evaluation dF = derivative(F) (both device, compact). */
+int hpup_plan_derivative(hpup_plan* plan, const void* F_dev, void* dF_dev, void* stream);
+
+/* Layout conversion between the MATLAB full layout (2B-1,2B-1,B,nx) and the
+ * device compact layout; both device pointers, plan precision. */
+int hpup_plan_full_to_compact(hpup_plan* plan, const void* Ffull_dev, void* F_dev, void* stream);
+int hpup_plan_compact_to_full(hpup_plan* plan, const void* F_dev, void* Ffull_dev, void* stream);
+
+/* Grid statistics (get_stat moments): rotation mean E[R] (3x3), omega mean
+ * (dim), omega covariance (dim x dim), omega marginal fx (nx) -- all FP64
+ * host outputs, from the device grid f_dev. */
+int hpup_plan_stat_moments(hpup_plan* plan, const void* f_dev, double* ER, double* Ex,
+                           double* Varx, double* fx, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPUP_H */

@@ -1,0 +1,239 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so built from refcpu.c).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, the smoke entry and bench.py's
+cpu_baseline leg to check or time the HIP path; never by the product.
+
+Array conventions follow refcpu.h: numpy arrays in Fortran (MATLAB) order,
+complex128 for complex data.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_S = ctypes.POINTER(ctypes.c_short)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        L.ref_clebsch_gordan.restype = ctypes.c_int
+        L.ref_derivative.restype = ctypes.c_int
+        L.ref_integrate.restype = ctypes.c_int
+        L.ref_getLambda.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_D):
+    return a.ctypes.data_as(t)
+
+
+def _f64(shape, cplx=False):
+    return np.zeros(shape, dtype=np.complex128 if cplx else np.float64, order="F")
+
+
+def grid(B):
+    al, be, ga, w = (np.zeros(2 * B) for _ in range(4))
+    lib().ref_grid(B, _p(al), _p(be), _p(ga), _p(w))
+    return al, be, ga, w
+
+
+def rotations(B):
+    R = _f64((3, 3, 2 * B, 2 * B, 2 * B))
+    lib().ref_rotations(B, _p(R))
+    return R
+
+
+def wigner_d(beta, lmax):
+    M = 2 * lmax + 1
+    d = _f64((M, M, lmax + 1))
+    lib().ref_wigner_d(ctypes.c_double(beta), lmax, _p(d))
+    return d
+
+
+def wigner_d_jacobi(beta, l, m, n):
+    out = ctypes.c_double(0.0)
+    lib().ref_wigner_d_jacobi(ctypes.c_double(beta), l, m, n, ctypes.byref(out))
+    return out.value
+
+
+def wigner_table(B):
+    M = 2 * B - 1
+    d = _f64((M, M, B, 2 * B))
+    lib().ref_wigner_table(B, _p(d))
+    return d
+
+
+def clebsch_gordan(l1, l2):
+    n = (2 * l1 + 1) * (2 * l2 + 1)
+    C = _f64((n, n))
+    lib().ref_clebsch_gordan(l1, l2, _p(C))
+    return C
+
+
+def cg_table(B):
+    """cell CG{l1+1,l2+1} as a dict keyed (l1, l2), plus a pointer array."""
+    cells = {}
+    for l2 in range(B):
+        for l1 in range(B):
+            cells[(l1, l2)] = clebsch_gordan(l1, l2)
+    return cells
+
+
+def _cg_ptrs(cells, B):
+    arr = (_D * (B * B))()
+    for (l1, l2), C in cells.items():
+        arr[l1 + B * l2] = _p(C)
+    return arr
+
+
+def getu(lmax):
+    M = 2 * lmax + 1
+    u = _f64((M, M, lmax + 1, 3), cplx=True)
+    lib().ref_getu(lmax, _p(u))
+    return u
+
+
+def fft_forward(B, f, d, w, dim, N):
+    M = 2 * B - 1
+    f = np.asfortranarray(f, dtype=np.float64)
+    F = _f64((M, M, B) + (N,) * dim, cplx=True)
+    lib().ref_fftSO3R_forward(B, dim, N, _p(f), _p(np.asfortranarray(d)), _p(np.ascontiguousarray(w)), _p(F))
+    return F
+
+
+def fft_backward(B, F, d, dim, N):
+    F = np.asfortranarray(F, dtype=np.complex128)
+    f = _f64((2 * B,) * 3 + (N,) * dim)
+    lib().ref_fftSO3R_backward(B, dim, N, _p(F), _p(np.asfortranarray(d)), _p(f))
+    return f
+
+
+def build_X(dim, N, L):
+    X = _f64((N,) * dim + (dim,), cplx=True)
+    lib().ref_build_X(dim, N, ctypes.c_double(L), _p(X))
+    return X
+
+
+def build_OJO(N, L, Jt):
+    O = _f64((N, N, N, 3), cplx=True)
+    lib().ref_build_OJO(N, ctypes.c_double(L), _p(np.asfortranarray(Jt, dtype=np.float64)), _p(O))
+    return O
+
+
+def build_MR(B, dim, Jt=None):
+    M = 2 * B - 1
+    MR = _f64((M, M, B, dim), cplx=True)
+    J = np.asfortranarray(Jt if Jt is not None else np.eye(3), dtype=np.float64)
+    lib().ref_build_MR(B, dim, _p(J), _p(MR))
+    return MR
+
+
+def derivative(B, dim, N, F, X, OJO, MR, b, G, L, u, cells, x0=0, x1=0):
+    F = np.asfortranarray(F, dtype=np.complex128)
+    dF = np.zeros_like(F, order="F")
+    ptrs = _cg_ptrs(cells, B)
+    O = _p(np.asfortranarray(OJO)) if OJO is not None else _D()
+    lib().ref_derivative(B, dim, N, _p(F), _p(np.asfortranarray(X)), O, _p(np.asfortranarray(MR)),
+                         _p(np.ascontiguousarray(b, dtype=np.float64)),
+                         _p(np.asfortranarray(G, dtype=np.float64)), ctypes.c_double(L),
+                         _p(np.asfortranarray(u)), ptrs, _p(dF), ctypes.c_long(x0), ctypes.c_long(x1))
+    return dF
+
+
+def integrate(B, dim, N, F, X, OJO, MR, b, G, dt, L, u, cells, method):
+    F = np.asfortranarray(F, dtype=np.complex128)
+    Fn = np.zeros_like(F, order="F")
+    ptrs = _cg_ptrs(cells, B)
+    O = _p(np.asfortranarray(OJO)) if OJO is not None else _D()
+    rc = lib().ref_integrate(B, dim, N, _p(F), _p(np.asfortranarray(X)), O, _p(np.asfortranarray(MR)),
+                             _p(np.ascontiguousarray(b, dtype=np.float64)),
+                             _p(np.asfortranarray(G, dtype=np.float64)), ctypes.c_double(dt),
+                             ctypes.c_double(L), _p(np.asfortranarray(u)), ptrs, method.encode(), _p(Fn))
+    if rc != 0:
+        raise ValueError("unknown method %r" % method)
+    return Fn
+
+
+def getLambda(B, N, R, x, dwall, h, r, theta_t, lambda_max):
+    nR, nx = 8 * B ** 3, N * N
+    lam = np.zeros(nR * nx)
+    indR = np.zeros(nR, dtype=np.int32)
+    PC = _f64((3, nR))
+    numR = lib().ref_getLambda(B, N, _p(np.asfortranarray(R)), _p(np.asfortranarray(x)),
+                               *(ctypes.c_double(v) for v in (dwall, h, r, theta_t, lambda_max)),
+                               _p(lam), _p(indR, _I), _p(PC))
+    return lam[: numR * nx].reshape((numR, N, N), order="F"), indR[:numR].copy(), PC
+
+
+def getOmega(B, N, R, x, indR, epsilon, PC, direction):
+    numR = len(indR)
+    Om = _f64((2, numR, N, N))
+    lib().ref_getOmega(B, N, _p(np.asfortranarray(R)), _p(np.asfortranarray(x)),
+                       _p(np.ascontiguousarray(indR, dtype=np.int32), _I), numR,
+                       ctypes.c_double(epsilon), _p(np.asfortranarray(PC)),
+                       1 if direction.lower() == "old" else 0, _p(Om))
+    return Om
+
+
+def getFcL(N, x, Omega, lam, Gd, nD):
+    numR = Omega.shape[1]
+    fcL = _f64((nD, numR, N, N))
+    idx = np.zeros((nD, numR, N, N), dtype=np.int16, order="F")
+    lib().ref_getFcL(N, _p(np.asfortranarray(x)), _p(np.asfortranarray(Omega)),
+                     _p(np.asfortranarray(lam)), numR, _p(np.asfortranarray(Gd, dtype=np.float64)),
+                     nD, _p(fcL), _p(idx, _S))
+    return fcL, idx
+
+
+def getIndRule(N, x, Omega):
+    numR = Omega.shape[1]
+    ind = np.zeros((4, numR, N, N), dtype=np.int32, order="F")
+    coeff = _f64((4, numR, N, N))
+    lib().ref_getIndRule(N, _p(np.asfortranarray(x)), _p(np.asfortranarray(Omega)), numR,
+                         _p(ind, _I), _p(coeff))
+    return ind, coeff
+
+
+def discrete_noise(B, N, f, lam, indR, fcL, fcL_indx):
+    df = np.zeros_like(np.asfortranarray(f))
+    nD = fcL.shape[0]
+    lib().ref_discrete_noise(B, N, _p(np.asfortranarray(f)), _p(np.asfortranarray(lam)),
+                             _p(np.ascontiguousarray(indR, dtype=np.int32), _I), len(indR),
+                             _p(np.asfortranarray(fcL)), _p(np.asfortranarray(fcL_indx, dtype=np.int16), _S),
+                             nD, _p(df))
+    return df
+
+
+def discrete_nonoise(B, N, f, lam, indR, lambda_indx, ind_interp, coeff_interp):
+    df = np.zeros_like(np.asfortranarray(f))
+    numR = len(indR)
+    keep = [np.ascontiguousarray(a, dtype=np.int32) for a in lambda_indx]
+    ptrs = (_I * numR)(*[_p(a, _I) for a in keep])
+    nums = np.array([len(a) for a in keep], dtype=np.int32)
+    lib().ref_discrete_nonoise(B, N, _p(np.asfortranarray(f)), _p(np.ascontiguousarray(lam, dtype=np.float64)),
+                               _p(np.ascontiguousarray(indR, dtype=np.int32), _I), numR, ptrs,
+                               _p(nums, _I), _p(np.asfortranarray(ind_interp, dtype=np.int32), _I),
+                               _p(np.asfortranarray(coeff_interp)), _p(df))
+    return df
+
+
+def stat_moments(B, dim, N, L, f, R, x, w):
+    ER = np.zeros((3, 3), order="F")
+    Ex = np.zeros(dim)
+    Varx = np.zeros((dim, dim), order="F")
+    fx = np.zeros((N,) * dim, order="F")
+    lib().ref_stat_moments(B, dim, N, ctypes.c_double(L), _p(np.asfortranarray(f)),
+                           _p(np.asfortranarray(R)), _p(np.asfortranarray(x)),
+                           _p(np.ascontiguousarray(w)), _p(ER), _p(Ex), _p(Varx), _p(fx))
+    return ER, Ex, Varx, fx

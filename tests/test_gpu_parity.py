@@ -1,0 +1,128 @@
+"""HIP path vs the FP64 oracle through the C ABI, at sizes the oracle runs in
+seconds.  Tolerances: 1e-10 relative L2 for the FP64 path (the reference's
+stated parity bar), 2e-5 for FP32."""
+import numpy as np
+import pytest
+
+import hpup
+import oracle as O
+from helpers import density, operators, rel_l2, valid_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("B,dim,N", [(4, 0, 1), (5, 2, 4), (3, 3, 4), (10, 0, 1)])
+def test_fft_forward_backward(B, dim, N):
+    _print("case fft", B, dim, N)
+    L = 6.0
+    f = density(B, dim, N, L, (6, 3, 1), seed=B)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    F_ref = O.fft_forward(B, f, d, w, dim, N)
+    F = hpup.fftSO3R_mex("forward", f, d, w)
+    assert F.shape == F_ref.shape
+    assert rel_l2(F, F_ref) < 1e-12
+    f_ref = O.fft_backward(B, F_ref, d, dim, N)
+    f2 = hpup.fftSO3R_mex("backward", F_ref, d)
+    assert rel_l2(f2, f_ref) < 1e-12
+
+
+def _reduced_case(B=4, N=8, L=8.0):
+    f = density(B, 2, N, L, (8, 4, 2), seed=3, Sigma=np.eye(2) * 1.0)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    F = O.fft_forward(B, f, d, w, 2, N)
+    X, _, MR, u, cells = operators(B, 2, N, L)
+    bt = np.array([0.2, 0.2])
+    G = np.diag([0.1, 0.1])
+    return F, X, MR, bt, G, u, cells, L
+
+
+@pytest.mark.parametrize("method", ["euler", "midpoint", "RK2", "RK4"])
+def test_propagate_reduced_fp64(method):
+    _print("case reduced fp64", method)
+    F, X, MR, bt, G, u, cells, L = _reduced_case()
+    B, N, dt = F.shape[2], F.shape[3], 0.01
+    ref = O.integrate(B, 2, N, F, X, None, MR, bt, G, dt, L, u, cells, method)
+    out = hpup.pendulum_propagate_reduced(F, X, MR, bt, G, dt, L, u, cells, method)
+    assert rel_l2(out, ref) < 1e-10
+    # invalid-triangle entries are zero
+    assert np.all(out[~np.broadcast_to(valid_mask(B, (N, N)), out.shape)] == 0)
+
+
+def test_propagate_reduced_fp32():
+    _print("case reduced fp32")
+    F, X, MR, bt, G, u, cells, L = _reduced_case()
+    B, N, dt = F.shape[2], F.shape[3], 0.01
+    ref = O.integrate(B, 2, N, F, X, None, MR, bt, G, dt, L, u, cells, "euler")
+    cells32 = {k: v.astype(np.float32) for k, v in cells.items()}
+    out = hpup.pendulum_propagate_reduced32(F.astype(np.complex64), X.astype(np.complex64), MR.astype(np.complex64),
+                                            bt.astype(np.float32), G.astype(np.float32), dt, L,
+                                            u.astype(np.complex64), cells32, "euler")
+    assert rel_l2(out, ref) < 2e-5
+
+
+def _full_case(B=3, N=6, L=10.0):
+    Sigma = np.array([[0.8, 0.1, 0.0], [0.1, 0.6, 0.05], [0.0, 0.05, 1.0]])
+    f = density(B, 3, N, L, (10, 5, 1), seed=5, Sigma=Sigma)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    F = O.fft_forward(B, f, d, w, 3, N)
+    Jt = np.diag([1.0, 0.93539, 0.24935])
+    X, OJO, MR, u, cells = operators(B, 3, N, L, Jt)
+    bt = np.array([0.2, 0.2, 0.2])
+    G = np.eye(3) * 0.1
+    return F, X, OJO, MR, bt, G, u, cells, L
+
+
+@pytest.mark.parametrize("method", ["euler", "RK4"])
+def test_propagate_3d_fp64(method):
+    _print("case 3d fp64", method)
+    F, X, OJO, MR, bt, G, u, cells, L = _full_case()
+    B, N, dt = F.shape[2], F.shape[3], 0.004
+    ref = O.integrate(B, 3, N, F, X, OJO, MR, bt, G, dt, L, u, cells, method)
+    out = hpup.pendulum_propagate(F, X, OJO, MR, bt, G, dt, L, u, cells, method)
+    assert rel_l2(out, ref) < 1e-10
+
+
+def test_derivative_3d_fp64_each_term():
+    """dF itself (dt = 1 Euler minus F) against the oracle, including a case with
+    only one term switched on at a time (damping, gyroscopic, gravity, noise)."""
+    _print("case 3d derivative terms")
+    F, X, OJO, MR, bt, G, u, cells, L = _full_case(B=4, N=4)
+    B, N = F.shape[2], F.shape[3]
+    zeros = np.zeros_like
+    cases = {
+        "all": (X, OJO, MR, bt, G),
+        "kinematic": (X, zeros(OJO), zeros(MR), zeros(bt), zeros(G)),
+        "damping": (X, zeros(OJO), zeros(MR), bt, zeros(G)),
+        "gyro": (X, OJO, zeros(MR), zeros(bt), zeros(G)),
+        "gravity": (X, zeros(OJO), MR, zeros(bt), zeros(G)),
+        "noise": (X, zeros(OJO), zeros(MR), zeros(bt), np.array([[0.1, 0.02, 0.0], [0.02, 0.2, 0.01], [0.0, 0.01, 0.3]])),
+    }
+    for name, (Xc, Oc, Mc, bc, Gc) in cases.items():
+        ref = O.derivative(B, 3, N, F, Xc, Oc, Mc, bc, Gc, L, u, cells)
+        out = hpup.pendulum_propagate(F, Xc, Oc, Mc, bc, Gc, 1.0, L, u, cells, "euler") - F
+        assert rel_l2(out, ref) < 1e-10, name
+
+
+def test_propagate_3d_fp32():
+    _print("case 3d fp32")
+    F, X, OJO, MR, bt, G, u, cells, L = _full_case()
+    B, N, dt = F.shape[2], F.shape[3], 0.004
+    ref = O.integrate(B, 3, N, F, X, OJO, MR, bt, G, dt, L, u, cells, "euler")
+    c = lambda a: a.astype(np.complex64)
+    cells32 = {k: v.astype(np.float32) for k, v in cells.items()}
+    out = hpup.pendulum_propagate32(c(F), c(X), c(OJO), c(MR), bt.astype(np.float32), G.astype(np.float32), dt, L,
+                                    c(u), cells32, "euler")
+    assert rel_l2(out, ref) < 2e-5
+
+
+def test_bad_method_is_an_error():
+    F, X, MR, bt, G, u, cells, L = _reduced_case(B=2, N=4)
+    with pytest.raises(hpup.HpupError, match="method"):
+        hpup.pendulum_propagate_reduced(F, X, MR, bt, G, 0.01, L, u, cells, "leapfrog")
