@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Propagation steps/s of the spectral Fokker-Planck step on the 3-D pendulum,
+SO(3) x R^3 with bandlimit B = 15 (30^3 rotation grid) and a 30^3 omega grid
+over [-5, 5]^3, one explicit Euler step = one derivative evaluation plus the
+update, coefficients resident in HBM.
+
+The mixed-precision path (default --precision fp32) is timed next to the plain
+FP64 path on the same initial coefficients; its relative L2 error against the
+FP64 coefficients after the same number of steps is reported.  Prints one JSON
+line (the bench contract).  Single GPU: --gpus N > 1 exits with a message.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "hybridpendulum-up_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+CFG = dict(B=15, dim=3, N=30, L=10.0, b=(0.2, 0.2, 0.2), G=0.1, Jt=(1.0, 0.93539, 0.24935), dt=0.004,
+           s=(10.0, 5.0, 1.0), seed=2026)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def initial_coeffs(plan, cfg):
+    """Matrix Fisher (seeded R0 diag(10,5,1)) x Gaussian with a seeded SPD
+    covariance (eigenvalues in [0.3, 1.5]), forward-transformed on the GPU."""
+    import numpy as np
+    import torch
+    from hpup import density as D
+    rng = np.random.default_rng(cfg["seed"])
+    R0 = D.random_rotation(rng)
+    Q = D.random_rotation(rng)
+    Sigma = Q @ np.diag(rng.uniform(0.3, 1.5, 3)) @ Q.T
+    fR, fx = D.matrix_fisher_gaussian(cfg["B"], cfg["dim"], cfg["N"], cfg["L"], cfg["s"], R0, Sigma=Sigma)
+    f = D.grid_density(fR, fx)
+    F = plan.forward(f)
+    torch.cuda.synchronize()
+    del f
+    return F
+
+
+def make_plan(cfg, precision):
+    import numpy as np
+    import hpup
+    return hpup.Plan(cfg["B"], cfg["dim"], cfg["N"], cfg["L"], b=cfg["b"], G=np.eye(3) * cfg["G"],
+                     Jt=np.diag(cfg["Jt"]), dt=cfg["dt"], method="euler", precision=precision)
+
+
+def timed_steps(plan, F, steps, warmup):
+    import torch
+    plan.step(F, warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    plan.step(F, steps)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def cpu_baseline(plan64, F0, cfg, budget_s=25.0):
+    """The reference's dense derivative (oracle, FP64) on a bounded sample of
+    output omega points of this workload, extrapolated linearly in the number
+    of output points to one full step."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import oracle as O
+    B, N, L = cfg["B"], cfg["N"], cfg["L"]
+    Jt = np.diag(cfg["Jt"])
+    Ffull = plan64.download_full(F0)
+    X = O.build_X(3, N, L)
+    OJO = O.build_OJO(N, L, Jt)
+    MR = O.build_MR(B, 3, Jt)
+    u = O.getu(B - 1)
+    cells = O.cg_table(B)
+    bt = np.array(cfg["b"])
+    G = np.eye(3) * cfg["G"]
+    threads = len(os.sched_getaffinity(0))
+    nx = N ** 3
+    samples = []
+    for S in (2, 6):
+        t0 = time.perf_counter()
+        O.derivative(B, 3, N, Ffull, X, OJO, MR, bt, G, L, u, cells, 0, S)
+        samples.append((S, time.perf_counter() - t0))
+        log("cpu sample", S, "points", "%.2f s" % samples[-1][1])
+        if samples[-1][1] > budget_s:
+            break
+    if len(samples) == 2:
+        (s1, t1), (s2, t2) = samples
+        b = (t2 - t1) / (s2 - s1)
+        a = max(t1 - b * s1, 0.0)
+    else:
+        s1, t1 = samples[0]
+        a, b = 0.0, t1 / s1
+    t_step = a + b * nx
+    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "reference dense derivative (oracle/refcpu.c, FP64, %d OpenMP threads) on the first %s of %d "
+                      "output omega points of this workload, extrapolated linearly in the point count to one "
+                      "Euler step (%.1f s/step)" % (threads, "/".join(str(s) for s, _ in samples), nx, t_step)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-reps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU sharding is "
+              "described in DESIGN.md, not implemented)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    cfg = dict(CFG)
+    K, W = args.steps, args.warmup
+
+    log("building FP64 plan and initial coefficients")
+    p64 = make_plan(cfg, "fp64")
+    F0 = initial_coeffs(p64, cfg)
+    im = p64.mass_index  # F^0_00(k = 0)
+    mass0 = float(F0[im].real)
+    F64 = F0.clone()
+    ms64 = timed_steps(p64, F64, K, W)
+    log("fp64 %.3f ms/step" % ms64)
+    mass_drift = abs(float(F64[im].real) - mass0) / abs(mass0)
+
+    if args.precision == "fp64":
+        pm, Fm, msm = p64, F64, ms64
+        err = 0.0
+    else:
+        pm = make_plan(cfg, args.precision)
+        Fm = F0.to(torch.complex64)
+        msm = timed_steps(pm, Fm, K, W)
+        log("%s %.3f ms/step" % (args.precision, msm))
+        err = float(torch.linalg.vector_norm((Fm.to(torch.complex128) - F64)) / torch.linalg.vector_norm(F64))
+
+    # per-kernel timing with HIP events on the stream the kernels run on
+    Fp = Fm.clone()
+    prof = pm.profile(Fp, args.profile_reps)
+    del Fp
+    name, kms, kbytes = max(prof, key=lambda t: t[1])
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    log("kernels:", ", ".join("%s %.3f ms" % (n, t) for n, t, _ in prof))
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(p64, F0, cfg)
+        except Exception as e:  # the CPU leg must not hide the GPU result
+            log("cpu baseline failed:", repr(e))
+
+    steps_s = 1000.0 / msm
+    out = {
+        "metric": "propagation steps/s, B=15 SO(3)xR^3: FP32-MFMA vs FP64 at <=1e-6 rel L2",
+        "value": steps_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": msm,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16x3": "bf16x3-f32acc", "fp64": "f64"}[args.precision],
+        "data": "synthetic (seeded matrix-Fisher x Gaussian initial density, forward-transformed on the GPU)",
+        "config": {"workload": "3-D pendulum SO(3)xR^3, B=15, 30^3 omega grid over [-5,5]^3, Euler dt=0.004",
+                   "B": 15, "omega_grid": [30, 30, 30], "coefficients": pm.ncoef,
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "fp64_steps_per_s": 1000.0 / ms64,
+        "speedup_vs_fp64": ms64 / msm,
+        "rel_l2_vs_fp64": err,
+        "fp64_mass_drift": mass_drift,
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kms,
+                     "algorithmic_bytes": kbytes},
+        "kernels_ms": {n: t for n, t, _ in prof},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,10 +69,58 @@ template <class T> struct XformTables {
 };
 
 template <class T> struct DriftTables {
-    DevBuf cg_src, cg_coef, u_src, u_coef, e3, kap, kc, kd, M3;
+    DevBuf cg_src, cg_coef, u_src, u_coef, e3, kap, kc, kd, M3, mix_src, mix_coef;
     DriftDev<T> view{};
+    MixDev<T> mix{};
+    // Merges the Clebsch-Gordan and u_1/u_2 term lists of each row by source row.
+    void build_mix(const DriftSpec& s) {
+        const int64_t nrc = nrc_of(s.B);
+        const int NC = s.dim + 2;
+        std::vector<std::vector<std::pair<int32_t, std::vector<double>>>> rows(nrc);
+        auto slot = [&](int64_t r, int32_t src) -> std::vector<double>& {
+            for (auto& e : rows[r])
+                if (e.first == src) return e.second;
+            rows[r].push_back({src, std::vector<double>(2 * NC, 0.0)});
+            return rows[r].back().second;
+        };
+        for (int64_t r = 0; r < nrc; r++) {
+            for (int t = 0; t < s.TG; t++) {
+                const int32_t src = s.cg_src[r * s.TG + t];
+                if (src < 0) break;
+                auto& c = slot(r, src);
+                for (int p = 0; p < s.dim; p++) {
+                    c[2 * p] += s.cg_coef[((r * s.TG + t) * s.dim + p) * 2];
+                    c[2 * p + 1] += s.cg_coef[((r * s.TG + t) * s.dim + p) * 2 + 1];
+                }
+            }
+            for (int q = 0; q < 2; q++)
+                for (int t = 0; t < s.TU; t++) {
+                    const int64_t i = (q * nrc + r) * s.TU + t;
+                    const int32_t src = s.u_src[i];
+                    if (src < 0) break;
+                    auto& c = slot(r, src);
+                    c[2 * (s.dim + q)] += s.u_coef[2 * i];
+                    c[2 * (s.dim + q) + 1] += s.u_coef[2 * i + 1];
+                }
+        }
+        int TS = 1;
+        for (auto& rw : rows) TS = std::max<int>(TS, (int)rw.size());
+        std::vector<int32_t> src(nrc * TS, -1);
+        std::vector<double> coef(nrc * TS * NC * 2, 0.0);
+        for (int64_t r = 0; r < nrc; r++)
+            for (size_t t = 0; t < rows[r].size(); t++) {
+                src[r * TS + t] = rows[r][t].first;
+                std::copy(rows[r][t].second.begin(), rows[r][t].second.end(), coef.begin() + (r * TS + t) * NC * 2);
+            }
+        mix_src = upload(src);
+        mix_coef = upload(cast_vec<T>(coef));
+        mix.TS = TS;
+        mix.src = mix_src.as<int32_t>();
+        mix.coef = mix_coef.as<cT<T>>();
+    }
     void build(const DriftSpec& s) {
         const int N = s.N;
+        build_mix(s);
         cg_src = upload(s.cg_src);
         cg_coef = upload(cast_vec<T>(s.cg_coef));
         u_src = upload(s.u_src);
@@ -128,6 +176,9 @@ struct PlanBase {
     virtual void full_to_compact(const void* a, void* b, hipStream_t st) = 0;
     virtual void compact_to_full(const void* a, void* b, hipStream_t st) = 0;
     virtual void stat(const void* f, double* ER, double* Ex, double* Varx, double* fx, hipStream_t st) = 0;
+    virtual int profile(void* F, int nreps, double* ms, int maxk, hipStream_t st) = 0;
+    virtual const char* profile_name(int i) const = 0;
+    virtual double profile_bytes(int i) const = 0;
 };
 
 template <class T> struct PlanT : PlanBase {
@@ -167,15 +218,23 @@ template <class T> struct PlanT : PlanBase {
         launch_inv_ag<T>(xf.view, G.p, f, st);
         HPUP_HIP(hipGetLastError());
     }
+    // MFMA kernels cover omega grids up to 32 points per axis; larger grids use
+    // the LDS-tiled FMA kernels (three passes).
+    bool fused() const { return model.N <= 32; }
     void stage(const void* Fin, const void* F0, void* Fout, const StageCoef<T>& sc, hipStream_t st) {
         const DriftDev<T>& D = dr.view;
+        if (fused()) {
+            if (D.dim == 3) launch_axis3_mfma<T>(D, Fin, Z3.p, st);
+            launch_drift_fused<T>(D, dr.mix, Fin, Z3.p, F0, Fout, ACC.p, sc, st);
+            return;
+        }
         launch_rotmix<T>(D, Fin, Wb.p, V1.p, V2.p, st);
         if (D.dim == 3) launch_axis_outer<T>(Fin, Z3.p, dr.M3.p, D.N, (int64_t)D.N * D.N, D.nrc, T(1), st);
         launch_drift_slab<T>(D, Fin, Z3.p, Wb.p, V1.p, V2.p, F0, Fout, ACC.p, sc, st);
     }
     void ensure_drift_buffers(bool rk) {
         HPUP_REQUIRE(has_drift, HPUP_ERR_ARG, "plan has no drift operator (dim must be 2 or 3)");
-        ensure(Wb, cbytes()); ensure(V1, cbytes()); ensure(V2, cbytes());
+        if (!fused()) { ensure(Wb, cbytes()); ensure(V1, cbytes()); ensure(V2, cbytes()); }
         if (model.dim == 3) ensure(Z3, cbytes());
         if (rk) { ensure(Fs, cbytes()); ensure(ACC, cbytes()); }
     }
@@ -213,6 +272,70 @@ template <class T> struct PlanT : PlanBase {
         ensure_drift_buffers(false);
         stage(F, nullptr, dF, StageCoef<T>{1, 0, 1, 0, 0, 0, 0, 0, 1}, st);
         HPUP_HIP(hipGetLastError());
+    }
+    int profile(void* Fv, int nreps, double* ms, int maxk, hipStream_t st) override {
+        ensure_drift_buffers(false);
+        HPUP_REQUIRE(nreps >= 1 && maxk >= 3, HPUP_ERR_ARG, "profile: need nreps >= 1 and room for 3 kernels");
+        const DriftDev<T>& D = dr.view;
+        const StageCoef<T> sc{(T)model.dt, 0, 1, 0, 0, 0, 0, 0, 0};
+        const bool fz = fused();
+        const int nk = fz ? 2 : 3;
+        hipEvent_t ev[4];
+        for (auto& e : ev) HPUP_HIP(hipEventCreate(&e));
+        double acc[3] = {0, 0, 0};
+        try {
+            for (int it = 0; it < nreps; it++) {
+                HPUP_HIP(hipEventRecord(ev[0], st));
+                if (fz) {
+                    if (D.dim == 3) launch_axis3_mfma<T>(D, Fv, Z3.p, st);
+                    HPUP_HIP(hipEventRecord(ev[1], st));
+                    launch_drift_fused<T>(D, dr.mix, Fv, Z3.p, nullptr, Fv, ACC.p, sc, st);
+                    HPUP_HIP(hipEventRecord(ev[2], st));
+                } else {
+                    launch_rotmix<T>(D, Fv, Wb.p, V1.p, V2.p, st);
+                    HPUP_HIP(hipEventRecord(ev[1], st));
+                    if (D.dim == 3) launch_axis_outer<T>(Fv, Z3.p, dr.M3.p, D.N, (int64_t)D.N * D.N, D.nrc, T(1), st);
+                    HPUP_HIP(hipEventRecord(ev[2], st));
+                    launch_drift_slab<T>(D, Fv, Z3.p, Wb.p, V1.p, V2.p, nullptr, Fv, ACC.p, sc, st);
+                    HPUP_HIP(hipEventRecord(ev[3], st));
+                }
+                HPUP_HIP(hipEventSynchronize(ev[nk]));
+                for (int k = 0; k < nk; k++) {
+                    float t = 0.f;
+                    HPUP_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                    acc[k] += t;
+                }
+            }
+        } catch (...) {
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto& e : ev) HPUP_HIP(hipEventDestroy(e));
+        for (int k = 0; k < nk; k++) ms[k] = acc[k] / nreps;
+        return nk;
+    }
+    const char* profile_name(int i) const override {
+        static const char* fz[2] = {"k_axis3_mfma", "k_drift_fused"};
+        static const char* names[3] = {"k_rotmix", "k_axis_outer", "k_drift_slab"};
+        if (fused()) return (i >= 0 && i < 2) ? fz[i] : "";
+        return (i >= 0 && i < 3) ? names[i] : "";
+    }
+    double profile_bytes(int i) const override {
+        const double c = (double)sizeof(cT<T>) * (double)ncoef;
+        const bool d3 = model.dim == 3;
+        if (fused()) {
+            switch (i) {
+            case 0: return d3 ? 2 * c : 0.0;   // read F; write T3 F
+            case 1: return (d3 ? 3 : 2) * c;   // read F (+ T3 F); write F
+            default: return 0.0;
+            }
+        }
+        switch (i) {
+        case 0: return 4 * c;                  // read F; write W, -U1 F, -U2 F
+        case 1: return d3 ? 2 * c : 0.0;       // read F; write T3 F
+        case 2: return (d3 ? 6 : 5) * c;       // read F, T3F, W, V1, V2; write F
+        default: return 0.0;
+        }
     }
     void full_to_compact(const void* a, void* b, hipStream_t st) override {
         launch_full_to_compact<T>(model.B, nx, a, b, st);
@@ -382,6 +505,17 @@ int hpup_plan_full_to_compact(hpup_plan* p, const void* a, void* b, void* stream
 int hpup_plan_compact_to_full(hpup_plan* p, const void* a, void* b, void* stream) {
     return guard([&] { p->impl->compact_to_full(a, b, (hipStream_t)stream); });
 }
+int hpup_plan_profile(hpup_plan* p, void* F, int nreps, double* ms, int max_kernels, void* stream) {
+    int n = 0;
+    int rc = guard([&] {
+        HPUP_REQUIRE(p && F && ms, HPUP_ERR_ARG, "null argument");
+        n = p->impl->profile(F, nreps, ms, max_kernels, (hipStream_t)stream);
+    });
+    return rc == HPUP_OK ? n : rc;
+}
+const char* hpup_plan_profile_name(const hpup_plan* p, int i) { return p ? p->impl->profile_name(i) : ""; }
+double hpup_plan_profile_bytes(const hpup_plan* p, int i) { return p ? p->impl->profile_bytes(i) : 0.0; }
+
 int hpup_plan_stat_moments(hpup_plan* p, const void* f, double* ER, double* Ex, double* Varx, double* fx,
                            void* stream) {
     return guard([&] { p->impl->stat(f, ER, Ex, Varx, fx, (hipStream_t)stream); });

@@ -34,6 +34,21 @@ template <class T> struct StageCoef {
     int read_f0, read_acc, write_acc, deriv_only;
 };
 
+// Rotation-mixing sources per output row r for the fused kernel: up to TS
+// distinct source rows; per source dim + 2 complex coefficients
+// [CG_1 .. CG_dim, -u_1, -u_2].
+template <class T> struct MixDev {
+    int TS;
+    const int32_t* src;   // nrc*TS, -1 padded
+    const cT<T>* coef;    // nrc*TS*(dim+2)
+};
+
+template <class T>
+void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
+                        void* Fout, void* ACC, const StageCoef<T>& sc, hipStream_t st);
+template <class T>
+void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, hipStream_t st);
+
 template <class T>
 void launch_rotmix(const DriftDev<T>& D, const void* F, void* W, void* V1, void* V2, hipStream_t st);
 template <class T>

@@ -35,9 +35,18 @@ struct Error {
 // ------------------------------------------------------------ index maths --
 
 inline int64_t nrc_of(int B) { return (int64_t)B * (2 * B - 1) * (2 * B + 1) / 3; }
-inline int64_t lblock_off(int l) { return (int64_t)l * (2 * l - 1) * (2 * l + 1) / 3; }
-// compact rotation index (reference compact layout, pendulum_propagate_reduced/integrate.cu:141-178)
-inline int64_t ridx(int l, int m, int n) { return lblock_off(l) + (int64_t)(n + l) * (2 * l + 1) + (m + l); }
+// Device compact rotation index, m-major: for m = -lmax..lmax, l = |m|..lmax,
+// n = -l..l.  Every coefficient the drift couples to (l, m, n) -- (l, m, n+-1)
+// and (l+-1, m, n+-1) -- lies within 2B+1 rows, so neighbour rows are read
+// from cache.  (The reference's MEX uses l-major blocks; the layout is
+// internal and converted at the boundary.)
+inline int64_t sumsq(int64_t n) { return n * (n + 1) * (2 * n + 1) / 6; }
+inline int64_t moff(int B, int m) {
+    const int lmax = B - 1;
+    const int64_t s = m <= 0 ? sumsq(lmax) - sumsq(-m) : sumsq(lmax) + sumsq(m - 1);
+    return (int64_t)(m + lmax) * B * B - s;
+}
+inline int64_t ridx_B(int B, int l, int m, int n) { return moff(B, m) + (int64_t)l * l - (int64_t)m * m + (n + l); }
 inline int64_t ipow64(int64_t a, int e) {
     int64_t r = 1;
     while (e-- > 0) r *= a;

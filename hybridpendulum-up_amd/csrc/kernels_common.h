@@ -28,4 +28,27 @@ template <class T> __device__ __forceinline__ void cfma(cT<T>& acc, cT<T> a, cT<
 // i * s * a  (multiplication by the imaginary scalar i s)
 template <class T> __device__ __forceinline__ cT<T> cimul(cT<T> a, T s) { return cmk<T>(-a.y * s, a.x * s); }
 
+// m-major compact rotation index (same formula as the host's ridx_B)
+__device__ __forceinline__ int64_t d_sumsq(int64_t n) { return n * (n + 1) * (2 * n + 1) / 6; }
+__device__ __forceinline__ int64_t d_moff(int B, int m) {
+    const int lmax = B - 1;
+    const int64_t s = m <= 0 ? d_sumsq(lmax) - d_sumsq(-m) : d_sumsq(lmax) + d_sumsq(m - 1);
+    return (int64_t)(m + lmax) * B * B - s;
+}
+__device__ __forceinline__ int64_t d_ridx(int B, int l, int m, int n) {
+    return d_moff(B, m) + (int64_t)l * l - (int64_t)m * m + (n + l);
+}
+// inverse of d_ridx
+__device__ inline void d_rlmn(int B, int64_t r, int& l, int& m, int& n) {
+    const int lmax = B - 1;
+    int mm = -lmax;
+    while (mm < lmax && d_moff(B, mm + 1) <= r) mm++;
+    const int64_t o = r - d_moff(B, mm);
+    int ll = mm < 0 ? -mm : mm;
+    while (ll < lmax && (int64_t)(ll + 1) * (ll + 1) - (int64_t)mm * mm <= o) ll++;
+    n = (int)(o - ((int64_t)ll * ll - (int64_t)mm * mm)) - ll;
+    m = mm;
+    l = ll;
+}
+
 }  // namespace hpup

@@ -275,14 +275,14 @@ DriftSpec build_drift_spec_from_arrays(int B, int dim, int N, double L, const do
                 };
                 for (int n = -l; n <= l; n++)
                     for (int m = -l; m <= l; m++) {
-                        const int64_t r = ridx(l, m, n);
+                        const int64_t r = ridx_B(B, l, m, n);
                         for (int m2 = -1; m2 <= 1; m2++)
                             for (int n2 = -1; n2 <= 1; n2++) {
                                 const int m1 = m - m2, n1 = n - n2;
                                 if (std::abs(m1) > l1 || std::abs(n1) > l1) continue;
                                 const double cc = cl * cgv(m, m1, m2) * cgv(n, n1, n2);
                                 if (cc == 0.0) continue;
-                                Term t{ridx(l1, m1, n1), {0.0, 0.0, 0.0}};
+                                Term t{ridx_B(B, l1, m1, n1), {0.0, 0.0, 0.0}};
                                 double mag = 0.0;
                                 for (int p = 0; p < dim; p++) {
                                     t.c[p] = cc * mr1(p, m2, n2);
@@ -326,10 +326,10 @@ DriftSpec build_drift_spec_from_arrays(int B, int dim, int N, double L, const do
         for (int l = 0; l <= lmax; l++)
             for (int n = -l; n <= l; n++)
                 for (int m = -l; m <= l; m++) {
-                    auto& v = ut[p * nrc + ridx(l, m, n)];
+                    auto& v = ut[p * nrc + ridx_B(B, l, m, n)];
                     for (int j = -l; j <= l; j++) {
                         cd c = uval(p, l, n, j);
-                        if (std::abs(c) > utol) v.push_back({ridx(l, m, j), -c});
+                        if (std::abs(c) > utol) v.push_back({ridx_B(B, l, m, j), -c});
                     }
                     TU = std::max<int>(TU, (int)v.size());
                 }
@@ -354,8 +354,8 @@ DriftSpec build_drift_spec_from_arrays(int B, int dim, int N, double L, const do
                         continue;
                     }
                     for (int m = -l; m <= l; m++) {
-                        s.e3[2 * ridx(l, m, n)] = -c.real();
-                        s.e3[2 * ridx(l, m, n) + 1] = -c.imag();
+                        s.e3[2 * ridx_B(B, l, m, n)] = -c.real();
+                        s.e3[2 * ridx_B(B, l, m, n) + 1] = -c.imag();
                     }
                 }
     }

@@ -14,10 +14,6 @@
 
 namespace hpup {
 
-__device__ __forceinline__ int64_t d_ridx(int l, int m, int n) {
-    return (int64_t)l * (2 * l - 1) * (2 * l + 1) / 3 + (int64_t)(n + l) * (2 * l + 1) + (m + l);
-}
-
 // ---------------------------------------------------------- forward 2-D --
 
 template <class T>
@@ -104,7 +100,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             if (l0 + t > lmax) break;
-            F[d_ridx(l0 + t, m, n) * X.nx + x] = acc[t];
+            F[d_ridx(B, l0 + t, m, n) * X.nx + x] = acc[t];
         }
     }
 }
@@ -129,7 +125,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
 #pragma unroll
         for (int t = 0; t < 8; t++) acc[t] = czero<T>();
         for (int l = lmin; l <= lmax; l++) {
-            const cT<T> fv = F[d_ridx(l, m, n) * X.nx + x];
+            const cT<T> fv = F[d_ridx(B, l, m, n) * X.nx + x];
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 if (j0 + t >= n2) break;
@@ -243,11 +239,8 @@ __global__ void k_full_to_compact(int B, int64_t nx, const cT<T>* __restrict__ f
         const int64_t r = r0 + threadIdx.x, x = x0 + yy;
         cT<T> v = czero<T>();
         if (r < nrc && x < nx) {
-            // r -> (l, m, n)
-            int l = 0;
-            while ((int64_t)(l + 1) * (2 * l + 1) * (2 * l + 3) / 3 <= r) l++;
-            const int64_t o = r - (int64_t)l * (2 * l - 1) * (2 * l + 1) / 3;
-            const int m = (int)(o % (2 * l + 1)) - l, n = (int)(o / (2 * l + 1)) - l;
+            int l, m, n;
+            d_rlmn(B, r, l, m, n);
             v = full[(m + lmax) + (int64_t)M * ((n + lmax) + (int64_t)M * l) + nfr * x];
         }
         tile[yy][threadIdx.x] = v;
@@ -270,7 +263,7 @@ __global__ void k_compact_to_full(int B, int64_t nx, const cT<T>* __restrict__ c
         cT<T> v = czero<T>();
         if (fr < nfr && x < nx) {
             const int m = (int)(fr % M) - lmax, n = (int)((fr / M) % M) - lmax, l = (int)(fr / ((int64_t)M * M));
-            if (abs(m) <= l && abs(n) <= l) v = comp[d_ridx(l, m, n) * nx + x];
+            if (abs(m) <= l && abs(n) <= l) v = comp[d_ridx(B, l, m, n) * nx + x];
         }
         tile[yy][threadIdx.x] = v;
     }

@@ -55,6 +55,7 @@ EXPORTS = [
     "hpup_plan_create", "hpup_plan_destroy", "hpup_plan_num_coeffs", "hpup_plan_num_grid",
     "hpup_plan_forward", "hpup_plan_backward", "hpup_plan_step", "hpup_plan_derivative",
     "hpup_plan_full_to_compact", "hpup_plan_compact_to_full", "hpup_plan_stat_moments",
+    "hpup_plan_profile", "hpup_plan_profile_name", "hpup_plan_profile_bytes",
 ]
 
 
@@ -77,8 +78,23 @@ def lib():
         L.hpup_plan_num_coeffs.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_grid.argtypes = [ctypes.c_void_p]
         L.hpup_plan_stat_moments.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _D, _D, _D, _D, ctypes.c_void_p]
+        L.hpup_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int,
+                                        ctypes.c_void_p]
+        L.hpup_plan_profile_name.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.hpup_plan_profile_name.restype = ctypes.c_char_p
+        L.hpup_plan_profile_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.hpup_plan_profile_bytes.restype = ctypes.c_double
         _lib = L
     return _lib
+
+
+def ridx(B, l, m, n):
+    """Device compact rotation row of (l, m, n): m-major, then l = |m|..B-1,
+    then n = -l..l (the same formula as the C library)."""
+    lmax = B - 1
+    sq = lambda k: k * (k + 1) * (2 * k + 1) // 6
+    s = sq(lmax) - sq(-m) if m <= 0 else sq(lmax) + sq(m - 1)
+    return (m + lmax) * B * B - s + l * l - m * m + (n + l)
 
 
 def _check(rc):
@@ -258,6 +274,12 @@ class Plan:
         return dF
 
     @property
+    def mass_index(self):
+        """Flat index of F^0_00(k = 0); the total probability is
+        F[mass_index].real * (L/N)^dim."""
+        return ridx(self.B, 0, 0, 0) * self.nx
+
+    @property
     def nfull(self):
         return (2 * self.B - 1) ** 2 * self.B * self.nx
 
@@ -284,6 +306,17 @@ class Plan:
         d = max(self.dim, 1)
         return (ER.reshape((3, 3), order="F"), Ex[: self.dim], Varx.reshape((d, d), order="F")[: self.dim, : self.dim],
                 fx.reshape((self.N,) * self.dim, order="F") if self.dim else fx)
+
+    def profile(self, F, nreps=3):
+        """Average milliseconds per kernel of an Euler step (advances F by nreps
+        steps), with each kernel's name and algorithmic bytes per launch."""
+        ms = np.zeros(8)
+        n = lib().hpup_plan_profile(self._h, self._chk(F, self.ncoef, self.cdtype), int(nreps), _p(ms), 8,
+                                    self._stream())
+        if n < 0:
+            _check(n)
+        return [(lib().hpup_plan_profile_name(self._h, i).decode(), float(ms[i]),
+                 float(lib().hpup_plan_profile_bytes(self._h, i))) for i in range(n)]
 
     # host conveniences (MATLAB full layout numpy arrays)
     def upload_full(self, Ffull_np):

@@ -196,6 +196,17 @@ int hpup_plan_derivative(hpup_plan* plan, const void* F_dev, void* dF_dev, void*
 int hpup_plan_full_to_compact(hpup_plan* plan, const void* Ffull_dev, void* F_dev, void* stream);
 int hpup_plan_compact_to_full(hpup_plan* plan, const void* F_dev, void* Ffull_dev, void* stream);
 
+/* Per-kernel timing of the Euler step: runs nreps steps on F_dev (in place)
+ * recording HIP events between the kernels of each step on `stream`, and
+ * writes the average milliseconds of each kernel into ms[0..n-1], n <= max.
+ * Returns n (> 0) or a negative hpup_status.  hpup_plan_profile_name(i)
+ * names kernel i. */
+int hpup_plan_profile(hpup_plan* plan, void* F_dev, int nreps, double* ms, int max_kernels, void* stream);
+const char* hpup_plan_profile_name(const hpup_plan* plan, int i);
+/* Algorithmic HBM bytes one launch of kernel i must move (inputs read once,
+ * outputs written once), for roofline reporting. */
+double hpup_plan_profile_bytes(const hpup_plan* plan, int i);
+
 /* Grid statistics (get_stat moments): rotation mean E[R] (3x3), omega mean
  * (dim), omega covariance (dim x dim), omega marginal fx (nx) -- all FP64
  * host outputs, from the device grid f_dev. */

@@ -126,3 +126,60 @@ def test_bad_method_is_an_error():
     F, X, MR, bt, G, u, cells, L = _reduced_case(B=2, N=4)
     with pytest.raises(hpup.HpupError, match="method"):
         hpup.pendulum_propagate_reduced(F, X, MR, bt, G, 0.01, L, u, cells, "leapfrog")
+
+
+def test_fallback_path_large_omega_grid():
+    """N > 32 runs the LDS-tiled FMA kernels instead of the MFMA ones."""
+    _print("case reduced N=34 fallback")
+    F, X, MR, bt, G, u, cells, L = _reduced_case(B=2, N=34)
+    B, N, dt = F.shape[2], F.shape[3], 0.01
+    ref = O.integrate(B, 2, N, F, X, None, MR, bt, G, dt, L, u, cells, "RK2")
+    out = hpup.pendulum_propagate_reduced(F, X, MR, bt, G, dt, L, u, cells, "RK2")
+    assert rel_l2(out, ref) < 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
+def test_plan_derivative_and_mass(precision, tol):
+    """Device-resident plan: derivative against the oracle (plan built from the
+    physical parameters, so it also checks the drivers' X/OJO/MR recipe), and
+    total probability F^0_00(k=0) unchanged by a step."""
+    import torch
+    _print("case plan", precision)
+    B, N, L = 4, 6, 10.0
+    Jt = np.diag([1.0, 0.93539, 0.24935])
+    Sigma = np.array([[0.8, 0.1, 0.0], [0.1, 0.6, 0.05], [0.0, 0.05, 1.0]])
+    f = density(B, 3, N, L, (10, 5, 1), seed=7, Sigma=Sigma)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    Ffull = O.fft_forward(B, f, d, w, 3, N)
+    X, OJO, MR, u, cells = operators(B, 3, N, L, Jt)
+    bt, G = np.array([0.2, 0.15, 0.25]), np.diag([0.1, 0.2, 0.05])
+    ref = O.derivative(B, 3, N, Ffull, X, OJO, MR, bt, G, L, u, cells)
+    p = hpup.Plan(B, 3, N, L, b=bt, G=G, Jt=Jt, dt=0.004, method="euler", precision=precision)
+    F = p.upload_full(Ffull)
+    dF = p.derivative(F)
+    torch.cuda.synchronize()
+    assert rel_l2(p.download_full(dF), ref) < tol
+    m0 = float(F[p.mass_index].real)
+    p.step(F, 3)
+    torch.cuda.synchronize()
+    assert abs(float(F[p.mass_index].real) - m0) <= 1e-12 * abs(m0) if precision == "fp64" else 1e-6 * abs(m0)
+
+
+def test_plan_transforms_roundtrip():
+    """forward(backward(F)) == F for bandlimited coefficients (plan API, FP64)."""
+    import torch
+    _print("case plan transforms")
+    B, N, L = 5, 8, 6.0
+    f = density(B, 3, N, L, (6, 3, 1), seed=9)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    Ffull = O.fft_forward(B, f, d, w, 3, N)
+    p = hpup.Plan(B, 3, N, L, precision="fp64")
+    F = p.upload_full(Ffull)
+    g = p.backward(F)
+    F2 = p.forward(g)
+    torch.cuda.synchronize()
+    assert rel_l2(p.download_full(F2), Ffull) < 1e-12
+    f_ref = O.fft_backward(B, Ffull, d, 3, N)
+    assert rel_l2(g.cpu().numpy(), f_ref.ravel(order="F")) < 1e-12
