@@ -18,6 +18,20 @@ namespace hpup {
 static thread_local std::string g_last_error;
 void set_error(const std::string& m) { g_last_error = m; }
 
+size_t lds_limit() {
+    int dev = 0, v = 0;
+    HPUP_HIP(hipGetDevice(&dev));
+    HPUP_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return (size_t)v;
+}
+
+void allow_lds(const void* kernel, size_t bytes) {
+    if (bytes <= kLdsTile) return;
+    HPUP_REQUIRE(bytes <= lds_limit(), HPUP_ERR_UNSUPPORTED,
+                 "kernel needs " + std::to_string(bytes) + " bytes of LDS, device allows " + std::to_string(lds_limit()));
+    HPUP_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 using cd = std::complex<double>;
 static const double kPi = 3.14159265358979323846;
 
@@ -158,6 +172,7 @@ template <class T> struct DriftTables {
         for (int i = 0; i < 9; i++) v.G[i] = (T)s.G[i];
         for (int i = 0; i < 3; i++) { v.b[i] = (T)s.b[i]; v.gyro[i] = (T)s.gyro[i]; }
         v.has_gyro = s.has_gyro ? 1 : 0;
+        v.grav[0] = v.grav[1] = v.grav[2] = nullptr;
     }
 };
 
@@ -185,6 +200,7 @@ template <class T> struct PlanT : PlanBase {
     XformTables<T> xf;
     DriftTables<T> dr;
     bool has_drift = false;
+    bool bf16x3 = false;  // circulants as 3xBF16 split products (FP32 storage)
     DevBuf G, Wb, V1, V2, Z3, Fs, ACC, tmp, fxb, fRb;
 
     void init_xform(const double* d, const double* wv) {
@@ -200,10 +216,12 @@ template <class T> struct PlanT : PlanBase {
     }
     size_t cbytes() const { return sizeof(cT<T>) * (size_t)ncoef; }
 
-    void forward(const void* f, void* F, hipStream_t st) override {
+    void forward(const void* f, void* F, hipStream_t st) override { forward_weighted(f, nullptr, 0, F, st); }
+    // forward transform of the grid product f(R, x) wR[R * wstride]
+    void forward_weighted(const void* f, const T* wR, int wstride, void* F, hipStream_t st) {
         const int M = 2 * model.B - 1;
         ensure(G, sizeof(cT<T>) * (size_t)M * M * 2 * model.B * nx);
-        launch_fwd_ag<T>(xf.view, f, G.p, st);
+        launch_fwd_ag<T>(xf.view, f, wR, wstride, G.p, st);
         launch_beta_fwd<T>(xf.view, G.p, F, st);
         launch_omega_dft<T>(xf.view, F, nullptr, false, st);
         HPUP_HIP(hipGetLastError());
@@ -220,12 +238,36 @@ template <class T> struct PlanT : PlanBase {
     }
     // MFMA kernels cover omega grids up to 32 points per axis; larger grids use
     // the LDS-tiled FMA kernels (three passes).
-    bool fused() const { return model.N <= 32; }
+    bool fused() const { return model.N <= 32 || model.dim == 0; }
+
+    // Density-form gravity (pendulum_reduced_den.m:453-516): the term is
+    // -sum_p c(k_p) fftSO3R(f mR_p) on the grid density f, with f the caller's
+    // grid for the first stage and the inverse transform of the stage input
+    // for the RK substeps.
+    bool den = false;
+    const void* den_f = nullptr;  // grid density consistent with the next stage input
+    DevBuf mR_dev, fgrid, gravb[3];
+    void den_gravity(const void* Fin, DriftDev<T>& D, hipStream_t st) {
+        const void* f = den_f;
+        den_f = nullptr;
+        if (!f) {
+            ensure(fgrid, sizeof(T) * (size_t)ngrid);
+            backward(Fin, fgrid.p, st);
+            f = fgrid.p;
+        }
+        for (int p = 0; p < model.dim; p++) {
+            ensure(gravb[p], cbytes());
+            forward_weighted(f, mR_dev.as<T>() + p, model.dim, gravb[p].p, st);
+            D.grav[p] = gravb[p].as<cT<T>>();
+        }
+    }
+
     void stage(const void* Fin, const void* F0, void* Fout, const StageCoef<T>& sc, hipStream_t st) {
-        const DriftDev<T>& D = dr.view;
+        DriftDev<T> D = dr.view;
+        if (den) den_gravity(Fin, D, st);
         if (fused()) {
-            if (D.dim == 3) launch_axis3_mfma<T>(D, Fin, Z3.p, st);
-            launch_drift_fused<T>(D, dr.mix, Fin, Z3.p, F0, Fout, ACC.p, sc, st);
+            if (D.dim == 3) launch_axis3_mfma<T>(D, Fin, Z3.p, bf16x3, st);
+            launch_drift_fused<T>(D, dr.mix, Fin, Z3.p, F0, Fout, ACC.p, sc, bf16x3, st);
             return;
         }
         launch_rotmix<T>(D, Fin, Wb.p, V1.p, V2.p, st);
@@ -234,6 +276,7 @@ template <class T> struct PlanT : PlanBase {
     }
     void ensure_drift_buffers(bool rk) {
         HPUP_REQUIRE(has_drift, HPUP_ERR_ARG, "plan has no drift operator (dim must be 2 or 3)");
+        HPUP_REQUIRE(fused() || !bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 drift needs an omega grid of at most 32");
         if (!fused()) { ensure(Wb, cbytes()); ensure(V1, cbytes()); ensure(V2, cbytes()); }
         if (model.dim == 3) ensure(Z3, cbytes());
         if (rk) { ensure(Fs, cbytes()); ensure(ACC, cbytes()); }
@@ -287,9 +330,9 @@ template <class T> struct PlanT : PlanBase {
             for (int it = 0; it < nreps; it++) {
                 HPUP_HIP(hipEventRecord(ev[0], st));
                 if (fz) {
-                    if (D.dim == 3) launch_axis3_mfma<T>(D, Fv, Z3.p, st);
+                    if (D.dim == 3) launch_axis3_mfma<T>(D, Fv, Z3.p, bf16x3, st);
                     HPUP_HIP(hipEventRecord(ev[1], st));
-                    launch_drift_fused<T>(D, dr.mix, Fv, Z3.p, nullptr, Fv, ACC.p, sc, st);
+                    launch_drift_fused<T>(D, dr.mix, Fv, Z3.p, nullptr, Fv, ACC.p, sc, bf16x3, st);
                     HPUP_HIP(hipEventRecord(ev[2], st));
                 } else {
                     launch_rotmix<T>(D, Fv, Wb.p, V1.p, V2.p, st);
@@ -401,8 +444,9 @@ static PlanBase* make_plan(const hpup_model& m, const double* d, const double* w
     if (m.precision == HPUP_FP64) {
         auto* q = new PlanT<double>();
         p.reset(q);
-    } else if (m.precision == HPUP_FP32) {
+    } else if (m.precision == HPUP_FP32 || m.precision == HPUP_BF16X3) {
         auto* q = new PlanT<float>();
+        q->bf16x3 = m.precision == HPUP_BF16X3;
         p.reset(q);
     } else {
         throw Error{HPUP_ERR_UNSUPPORTED, "precision mode not available"};
@@ -630,6 +674,42 @@ static void propagate_gateway(int dim, const T* F, const int64_t* dims, const T*
 }  // namespace hpup
 
 extern "C" {
+
+int hpup_pendulum_propagate_reduced_den(const double* F, const int64_t* dims, const double* f, const double* X,
+                                        const double* mR, const double* bt, const double* G, double dtt, double L,
+                                        const double* u, const double* d, const double* w, const char* method,
+                                        double* Fnew) {
+    return guard([&] {
+        HPUP_REQUIRE(F && dims && f && X && mR && bt && G && u && d && w && method && Fnew, HPUP_ERR_ARG,
+                     "null argument");
+        const int meth = hpup_parse_method(method);
+        HPUP_REQUIRE(meth >= 0, HPUP_ERR_METHOD, std::string("unknown method '") + method + "'");
+        const int B = (int)dims[2], M = 2 * B - 1, N = (int)dims[3];
+        HPUP_REQUIRE(dims[0] == M && dims[1] == M && dims[4] == N, HPUP_ERR_ARG, "F must be (2B-1,2B-1,B,N,N)");
+        const int64_t nx = (int64_t)N * N, nrot = (int64_t)M * M * B, nR = 8LL * B * B * B;
+        // the operator without the Clebsch-Gordan gravity term (MR = 0)
+        std::vector<double> MR0(2 * nrot * 2, 0.0);
+        std::vector<const double*> cptr((size_t)B * B, nullptr);
+        double btd[3] = {bt[0], bt[1], 0}, Gd[9] = {G[0], G[1], G[2], G[3]};
+        DriftSpec spec = build_drift_spec_from_arrays(B, 2, N, L, X, nullptr, MR0.data(), btd, Gd, u, cptr.data());
+        hpup_model m{};
+        m.B = B; m.dim = 2; m.N = N; m.L = L; m.dt = dtt; m.method = meth; m.precision = HPUP_FP64;
+        m.b[0] = btd[0]; m.b[1] = btd[1];
+        for (int i = 0; i < 4; i++) m.G[i] = Gd[i];
+        std::unique_ptr<PlanBase> p(make_plan(m, d, w, &spec));
+        auto* q = static_cast<PlanT<double>*>(p.get());
+        q->den = true;
+        q->mR_dev = upload(std::vector<double>(mR, mR + 2 * nR));
+        DevBuf fd(sizeof(double) * nR * nx), Ff(sizeof(double) * 2 * nrot * nx), Fc(sizeof(double) * 2 * p->ncoef);
+        HPUP_HIP(hipMemcpy(fd.p, f, fd.bytes, hipMemcpyHostToDevice));
+        HPUP_HIP(hipMemcpy(Ff.p, F, Ff.bytes, hipMemcpyHostToDevice));
+        p->full_to_compact(Ff.p, Fc.p, nullptr);
+        q->den_f = fd.p;
+        p->step(Fc.p, 1, nullptr);
+        p->compact_to_full(Fc.p, Ff.p, nullptr);
+        HPUP_HIP(hipMemcpy(Fnew, Ff.p, Ff.bytes, hipMemcpyDeviceToHost));
+    });
+}
 
 int hpup_pendulum_propagate_reduced(const double* F, const int64_t* dims, const double* X, const double* MR,
                                     const double* bt, const double* G, double dtt, double L, const double* u,

@@ -6,8 +6,13 @@
 
 namespace hpup {
 
-// LDS budget per workgroup used by every kernel launch.
-constexpr size_t kLdsLimit = 64 * 1024;
+// LDS bytes one workgroup may allocate on the current device (160 KB on
+// gfx950); kernels size their tiles to kLdsTile and only the large-grid
+// fallbacks go beyond it.
+constexpr size_t kLdsTile = 64 * 1024;
+size_t lds_limit();
+// Raises a kernel's dynamic-LDS cap when one launch needs more than 64 KB.
+void allow_lds(const void* kernel, size_t bytes);
 
 template <class T> struct DriftDev {
     int B, dim, N;
@@ -25,6 +30,9 @@ template <class T> struct DriftDev {
     T b[3];
     T gyro[3];
     int has_gyro;
+    // density-form gravity: transforms of f mR_p on the coefficient layout
+    // (nullptr when the gravity term comes from the Clebsch-Gordan table)
+    const cT<T>* grav[3];
 };
 
 // Integrator epilogue of the slab kernel:
@@ -43,11 +51,12 @@ template <class T> struct MixDev {
     const cT<T>* coef;    // nrc*TS*(dim+2)
 };
 
+// bf16x3: circulants as 3xBF16 split products (FP32 storage only)
 template <class T>
 void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
-                        void* Fout, void* ACC, const StageCoef<T>& sc, hipStream_t st);
+                        void* Fout, void* ACC, const StageCoef<T>& sc, bool bf16x3, hipStream_t st);
 template <class T>
-void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, hipStream_t st);
+void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf16x3, hipStream_t st);
 
 template <class T>
 void launch_rotmix(const DriftDev<T>& D, const void* F, void* W, void* V1, void* V2, hipStream_t st);
@@ -74,7 +83,8 @@ template <class T> struct XformDev {
 };
 
 template <class T>
-void launch_fwd_ag(const XformDev<T>& X, const void* f, void* G, hipStream_t st);
+// wR (nullable): transform the product f(R, x) wR[R * wstride] (grid weight)
+void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st);
 template <class T>
 void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t st);
 template <class T>

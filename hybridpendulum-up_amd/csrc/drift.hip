@@ -52,6 +52,8 @@ __global__ __launch_bounds__(256) void k_rotmix(DriftDev<T> D, const cT<T>* __re
         }
     cT<T> w = cscale<T>(Fr, dd);
     for (int p = 0; p < D.dim; p++) w = csub<T>(w, cimul<T>(acc[p], kc[p]));
+    for (int p = 0; p < D.dim; p++)  // gravity evaluated on the grid (density form)
+        if (D.grav[p]) w = csub<T>(w, cimul<T>(D.grav[p][r * D.nx + x], kc[p]));
     W[r * D.nx + x] = w;
 
     for (int q = 0; q < 2; q++) {
@@ -205,7 +207,7 @@ void launch_axis_outer(const void* in, void* out, const void* Mat, int N, int64_
                        T scale, hipStream_t st) {
     dim3 grid((unsigned)((inner + 63) / 64), (unsigned)nlines);
     size_t sh = sizeof(cT<T>) * ((size_t)N * N + 64 * (size_t)N);
-    HPUP_REQUIRE(sh <= kLdsLimit, HPUP_ERR_UNSUPPORTED, "omega grid too large for the axis kernel at this precision");
+    allow_lds((const void*)k_axis_outer<T>, sh);
     hipLaunchKernelGGL(k_axis_outer<T>, grid, dim3(64, 4), sh, st, (const cT<T>*)in, (cT<T>*)out,
                        (const cT<T>*)Mat, N, inner, scale);
 }
@@ -217,9 +219,10 @@ void launch_drift_slab(const DriftDev<T>& D, const void* Fin, const void* Z3, co
     const int N = D.N, NN = N * N;
     const int64_t nslab = D.nrc * (D.dim == 3 ? N : 1);
     const size_t sh = sizeof(cT<T>) * (2 * (size_t)NN + 2 * (size_t)N);
-    HPUP_REQUIRE(sh <= kLdsLimit, HPUP_ERR_UNSUPPORTED, "omega grid too large for the slab kernel at this precision");
+    HPUP_REQUIRE(NN <= 16 * 256, HPUP_ERR_UNSUPPORTED, "omega grid larger than 64 per axis");
     const int ept = (NN + 255) / 256;
     auto go = [&](auto kern) {
+        allow_lds((const void*)kern, sh);
         hipLaunchKernelGGL(kern, dim3((unsigned)nslab), dim3(256), sh, st, D, (const cT<T>*)Fin, (const cT<T>*)Z3,
                            (const cT<T>*)W, (const cT<T>*)V1, (const cT<T>*)V2, (const cT<T>*)F0, (cT<T>*)Fout,
                            (cT<T>*)ACC, sc);

@@ -43,6 +43,25 @@ __global__ void k_lambda(const double* __restrict__ R, const double* __restrict_
     lam[k + (int64_t)numR * ix] = vC < 0.0 ? 0.0 : l0;
 }
 
+// getLambda.cu:218-246 (compact form): lambda per region, and the flag of
+// omega points with non-negative contact velocity.
+__global__ void k_lambda_compact(const double* __restrict__ R, const double* __restrict__ x,
+                                 const double* __restrict__ PC, const int32_t* __restrict__ indR, int numR, int64_t nx,
+                                 const double* __restrict__ theta, const signed char* __restrict__ cat, double theta0,
+                                 double tht, double lmax, double* __restrict__ lam, signed char* __restrict__ flag) {
+    const int64_t ix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (ix >= nx) return;
+    const int64_t iR = indR[k];
+    if (ix == 0)
+        lam[k] = cat[iR] == 2 ? lmax : lmax / 2 * sin(M_PI / (2 * tht) * (theta[iR] - theta0)) + lmax / 2;
+    const double* Rr = R + 9 * iR;
+    const double o1 = Rr[1] * x[2 * ix] + Rr[4] * x[2 * ix + 1];
+    const double o2 = Rr[2] * x[2 * ix] + Rr[5] * x[2 * ix + 1];
+    const double vC = o1 * PC[3 * iR + 2] - o2 * PC[3 * iR + 1];
+    flag[(int64_t)k * nx + ix] = vC < 0.0 ? 0 : 1;
+}
+
 // getOmega.cu:64-137
 __global__ void k_omega(const double* __restrict__ R, const double* __restrict__ x, const int32_t* __restrict__ indR,
                         int numR, int64_t nx, double eps, const double* __restrict__ PC, int old_dir, double xlo,
@@ -272,6 +291,43 @@ int hpup_getLambda(const double* R, int B, const double* x, int N, double d, dou
                            theta_t, lambda_max, lam.as<double>());
         HPUP_HIP(hipGetLastError());
         down(lambda, lam, (size_t)cnt * nx);
+    });
+}
+
+int hpup_getLambda_compact(const double* R, int B, const double* x, int N, double d, double h, double r,
+                           double theta_t, double lambda_max, double* lambda, int32_t* indR, int* numR, int32_t* indx,
+                           int32_t* numx, double* PC) {
+    return guard2([&] {
+        HPUP_REQUIRE(R && x && lambda && indR && numR && indx && numx && PC && B > 0 && N > 1, HPUP_ERR_ARG,
+                     "bad argument");
+        const int64_t nR = 8LL * B * B * B, nx = (int64_t)N * N;
+        const double theta0 = std::asin(d / std::sqrt(h * h + r * r)) - std::asin(r / std::sqrt(h * h + r * r));
+        DevBuf Rd = up(R, 9 * nR), xd = up(x, 2 * nx), th(sizeof(double) * nR), cat(nR), PCd(sizeof(double) * 3 * nR);
+        hipLaunchKernelGGL(k_theta_pc, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, 0, Rd.as<double>(), nR,
+                           theta0, theta_t, h, r, th.as<double>(), cat.as<signed char>(), PCd.as<double>());
+        HPUP_HIP(hipGetLastError());
+        std::vector<signed char> hc(nR);
+        down(hc.data(), cat, nR);
+        int cnt = 0;
+        for (int64_t i = 0; i < nR; i++)
+            if (hc[i] != 0) indR[cnt++] = (int32_t)i;
+        *numR = cnt;
+        down(PC, PCd, 3 * nR);
+        if (cnt == 0) return;
+        DevBuf ir = up(indR, cnt), lam(sizeof(double) * cnt), fl((size_t)cnt * nx);
+        hipLaunchKernelGGL(k_lambda_compact, grid_rx(nx, cnt), dim3(128), 0, 0, Rd.as<double>(), xd.as<double>(),
+                           PCd.as<double>(), ir.as<int32_t>(), cnt, nx, th.as<double>(), cat.as<signed char>(), theta0,
+                           theta_t, lambda_max, lam.as<double>(), fl.as<signed char>());
+        HPUP_HIP(hipGetLastError());
+        down(lambda, lam, cnt);
+        std::vector<signed char> hf((size_t)cnt * nx);
+        down(hf.data(), fl, hf.size());
+        for (int k = 0; k < cnt; k++) {
+            int c = 0;
+            for (int64_t ix = 0; ix < nx; ix++)
+                if (hf[(size_t)k * nx + ix]) indx[(int64_t)k * nx + c++] = (int32_t)ix;
+            numx[k] = c;
+        }
     });
 }
 

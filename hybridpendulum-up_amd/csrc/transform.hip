@@ -17,8 +17,8 @@ namespace hpup {
 // ---------------------------------------------------------- forward 2-D --
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fwd_ag(XformDev<T> X, const T* __restrict__ f, cT<T>* __restrict__ G,
-                                                int XC) {
+__global__ __launch_bounds__(256) void k_fwd_ag(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
+                                                int wstride, cT<T>* __restrict__ G, int XC) {
     extern __shared__ char smem_raw[];
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);      // [xl][g][m>=0]  XC*n2*B
@@ -31,7 +31,10 @@ __global__ __launch_bounds__(256) void k_fwd_ag(XformDev<T> X, const T* __restri
     for (int i = tid; i < XC * n2 * n2; i += blockDim.x) {
         const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
         const int64_t x = x0 + xl;
-        sf[i] = x < X.nx ? f[a + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x))] : T(0);
+        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
+        T v = x < X.nx ? f[iR + (int64_t)n2 * n2 * n2 * x] : T(0);
+        if (wR) v *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
+        sf[i] = v;
     }
     __syncthreads();
     // alpha DFT, m >= 0:  H = sum_a f e^{+i m alpha_a}
@@ -284,14 +287,14 @@ static int pick_xc(size_t per_x, size_t budget, int64_t nx) {
 }
 
 template <class T>
-void launch_fwd_ag(const XformDev<T>& X, const void* f, void* G, hipStream_t st) {
+void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
     const int n2 = 2 * X.B;
     const size_t per_x = sizeof(T) * n2 * n2 + sizeof(cT<T>) * n2 * X.B;
-    const int XC = pick_xc(per_x, kLdsLimit - sizeof(cT<T>) * n2, X.nx);
+    const int XC = pick_xc(per_x, kLdsTile - sizeof(cT<T>) * n2, X.nx);
     const size_t sh = per_x * XC + sizeof(cT<T>) * n2;
-    HPUP_REQUIRE(sh <= kLdsLimit, HPUP_ERR_UNSUPPORTED, "bandlimit too large for the forward alpha/gamma kernel");
+    allow_lds((const void*)k_fwd_ag<T>, sh);
     dim3 grid((unsigned)((X.nx + XC - 1) / XC), (unsigned)n2);
-    hipLaunchKernelGGL(k_fwd_ag<T>, grid, dim3(256), sh, st, X, (const T*)f, (cT<T>*)G, XC);
+    hipLaunchKernelGGL(k_fwd_ag<T>, grid, dim3(256), sh, st, X, (const T*)f, (const T*)wR, wstride, (cT<T>*)G, XC);
 }
 
 template <class T>
@@ -314,9 +317,9 @@ template <class T>
 void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
     const int n2 = 2 * X.B, M = 2 * X.B - 1;
     const size_t per_x = sizeof(cT<T>) * ((size_t)M * n2);
-    const int XC = pick_xc(per_x, kLdsLimit - sizeof(cT<T>) * n2, X.nx);
+    const int XC = pick_xc(per_x, kLdsTile - sizeof(cT<T>) * n2, X.nx);
     const size_t sh = per_x * XC + sizeof(cT<T>) * n2;
-    HPUP_REQUIRE(sh <= kLdsLimit, HPUP_ERR_UNSUPPORTED, "bandlimit too large for the inverse alpha/gamma kernel");
+    allow_lds((const void*)k_inv_ag<T>, sh);
     dim3 grid((unsigned)((X.nx + XC - 1) / XC), (unsigned)n2);
     hipLaunchKernelGGL(k_inv_ag<T>, grid, dim3(256), sh, st, X, (const cT<T>*)G, (T*)f, XC);
 }
@@ -332,11 +335,10 @@ void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hi
     const T sc2 = inverse ? T(1) / T(NN) : T(1);
     const int64_t nslab = X.nrc * (X.dim == 3 ? N : 1);
     const int nt = NN <= 256 ? 256 : 1024;
-    HPUP_REQUIRE(sizeof(cT<T>) * (2 * (size_t)NN + N) <= kLdsLimit, HPUP_ERR_UNSUPPORTED,
-                 "omega grid too large for the slab DFT kernel at this precision");
-    // twiddle table = forward/inverse row k=1 scaled back to unit modulus
-    hipLaunchKernelGGL(k_slab_dft<T>, dim3((unsigned)nslab), dim3(nt), sizeof(cT<T>) * (2 * (size_t)NN + N), st,
-                       (cT<T>*)F, inverse ? X.twNi : X.twNf, N, sc2);
+    const size_t sh = sizeof(cT<T>) * (2 * (size_t)NN + N);
+    allow_lds((const void*)k_slab_dft<T>, sh);
+    hipLaunchKernelGGL(k_slab_dft<T>, dim3((unsigned)nslab), dim3(nt), sh, st, (cT<T>*)F, inverse ? X.twNi : X.twNf,
+                       N, sc2);
     if (X.dim == 3)
         launch_axis_outer<T>(F, F, Mat, N, (int64_t)NN, X.nrc, T(1), st);
 }
@@ -359,7 +361,7 @@ void launch_compact_to_full(int B, int64_t nx, const void* compact, void* full, 
 }
 
 #define HPUP_INST(T)                                                                                  \
-    template void launch_fwd_ag<T>(const XformDev<T>&, const void*, void*, hipStream_t);             \
+    template void launch_fwd_ag<T>(const XformDev<T>&, const void*, const void*, int, void*, hipStream_t); \
     template void launch_beta_fwd<T>(const XformDev<T>&, const void*, void*, hipStream_t);           \
     template void launch_beta_inv<T>(const XformDev<T>&, const void*, void*, hipStream_t);           \
     template void launch_inv_ag<T>(const XformDev<T>&, const void*, void*, hipStream_t);             \

@@ -49,8 +49,8 @@ EXPORTS = [
     "hpup_last_error", "hpup_version", "hpup_parse_method",
     "hpup_fftSO3R_forward", "hpup_fftSO3R_backward",
     "hpup_pendulum_propagate_reduced", "hpup_pendulum_propagate_reduced32",
-    "hpup_pendulum_propagate", "hpup_pendulum_propagate32",
-    "hpup_getLambda", "hpup_getOmega", "hpup_getFcL", "hpup_getIndRule",
+    "hpup_pendulum_propagate", "hpup_pendulum_propagate32", "hpup_pendulum_propagate_reduced_den",
+    "hpup_getLambda", "hpup_getLambda_compact", "hpup_getOmega", "hpup_getFcL", "hpup_getIndRule",
     "hpup_discrete_propagate_noise", "hpup_discrete_propagate_nonoise",
     "hpup_plan_create", "hpup_plan_destroy", "hpup_plan_num_coeffs", "hpup_plan_num_grid",
     "hpup_plan_forward", "hpup_plan_backward", "hpup_plan_step", "hpup_plan_derivative",
@@ -177,12 +177,111 @@ def pendulum_propagate_reduced32(F, X, MR, bt, G, dtt, L, u, CG, method):
                       method)
 
 
+def pendulum_propagate_reduced_den(F, f, X, mR, bt, G, dtt, L, u, d, w, method):
+    """Fnew = pendulum_propagate_reduced_den(F, f, X, mR, bt, G, dtt, L, u, d, w, method)."""
+    F = _interleaved(F, np.complex128)
+    out = np.zeros_like(F, order="F")
+    _check(lib().hpup_pendulum_propagate_reduced_den(
+        _p(F), _dims(F), _p(np.asfortranarray(f, dtype=np.float64)), _p(_interleaved(X, np.complex128)),
+        _p(np.asfortranarray(mR, dtype=np.float64)), _p(np.ascontiguousarray(np.ravel(bt), dtype=np.float64)),
+        _p(np.asfortranarray(G, dtype=np.float64)), ctypes.c_double(dtt), ctypes.c_double(L),
+        _p(_interleaved(u, np.complex128)), _p(np.asfortranarray(d, dtype=np.float64)),
+        _p(np.ascontiguousarray(np.ravel(w), dtype=np.float64)), method.encode(), _p(out)))
+    return out
+
+
 def pendulum_propagate(F, X, OJO, MR, bt, G, dtt, L, u, CG, method):
     return _propagate(lib().hpup_pendulum_propagate, np.float64, F, X, OJO, MR, bt, G, dtt, L, u, CG, method)
 
 
 def pendulum_propagate32(F, X, OJO, MR, bt, G, dtt, L, u, CG, method):
     return _propagate(lib().hpup_pendulum_propagate32, np.float32, F, X, OJO, MR, bt, G, dtt, L, u, CG, method)
+
+
+# ------------------------------------------------------------ hybrid reset --
+# Reduced model (2-D omega).  Index arrays are 0-based int32/int16 as the
+# reference's C code produces them; x is the (2, N, N) omega grid.
+
+def getLambda_mex(R, x, d, h, r, theta_t, lambda_max, compact=False):
+    """[lambda, lambda_indR, PC] = getLambda_mex(..., false) or
+    [lambda, lambda_indR, lambda_indx, PC] = getLambda_mex(..., true)."""
+    R = np.asfortranarray(R, dtype=np.float64)
+    x = np.asfortranarray(x, dtype=np.float64)
+    B, N = R.shape[2] // 2, x.shape[1]
+    nR, nx = 8 * B ** 3, N * N
+    indR = np.zeros(nR, dtype=np.int32)
+    PC = np.zeros((3, nR), order="F")
+    numR = ctypes.c_int(0)
+    vals = [ctypes.c_double(v) for v in (d, h, r, theta_t, lambda_max)]
+    if not compact:
+        lam = np.zeros(nR * nx)
+        _check(lib().hpup_getLambda(_p(R), B, _p(x), N, *vals, _p(lam), _p(indR, _I32), ctypes.byref(numR), _p(PC)))
+        k = numR.value
+        return lam[: k * nx].reshape((k, N, N), order="F"), indR[:k].copy(), PC
+    lam = np.zeros(nR)
+    indx = np.zeros(nR * nx, dtype=np.int32)
+    numx = np.zeros(nR, dtype=np.int32)
+    _check(lib().hpup_getLambda_compact(_p(R), B, _p(x), N, *vals, _p(lam), _p(indR, _I32), ctypes.byref(numR),
+                                        _p(indx, _I32), _p(numx, _I32), _p(PC)))
+    k = numR.value
+    return lam[:k].copy(), indR[:k].copy(), [indx[i * nx: i * nx + numx[i]].copy() for i in range(k)], PC
+
+
+def getOmega_mex(R, x, indR, epsilon, PC, direction):
+    R = np.asfortranarray(R, dtype=np.float64)
+    x = np.asfortranarray(x, dtype=np.float64)
+    B, N = R.shape[2] // 2, x.shape[1]
+    indR = np.ascontiguousarray(indR, dtype=np.int32)
+    Om = np.zeros((2, len(indR), N, N), order="F")
+    _check(lib().hpup_getOmega(_p(R), B, _p(x), N, _p(indR, _I32), len(indR), ctypes.c_double(epsilon),
+                               _p(np.asfortranarray(PC, dtype=np.float64)), direction.encode(), _p(Om)))
+    return Om
+
+
+def getFcL_mex(x, Omega, lam, Gd, nD):
+    x = np.asfortranarray(x, dtype=np.float64)
+    N, numR = x.shape[1], Omega.shape[1]
+    fcL = np.zeros((nD, numR, N, N), order="F")
+    idx = np.zeros((nD, numR, N, N), dtype=np.int16, order="F")
+    _check(lib().hpup_getFcL(_p(x), N, _p(np.asfortranarray(Omega, dtype=np.float64)),
+                             _p(np.asfortranarray(lam, dtype=np.float64)), numR,
+                             _p(np.asfortranarray(Gd, dtype=np.float64)), int(nD), _p(fcL), _p(idx, _I16)))
+    return fcL, idx
+
+
+def getIndRule_mex(x, Omega):
+    x = np.asfortranarray(x, dtype=np.float64)
+    N, numR = x.shape[1], Omega.shape[1]
+    ind = np.zeros((4, numR, N, N), dtype=np.int32, order="F")
+    coeff = np.zeros((4, numR, N, N), order="F")
+    _check(lib().hpup_getIndRule(_p(x), N, _p(np.asfortranarray(Omega, dtype=np.float64)), numR, _p(ind, _I32),
+                                 _p(coeff)))
+    return ind, coeff
+
+
+def pendulum_reduced_discrete_propagate(noise, f, lam, indR, *args):
+    """df = pendulum_reduced_discrete_propagate(true, f, lambda, indR, fcL, fcL_indx) or
+    (false, f, lambda, indR, lambda_indx, ind_interp, coeff_interp)."""
+    f = np.asfortranarray(f, dtype=np.float64)
+    B, N = f.shape[0] // 2, f.shape[3]
+    indR = np.ascontiguousarray(indR, dtype=np.int32)
+    df = np.zeros_like(f, order="F")
+    if noise:
+        fcL, idx = args
+        _check(lib().hpup_discrete_propagate_noise(_p(f), B, N, _p(np.asfortranarray(lam, dtype=np.float64)),
+                                                   _p(indR, _I32), len(indR), _p(np.asfortranarray(fcL)),
+                                                   _p(np.asfortranarray(idx, dtype=np.int16), _I16), fcL.shape[0],
+                                                   _p(df)))
+        return df
+    lambda_indx, ind_interp, coeff_interp = args
+    keep = [np.ascontiguousarray(a, dtype=np.int32) for a in lambda_indx]
+    ptrs = (_I32 * max(len(keep), 1))(*[_p(a, _I32) for a in keep])
+    nums = np.array([len(a) for a in keep] or [0], dtype=np.int32)
+    _check(lib().hpup_discrete_propagate_nonoise(_p(f), B, N, _p(np.ascontiguousarray(lam, dtype=np.float64)),
+                                                 _p(indR, _I32), len(indR), ptrs, _p(nums, _I32),
+                                                 _p(np.asfortranarray(ind_interp, dtype=np.int32), _I32),
+                                                 _p(np.asfortranarray(coeff_interp, dtype=np.float64)), _p(df)))
+    return df
 
 
 # ------------------------------------------------------------------ plan --

@@ -101,6 +101,19 @@ int hpup_pendulum_propagate32(const float* F, const int64_t* dims_F, const float
                               const float* G, float dtt, float L, const float* u,
                               const float* const* CG, const char* method, float* Fnew);
 
+/* Fnew = pendulum_propagate_reduced_den(F, f, X, mR, bt, G, dtt, L, u, d, w, method)
+ * Replaces propagation/mex/sources/pendulum_propagate_reduced_den/kernel.cu:9-324
+ * (the density form the hybrid driver calls, pendulum_reduced_hybrid.m:228):
+ * the gravity moment is evaluated on the grid, -c(k_p) fftSO3R(f mR_p), with f
+ * the given grid density (2B,2B,2B,N,N) for the first stage and the inverse
+ * transform of the stage coefficients for RK substeps.  mR real (2,2B,2B,2B);
+ * d (2B-1,2B-1,B,2B) and w (2B) as for fftSO3R_mex.  FP64 only (the
+ * reference's FP32 build of this gateway does not compile). */
+int hpup_pendulum_propagate_reduced_den(const double* F, const int64_t* dims_F, const double* f, const double* X,
+                                        const double* mR, const double* bt, const double* G, double dtt, double L,
+                                        const double* u, const double* d, const double* w, const char* method,
+                                        double* Fnew);
+
 /* ---------------------------------------------------------------------------
  * Hybrid reset gateways (reduced model, 2-D omega grid).  Index arrays are
  * 0-based as the reference's C code produces and consumes them.
@@ -114,6 +127,15 @@ int hpup_pendulum_propagate32(const float* F, const int64_t* dims_F, const float
 int hpup_getLambda(const double* R, int B, const double* x, int N, double d, double h, double r,
                    double theta_t, double lambda_max, double* lambda, int32_t* indR, int* numR,
                    double* PC);
+
+/* [lambda, lambda_indR, lambda_indx, PC] = getLambda_mex(R, x, d, h, r, theta_t, lambda_max, true)
+ * Replaces .../getLambda.cu:94-190 (compact branch of mexLambda.cpp): lambda
+ * per collision-region rotation (numR), and for region k the ascending omega
+ * indices with non-negative contact velocity in indx[k*N*N + t], t < numx[k].
+ * lambda, indR, numx must hold nR entries, indx nR*N*N. */
+int hpup_getLambda_compact(const double* R, int B, const double* x, int N, double d, double h, double r,
+                           double theta_t, double lambda_max, double* lambda, int32_t* indR, int* numR,
+                           int32_t* indx, int32_t* numx, double* PC);
 
 /* Omega = getOmega_mex(R, x, indR, epsilon, PC, 'new'|'old')
  * Replaces .../getOmega.cu:6-137 (gateway mexOmega.cpp).  Omega real (2, numR, N, N). */

@@ -30,6 +30,7 @@ def lib():
         L.ref_derivative.restype = ctypes.c_int
         L.ref_integrate.restype = ctypes.c_int
         L.ref_getLambda.restype = ctypes.c_int
+        L.ref_getLambda_compact.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -176,6 +177,20 @@ def getLambda(B, N, R, x, dwall, h, r, theta_t, lambda_max):
     return lam[: numR * nx].reshape((numR, N, N), order="F"), indR[:numR].copy(), PC
 
 
+def getLambda_compact(B, N, R, x, dwall, h, r, theta_t, lambda_max):
+    nR, nx = 8 * B ** 3, N * N
+    lam = np.zeros(nR)
+    indR = np.zeros(nR, dtype=np.int32)
+    indx = np.zeros(nR * nx, dtype=np.int32)
+    numx = np.zeros(nR, dtype=np.int32)
+    PC = _f64((3, nR))
+    numR = lib().ref_getLambda_compact(B, N, _p(np.asfortranarray(R)), _p(np.asfortranarray(x)),
+                                       *(ctypes.c_double(v) for v in (dwall, h, r, theta_t, lambda_max)),
+                                       _p(lam), _p(indR, _I), _p(indx, _I), _p(numx, _I), _p(PC))
+    lists = [indx[k * nx: k * nx + numx[k]].copy() for k in range(numR)]
+    return lam[:numR].copy(), indR[:numR].copy(), lists, PC
+
+
 def getOmega(B, N, R, x, indR, epsilon, PC, direction):
     numR = len(indR)
     Om = _f64((2, numR, N, N))
@@ -237,3 +252,33 @@ def stat_moments(B, dim, N, L, f, R, x, w):
                            _p(np.asfortranarray(R)), _p(np.asfortranarray(x)),
                            _p(np.ascontiguousarray(w)), _p(ER), _p(Ex), _p(Varx), _p(fx))
     return ER, Ex, Varx, fx
+
+
+def derivative_den(B, N, F, f, X, mR, b, G, L, u, d, w):
+    F = np.asfortranarray(F, dtype=np.complex128)
+    dF = np.zeros_like(F, order="F")
+    fp = _p(np.asfortranarray(f, dtype=np.float64)) if f is not None else _D()
+    lib().ref_derivative_den(B, N, _p(F), fp, _p(np.asfortranarray(X)), _p(np.asfortranarray(mR)),
+                             _p(np.ascontiguousarray(b, dtype=np.float64)), _p(np.asfortranarray(G, dtype=np.float64)),
+                             ctypes.c_double(L), _p(np.asfortranarray(u)), _p(np.asfortranarray(d)),
+                             _p(np.ascontiguousarray(w)), _p(dF))
+    return dF
+
+
+def integrate_den(B, N, F, f, X, mR, b, G, dt, L, u, d, w, method):
+    F = np.asfortranarray(F, dtype=np.complex128)
+    Fn = np.zeros_like(F, order="F")
+    rc = lib().ref_integrate_den(B, N, _p(F), _p(np.asfortranarray(f, dtype=np.float64)), _p(np.asfortranarray(X)),
+                                 _p(np.asfortranarray(mR)), _p(np.ascontiguousarray(b, dtype=np.float64)),
+                                 _p(np.asfortranarray(G, dtype=np.float64)), ctypes.c_double(dt), ctypes.c_double(L),
+                                 _p(np.asfortranarray(u)), _p(np.asfortranarray(d)), _p(np.ascontiguousarray(w)),
+                                 method.encode(), _p(Fn))
+    if rc != 0:
+        raise ValueError("unknown method %r" % method)
+    return Fn
+
+
+def grid_mR(B):
+    """mR = [R32; -R31] on the rotation grid, (2, 2B, 2B, 2B) (pendulum_reduced_hybrid.m:115)."""
+    R = rotations(B)
+    return np.asfortranarray(np.stack([R[2, 1], -R[2, 0]], axis=0))

@@ -472,10 +472,11 @@ int ref_derivative(int B, int dim, int N, const double* F, const double* X, cons
     long nxs = x1 - x0;
     /* gravity: Clebsch-Gordan product (pendulum.m:470-506).  As in the
      * reference, K = CG_m.' * MR^{l2} * CG_n is formed once per
-     * (l, l2, l1, m, n) and applied to every omega point. */
+     * (l, l2, l1, m, n) and applied to every omega point.  CG = NULL skips
+     * the term (the density-form variant evaluates it on the grid). */
     cplx* tG = (cplx*)calloc((size_t)nrot * nxs * dim, sizeof(cplx));
-#pragma omp parallel
-    {
+#pragma omp parallel if (CG != NULL)
+    if (CG) {
         cplx* KM = (cplx*)malloc(sizeof(cplx) * M * M);
         cplx* K = (cplx*)malloc(sizeof(cplx) * M * M);
 #pragma omp for schedule(dynamic, 1)
@@ -699,6 +700,35 @@ int ref_getLambda(int B, int N, const double* R, const double* x, double dwall, 
     return numR;
 }
 
+/* getLambda.cu:94-190 (compact = true): lambda per region rotation and, per
+ * region, the ascending omega indices whose contact velocity is >= 0. */
+int ref_getLambda_compact(int B, int N, const double* R, const double* x, double dwall, double h, double r,
+                          double theta_t, double lambda_max, double* lambda, int* indR, int* indx, int* numx,
+                          double* PC)
+{
+    long nR = 8L * B * B * B, nx = (long)N * N;
+    double* full = (double*)malloc(sizeof(double) * nR * nx);
+    int numR = ref_getLambda(B, N, R, x, dwall, h, r, theta_t, lambda_max, full, indR, PC);
+    double theta0 = asin(dwall / sqrt(h * h + r * r)) - asin(r / sqrt(h * h + r * r));
+    for (int k = 0; k < numR; k++) {
+        long iR = indR[k];
+        double th = asin(R[9 * iR + 6]);
+        lambda[k] = th > theta0 + theta_t ? lambda_max
+                                          : lambda_max / 2 * sin(PI / (2 * theta_t) * (th - theta0)) + lambda_max / 2;
+        const double* Rr = R + 9 * iR;
+        int c = 0;
+        for (long ix = 0; ix < nx; ix++) {
+            double o1 = Rr[1] * x[2 * ix] + Rr[4] * x[2 * ix + 1];
+            double o2 = Rr[2] * x[2 * ix] + Rr[5] * x[2 * ix + 1];
+            double vC = o1 * PC[3 * iR + 2] - o2 * PC[3 * iR + 1];
+            if (!(vC < 0.0)) indx[(long)k * nx + c++] = (int)ix;
+        }
+        numx[k] = c;
+    }
+    free(full);
+    return numR;
+}
+
 /* getOmega.cu:64-137: tangent t = r3 x e1 / |.|; 'new': omega+ = omega -
  * (1+eps)(omega.t)t; 'old': (1+eps)/eps, NaN when the contact velocity sign
  * is invalid or the result leaves the grid box.  Omega is (2, numR, nx). */
@@ -910,4 +940,91 @@ void ref_stat_moments(int B, int dim, int N, double L, const double* f, const do
             for (long ix = 0; ix < nx; ix++) s += x[dim * ix + p] * x[dim * ix + q] * fx[ix];
             Varx[p + dim * q] = s * dv - Ex[p] * Ex[q];
         }
+}
+
+/* ------------------------------------------------- density-form variant ---- */
+
+/* pendulum_reduced_den.m:397-533: the reduced derivative with the gravity
+ * moment evaluated pseudo-spectrally, -sum_p c(k_p) fftSO3R(f mR_p), on the
+ * grid density f; f = NULL reconstructs it from F by the inverse transform
+ * (:453-476; the reference keeps the complex ifftn result there, its
+ * imaginary part is rounding for a real-consistent F, this restatement keeps
+ * the real part).  The other terms are ref_derivative's with MR = 0. */
+int ref_derivative_den(int B, int N, const double* F, const double* f, const double* X, const double* mR,
+                       const double* b, const double* G, double L, const double* u, const double* d,
+                       const double* w, double* dF)
+{
+    long M = 2 * B - 1, nrot = M * M * B, nx = (long)N * N, n2 = 2 * B, nR = n2 * n2 * n2, Bx = N / 2;
+    double* MR0 = (double*)calloc((size_t)2 * nrot * 2, sizeof(double));
+    ref_derivative(B, 2, N, F, X, NULL, MR0, b, G, L, u, NULL, dF, 0, 0);
+    double* fg = NULL;
+    if (!f) {
+        fg = (double*)malloc(sizeof(double) * nR * nx);
+        ref_fftSO3R_backward(B, 2, N, F, d, fg);
+        f = fg;
+    }
+    double* prod = (double*)malloc(sizeof(double) * nR * nx);
+    double* FM = (double*)malloc(sizeof(double) * 2 * nrot * nx);
+    cplx* dFc = (cplx*)dF;
+    const cplx* FMc = (const cplx*)FM;
+    for (int p = 0; p < 2; p++) {
+        for (long x = 0; x < nx; x++)
+            for (long iR = 0; iR < nR; iR++) prod[iR + nR * x] = f[iR + nR * x] * mR[p + 2 * iR];
+        ref_fftSO3R_forward(B, 2, N, prod, d, w, FM);
+        for (long x = 0; x < nx; x++) {
+            long kp = p == 0 ? x % N : x / N;
+            cplx c = deriv_x(kp, Bx, L);
+            for (long r = 0; r < nrot; r++) dFc[r + nrot * x] -= FMc[r + nrot * x] * c;
+        }
+    }
+    free(prod); free(FM); free(fg); free(MR0);
+    return 0;
+}
+
+/* pendulum_reduced_den.m:359-395: the first stage uses the given f, the RK
+ * substeps reconstruct it from their coefficients. */
+int ref_integrate_den(int B, int N, const double* F, const double* f, const double* X, const double* mR,
+                      const double* b, const double* G, double dt, double L, const double* u, const double* d,
+                      const double* w, const char* method, double* Fnew)
+{
+    long nrot = (long)(2 * B - 1) * (2 * B - 1) * B;
+    long n = 2 * nrot * (long)N * N;
+    int meth;
+    if (!strcasecmp(method, "euler")) meth = 0;
+    else if (!strcasecmp(method, "midpoint")) meth = 1;
+    else if (!strcasecmp(method, "RK2")) meth = 2;
+    else if (!strcasecmp(method, "RK4")) meth = 3;
+    else return -1;
+    double* k1 = (double*)malloc(sizeof(double) * n);
+    ref_derivative_den(B, N, F, f, X, mR, b, G, L, u, d, w, k1);
+    if (meth == 0) {
+        for (long i = 0; i < n; i++) Fnew[i] = F[i] + dt * k1[i];
+        free(k1);
+        return 0;
+    }
+    double* F2 = (double*)malloc(sizeof(double) * n);
+    double* k2 = (double*)malloc(sizeof(double) * n);
+    if (meth == 2) {
+        for (long i = 0; i < n; i++) F2[i] = F[i] + k1[i] * dt;
+        ref_derivative_den(B, N, F2, NULL, X, mR, b, G, L, u, d, w, k2);
+        for (long i = 0; i < n; i++) Fnew[i] = F[i] + dt / 2 * (k1[i] + k2[i]);
+        free(k1); free(F2); free(k2);
+        return 0;
+    }
+    for (long i = 0; i < n; i++) F2[i] = F[i] + k1[i] * dt / 2;
+    ref_derivative_den(B, N, F2, NULL, X, mR, b, G, L, u, d, w, k2);
+    if (meth == 1) {
+        for (long i = 0; i < n; i++) Fnew[i] = F[i] + dt * k2[i];
+        free(k1); free(F2); free(k2);
+        return 0;
+    }
+    double* k3 = (double*)malloc(sizeof(double) * n);
+    double* k4 = (double*)malloc(sizeof(double) * n);
+    for (long i = 0; i < n; i++) F2[i] = F[i] + k2[i] * dt / 2;
+    ref_derivative_den(B, N, F2, NULL, X, mR, b, G, L, u, d, w, k3);
+    for (long i = 0; i < n; i++) F2[i] = F[i] + k3[i] * dt;
+    ref_derivative_den(B, N, F2, NULL, X, mR, b, G, L, u, d, w, k4);
+    for (long i = 0; i < n; i++) Fnew[i] = F[i] + 1.0 / 6.0 * dt * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
+    free(k1); free(F2); free(k2); free(k3); free(k4);
+    return 0;
 }

@@ -45,10 +45,22 @@ int  ref_integrate(int B, int dim, int N, const double* F, const double* X,
                    const double* G, double dt, double L, const double* u,
                    const double* const* CG, const char* method, double* Fnew);
 
+/* density-form reduced derivative and integrator (pendulum_reduced_den.m) */
+int  ref_derivative_den(int B, int N, const double* F, const double* f, const double* X,
+                        const double* mR, const double* b, const double* G, double L,
+                        const double* u, const double* d, const double* w, double* dF);
+int  ref_integrate_den(int B, int N, const double* F, const double* f, const double* X,
+                       const double* mR, const double* b, const double* G, double dt, double L,
+                       const double* u, const double* d, const double* w, const char* method,
+                       double* Fnew);
+
 /* hybrid reset, reduced (2-D omega) model */
 int  ref_getLambda(int B, int N, const double* R, const double* x, double dwall,
                    double h, double r, double theta_t, double lambda_max,
                    double* lambda_full, int* indR, double* PC);
+int  ref_getLambda_compact(int B, int N, const double* R, const double* x, double dwall,
+                           double h, double r, double theta_t, double lambda_max,
+                           double* lambda, int* indR, int* indx, int* numx, double* PC);
 void ref_getOmega(int B, int N, const double* R, const double* x, const int* indR,
                   int numR, double epsilon, const double* PC, int old_dir,
                   double* Omega);

@@ -128,6 +128,24 @@ def test_bad_method_is_an_error():
         hpup.pendulum_propagate_reduced(F, X, MR, bt, G, 0.01, L, u, cells, "leapfrog")
 
 
+@pytest.mark.parametrize("B,N,method", [(4, 8, "euler"), (4, 8, "RK4"), (3, 34, "RK2")])
+def test_propagate_reduced_den(B, N, method):
+    """Density-form gateway (gravity on the grid, pendulum_reduced_den.m); N = 34
+    runs the large-grid kernels with more than 64 KB of LDS."""
+    _print("case reduced den", B, N, method)
+    L = 3.4
+    f = density(B, 2, N, L, (8, 4, 2), seed=6, Sigma=np.eye(2) * 0.6)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    F = O.fft_forward(B, f, d, w, 2, N)
+    X, _, _, u, _ = operators(B, 2, N, L)
+    mR = O.grid_mR(B)
+    bt, G, dt = np.array([0.2, 0.25]), np.diag([0.1, 0.15]), 0.01
+    ref = O.integrate_den(B, N, F, f, X, mR, bt, G, dt, L, u, d, w, method)
+    out = hpup.pendulum_propagate_reduced_den(F, f, X, mR, bt, G, dt, L, u, d, w, method)
+    assert rel_l2(out, ref) < 1e-10
+
+
 def test_fallback_path_large_omega_grid():
     """N > 32 runs the LDS-tiled FMA kernels instead of the MFMA ones."""
     _print("case reduced N=34 fallback")
@@ -138,7 +156,7 @@ def test_fallback_path_large_omega_grid():
     assert rel_l2(out, ref) < 1e-10
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5), ("bf16x3", 1e-4)])
 def test_plan_derivative_and_mass(precision, tol):
     """Device-resident plan: derivative against the oracle (plan built from the
     physical parameters, so it also checks the drivers' X/OJO/MR recipe), and
