@@ -11,6 +11,7 @@
 
 #include "drift.h"
 #include "hpup_internal.h"
+#include "matfisher.h"
 #include "stats.h"
 
 namespace hpup {
@@ -178,6 +179,57 @@ template <class T> struct DriftTables {
 
 // ---------------------------------------------------------------- plan --
 
+// One device pass of get_stat over the grid f (precision T): fR[nR],
+// gR[dim][nR], fx[nx] copied to the host.  work/out grow as needed.
+template <class T>
+static void grid_marginals(int B, int dim, int64_t nx, const void* f, const double* w_dev, const double* xg_dev,
+                           DevBuf& work, DevBuf& out, std::vector<double>& hR, std::vector<double>& hG,
+                           std::vector<double>& hx, hipStream_t st) {
+    const StatShape s = stat_shape(B, dim, nx);
+    HPUP_REQUIRE(dim == 0 || xg_dev, HPUP_ERR_ARG, "omega coordinates missing");
+    const size_t nout = (size_t)(s.nR * (1 + dim) + nx);
+    if (work.bytes < sizeof(double) * s.work_doubles) work.alloc(sizeof(double) * s.work_doubles);
+    if (out.bytes < sizeof(double) * nout) out.alloc(sizeof(double) * nout);
+    double* o = out.as<double>();
+    launch_stat_pass<T>(s, f, B, w_dev, xg_dev, work.as<double>(), o, o + s.nR, o + s.nR * (1 + dim), st);
+    HPUP_HIP(hipGetLastError());
+    hR.resize(s.nR);
+    hG.resize((size_t)dim * s.nR);
+    hx.resize(nx);
+    HPUP_HIP(hipMemcpyAsync(hR.data(), o, sizeof(double) * s.nR, hipMemcpyDeviceToHost, st));
+    if (dim) HPUP_HIP(hipMemcpyAsync(hG.data(), o + s.nR, sizeof(double) * dim * s.nR, hipMemcpyDeviceToHost, st));
+    HPUP_HIP(hipMemcpyAsync(hx.data(), o + s.nR * (1 + dim), sizeof(double) * nx, hipMemcpyDeviceToHost, st));
+    HPUP_HIP(hipStreamSynchronize(st));
+}
+
+// R = Rz(alpha) Ry(beta) Rz(gamma) on the grid alpha_i = pi i/B,
+// beta_j = pi (2j+1)/(4B), gamma_k = pi k/B (fftSO3R.m:38-49), column-major.
+struct EulerGrid {
+    int n2;
+    std::vector<double> ca, sa, cb, sb;
+    explicit EulerGrid(int B) : n2(2 * B), ca(n2), sa(n2), cb(n2), sb(n2) {
+        for (int i = 0; i < n2; i++) {
+            ca[i] = std::cos(kPi * i / B);
+            sa[i] = std::sin(kPi * i / B);
+            cb[i] = std::cos(kPi * (2 * i + 1) / (4.0 * B));
+            sb[i] = std::sin(kPi * (2 * i + 1) / (4.0 * B));
+        }
+    }
+    void operator()(int64_t iR, double* R) const {
+        const int64_t i = iR % n2, j = (iR / n2) % n2, k = iR / ((int64_t)n2 * n2);
+        const double a = ca[i], s = sa[i], b = cb[j], t = sb[j], c = ca[k], u = sa[k];
+        R[0] = a * b * c - s * u;
+        R[1] = s * b * c + a * u;
+        R[2] = -t * c;
+        R[3] = -a * b * u - s * c;
+        R[4] = -s * b * u + a * c;
+        R[5] = t * u;
+        R[6] = a * t;
+        R[7] = s * t;
+        R[8] = b;
+    }
+};
+
 struct PlanBase {
     hpup_model model{};
     int64_t nx = 1, nrc = 0, ncoef = 0, ngrid = 0;
@@ -190,7 +242,13 @@ struct PlanBase {
     virtual void derivative(const void* F, void* dF, hipStream_t st) = 0;
     virtual void full_to_compact(const void* a, void* b, hipStream_t st) = 0;
     virtual void compact_to_full(const void* a, void* b, hipStream_t st) = 0;
-    virtual void stat(const void* f, double* ER, double* Ex, double* Varx, double* fx, hipStream_t st) = 0;
+    // device pass of get_stat: rotation marginal, first omega moment per
+    // rotation, omega marginal (host copies)
+    virtual void marginals(const void* f, const double* xg, std::vector<double>& fR, std::vector<double>& gR,
+                           std::vector<double>& fx, hipStream_t st) = 0;
+    std::vector<double> xg_host;  // omega coordinates [ix][a]
+    DevBuf xg_dev;
+    void get_stat(const void* f, hpup_stat& out, double* fx, bool fit, hipStream_t st);
     virtual int profile(void* F, int nreps, double* ms, int maxk, hipStream_t st) = 0;
     virtual const char* profile_name(int i) const = 0;
     virtual double profile_bytes(int i) const = 0;
@@ -201,7 +259,7 @@ template <class T> struct PlanT : PlanBase {
     DriftTables<T> dr;
     bool has_drift = false;
     bool bf16x3 = false;  // circulants as 3xBF16 split products (FP32 storage)
-    DevBuf G, Wb, V1, V2, Z3, Fs, ACC, tmp, fxb, fRb;
+    DevBuf G, Wb, V1, V2, Z3, Fs, ACC, tmp, st_work, st_out;
 
     void init_xform(const double* d, const double* wv) {
         const hpup_model& m = model;
@@ -388,53 +446,29 @@ template <class T> struct PlanT : PlanBase {
         launch_compact_to_full<T>(model.B, nx, a, b, st);
         HPUP_HIP(hipGetLastError());
     }
-    void stat(const void* f, double* ER, double* Ex, double* Varx, double* fx, hipStream_t st) override {
-        const int B = model.B, n2 = 2 * B, dim = model.dim, N = std::max(model.N, 1);
-        const int64_t nR = (int64_t)n2 * n2 * n2;
-        ensure(fxb, sizeof(double) * nx);
-        ensure(fRb, sizeof(double) * nR);
-        launch_marginals<T>(f, B, nx, w_dev.as<double>(), fxb.as<double>(), fRb.as<double>(), st);
-        HPUP_HIP(hipGetLastError());
-        std::vector<double> hx(nx), hR(nR);
-        HPUP_HIP(hipMemcpyAsync(hx.data(), fxb.p, sizeof(double) * nx, hipMemcpyDeviceToHost, st));
-        HPUP_HIP(hipMemcpyAsync(hR.data(), fRb.p, sizeof(double) * nR, hipMemcpyDeviceToHost, st));
-        HPUP_HIP(hipStreamSynchronize(st));
-        const double L = model.L;
-        const double dv = dim > 0 ? std::pow(L / N, dim) : 1.0;
-        // E[R] = sum_R R w_j fR dv  (R = Rz(alpha) Ry(beta) Rz(gamma), fftSO3R.m:38-49)
-        double acc[9] = {0};
-        for (int64_t k = 0; k < n2; k++)
-            for (int64_t j = 0; j < n2; j++)
-                for (int64_t i = 0; i < n2; i++) {
-                    const double a = kPi / B * i, b = kPi / (4.0 * B) * (2 * j + 1), g = kPi / B * k;
-                    const double ca = std::cos(a), sa = std::sin(a), cb = std::cos(b), sb = std::sin(b),
-                                 cg = std::cos(g), sg = std::sin(g);
-                    const double R[9] = {ca * cb * cg - sa * sg, sa * cb * cg + ca * sg, -sb * cg,
-                                         -ca * cb * sg - sa * cg, -sa * cb * sg + ca * cg, sb * sg,
-                                         ca * sb, sa * sb, cb};
-                    const double wt = hR[i + n2 * (j + n2 * k)] * w[j] * dv;
-                    for (int q = 0; q < 9; q++) acc[q] += R[q] * wt;
-                }
-        if (ER) std::memcpy(ER, acc, sizeof(acc));
-        if (fx) std::memcpy(fx, hx.data(), sizeof(double) * nx);
-        if (dim > 0) {
-            double E[3] = {0, 0, 0}, S[9] = {0};
-            for (int64_t x = 0; x < nx; x++) {
-                double v[3];
-                for (int p = 0; p < dim; p++) v[p] = -L / 2 + L / N * ((x / ipow64(N, p)) % N);
-                for (int p = 0; p < dim; p++) {
-                    E[p] += v[p] * hx[x];
-                    for (int q = 0; q < dim; q++) S[p + dim * q] += v[p] * v[q] * hx[x];
-                }
-            }
-            for (int p = 0; p < dim; p++) E[p] *= dv;
-            if (Ex) for (int p = 0; p < dim; p++) Ex[p] = E[p];
-            if (Varx)
-                for (int p = 0; p < dim; p++)
-                    for (int q = 0; q < dim; q++) Varx[p + dim * q] = S[p + dim * q] * dv - E[p] * E[q];
-        }
+    void marginals(const void* f, const double* xg, std::vector<double>& hR, std::vector<double>& hG,
+                   std::vector<double>& hx, hipStream_t st) override {
+        grid_marginals<T>(model.B, model.dim, nx, f, w_dev.as<double>(), xg, st_work, st_out, hR, hG, hx, st);
     }
 };
+
+void PlanBase::get_stat(const void* f, hpup_stat& out, double* fx, bool fit, hipStream_t st) {
+    const int B = model.B, dim = model.dim, N = std::max(model.N, 1);
+    if (dim > 0 && !xg_dev.p) {
+        xg_host.assign((size_t)dim * nx, 0.0);
+        for (int64_t ix = 0; ix < nx; ix++)
+            for (int a = 0; a < dim; a++)
+                xg_host[(size_t)dim * ix + a] = -model.L / 2 + model.L / N * (double)((ix / ipow64(N, a)) % N);
+        xg_dev = upload(xg_host);
+    }
+    std::vector<double> hR, hG, hx;
+    marginals(f, xg_dev.as<double>(), hR, hG, hx, st);
+    const double dv = dim > 0 ? std::pow(model.L / N, dim) : 1.0;
+    const EulerGrid eg(B);
+    mf::finish_stat(dim, (int64_t)hR.size(), nx, 2 * B, w.data(), hR.data(), hG.data(), hx.data(), xg_host.data(),
+                    dv, eg, fit, out);
+    if (fx) std::memcpy(fx, hx.data(), sizeof(double) * nx);
+}
 
 static PlanBase* make_plan(const hpup_model& m, const double* d, const double* w, const DriftSpec* spec) {
     HPUP_REQUIRE(m.B >= 1 && m.B <= 128, HPUP_ERR_ARG, "B out of range");
@@ -562,7 +596,92 @@ double hpup_plan_profile_bytes(const hpup_plan* p, int i) { return p ? p->impl->
 
 int hpup_plan_stat_moments(hpup_plan* p, const void* f, double* ER, double* Ex, double* Varx, double* fx,
                            void* stream) {
-    return guard([&] { p->impl->stat(f, ER, Ex, Varx, fx, (hipStream_t)stream); });
+    return guard([&] {
+        HPUP_REQUIRE(p && f, HPUP_ERR_ARG, "null argument");
+        hpup_stat s;
+        p->impl->get_stat(f, s, fx, false, (hipStream_t)stream);
+        const int dim = p->impl->model.dim;
+        if (ER) std::memcpy(ER, s.ER, sizeof(s.ER));
+        if (Ex) std::memcpy(Ex, s.Ex, sizeof(double) * dim);
+        if (Varx) std::memcpy(Varx, s.Varx, sizeof(double) * dim * dim);
+    });
+}
+
+int hpup_plan_get_stat(hpup_plan* p, const void* f, hpup_stat* out, double* fx, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p && f && out, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(p->impl->model.dim > 0, HPUP_ERR_ARG, "get_stat needs an omega grid (dim 2 or 3)");
+        p->impl->get_stat(f, *out, fx, true, (hipStream_t)stream);
+    });
+}
+
+// --------------------------------------------- gateways: get_stat, fit --
+
+int hpup_get_stat(const double* f, const int64_t* dims, int ndims, const double* R, const double* x,
+                  const double* w, hpup_stat* out, double* fx) {
+    return guard([&] {
+        HPUP_REQUIRE(f && dims && R && x && w && out, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(ndims == 5 || ndims == 6, HPUP_ERR_ARG, "f must have rank 5 or 6");
+        HPUP_REQUIRE(dims[0] % 2 == 0 && dims[0] > 0 && dims[0] == dims[1] && dims[1] == dims[2], HPUP_ERR_ARG,
+                     "f must be (2B,2B,2B,N,...)");
+        const int B = (int)(dims[0] / 2), dim = ndims - 3, N = (int)dims[3];
+        HPUP_REQUIRE(N >= 2, HPUP_ERR_ARG, "omega grid needs N >= 2");
+        for (int i = 4; i < ndims; i++) HPUP_REQUIRE(dims[i] == N, HPUP_ERR_ARG, "omega axes must be equal");
+        const int64_t nx = ipow64(N, dim), n2 = 2 * B, nR = n2 * n2 * n2;
+        // L = x(1,end,1,1) + x(1,2,1,1) - 2 x(1,1,1,1), cell (L/N)^dim (pendulum.m:549-551)
+        const double L = x[(size_t)dim * (N - 1)] + x[dim] - 2 * x[0];
+        const double dv = std::pow(L / N, dim);
+        DevBuf fd(sizeof(double) * nR * nx), wd, xd, work, ob;
+        HPUP_HIP(hipMemcpy(fd.p, f, fd.bytes, hipMemcpyHostToDevice));
+        wd = upload(std::vector<double>(w, w + n2));
+        std::vector<double> xh(x, x + (size_t)dim * nx);
+        xd = upload(xh);
+        std::vector<double> hR, hG, hx;
+        grid_marginals<double>(B, dim, nx, fd.p, wd.as<double>(), xd.as<double>(), work, ob, hR, hG, hx, nullptr);
+        auto Rof = [R](int64_t iR, double* r) { std::memcpy(r, R + 9 * iR, 9 * sizeof(double)); };
+        mf::finish_stat(dim, nR, nx, (int)n2, w, hR.data(), hG.data(), hx.data(), xh.data(), dv, Rof, true, *out);
+        if (fx) std::memcpy(fx, hx.data(), sizeof(double) * nx);
+    });
+}
+
+int hpup_psvd(const double* F, double* U, double* S, double* V) {
+    return guard([&] {
+        HPUP_REQUIRE(F && U && S && V, HPUP_ERR_ARG, "null argument");
+        double s[3];
+        mf::psvd(F, U, s, V);
+        for (int i = 0; i < 9; i++) S[i] = 0.0;
+        for (int i = 0; i < 3; i++) S[4 * i] = s[i];
+    });
+}
+
+int hpup_pdf_MF_normal(const double* s, int bool_scaled, double* c, double* dc, double* ddc) {
+    return guard([&] {
+        HPUP_REQUIRE(s && c, HPUP_ERR_ARG, "null argument");
+        const mf::Normalizer n = mf::normalizer(s);
+        // c = C e^K, dc = D e^K, ddc = H e^K; the scaled form multiplies by
+        // e^{-sum s} and differentiates that product
+        const double sig = s[0] + s[1] + s[2];
+        const double e = std::exp(n.K - (bool_scaled ? sig : 0.0));
+        *c = n.C * e;
+        if (dc)
+            for (int i = 0; i < 3; i++) dc[i] = (bool_scaled ? n.D[i] - n.C : n.D[i]) * e;
+        if (ddc)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    ddc[i + 3 * j] =
+                        (bool_scaled ? n.H[i + 3 * j] - n.D[i] - n.D[j] + n.C : n.H[i + 3 * j]) * e;
+    });
+}
+
+int hpup_pdf_MF_M2S(const double* d, const double* s0, double* s, double* fval, int* niter) {
+    return guard([&] {
+        HPUP_REQUIRE(d && s, HPUP_ERR_ARG, "null argument");
+        const double z[3] = {0, 0, 0};
+        double fv[3];
+        const int it = mf::fit_s(d, s0 ? s0 : z, s, fv);
+        if (fval) std::memcpy(fval, fv, sizeof(fv));
+        if (niter) *niter = it;
+    });
 }
 
 // ------------------------------------------------------- gateways: FFT --

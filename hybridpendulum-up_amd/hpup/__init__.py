@@ -36,6 +36,23 @@ class HpupError(RuntimeError):
     pass
 
 
+class StatOut(ctypes.Structure):
+    """hpup_stat: get_stat outputs, 3x3 blocks column-major (omega blocks with
+    leading dimension dim)."""
+    _fields_ = [(n, ctypes.c_double * k) for n, k in
+                (("ER", 9), ("Ex", 3), ("Varx", 9), ("EvR", 3), ("ExvR", 9), ("EvRvR", 9),
+                 ("U", 9), ("S", 9), ("V", 9), ("P", 9), ("Miu", 3), ("Sigma", 9))] + \
+              [("niter", ctypes.c_int), ("fit_ok", ctypes.c_int)]
+
+    def as_dict(self, dim):
+        m = lambda n, r, c: np.array(getattr(self, n)[: r * c]).reshape((r, c), order="F")
+        return {"ER": m("ER", 3, 3), "Ex": np.array(self.Ex[:dim]), "Varx": m("Varx", dim, dim),
+                "EvR": np.array(self.EvR[:3]), "ExvR": m("ExvR", dim, 3), "EvRvR": m("EvRvR", 3, 3),
+                "U": m("U", 3, 3), "S": m("S", 3, 3), "V": m("V", 3, 3), "P": m("P", dim, 3),
+                "Miu": np.array(self.Miu[:dim]), "Sigma": m("Sigma", dim, dim),
+                "niter": int(self.niter), "fit_ok": bool(self.fit_ok)}
+
+
 class Model(ctypes.Structure):
     _fields_ = [("B", ctypes.c_int), ("dim", ctypes.c_int), ("N", ctypes.c_int), ("L", ctypes.c_double),
                 ("b", ctypes.c_double * 3), ("G", ctypes.c_double * 9), ("Jt", ctypes.c_double * 9),
@@ -56,6 +73,7 @@ EXPORTS = [
     "hpup_plan_forward", "hpup_plan_backward", "hpup_plan_step", "hpup_plan_derivative",
     "hpup_plan_full_to_compact", "hpup_plan_compact_to_full", "hpup_plan_stat_moments",
     "hpup_plan_profile", "hpup_plan_profile_name", "hpup_plan_profile_bytes",
+    "hpup_get_stat", "hpup_psvd", "hpup_pdf_MF_normal", "hpup_pdf_MF_M2S", "hpup_plan_get_stat",
 ]
 
 
@@ -84,6 +102,12 @@ def lib():
         L.hpup_plan_profile_name.restype = ctypes.c_char_p
         L.hpup_plan_profile_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.hpup_plan_profile_bytes.restype = ctypes.c_double
+        L.hpup_plan_get_stat.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(StatOut), _D,
+                                         ctypes.c_void_p]
+        L.hpup_get_stat.argtypes = [_D, _I64, ctypes.c_int, _D, _D, _D, ctypes.POINTER(StatOut), _D]
+        L.hpup_psvd.argtypes = [_D, _D, _D, _D]
+        L.hpup_pdf_MF_normal.argtypes = [_D, ctypes.c_int, _D, _D, _D]
+        L.hpup_pdf_MF_M2S.argtypes = [_D, _D, _D, _D, ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -286,6 +310,54 @@ def pendulum_reduced_discrete_propagate(noise, f, lam, indR, *args):
 
 # ------------------------------------------------------------------ plan --
 
+# ---------------------------------------------------- statistics, fit --
+
+def get_stat(f, R, x, w, return_fx=False):
+    """[ER, Ex, Varx, EvR, ExvR, EvRvR, U, S, V, P, Miu, Sigma(, fx)] = get_stat(f, R, x, w)
+    (the drivers' local function, pendulum.m:547-594) as a dict; the grid
+    sums run on the GPU."""
+    f = np.asfortranarray(f, dtype=np.float64)
+    dim = f.ndim - 3
+    out = StatOut()
+    fx = np.zeros(f.shape[3:], order="F")
+    _check(lib().hpup_get_stat(_p(f), _dims(f), f.ndim, _p(np.asfortranarray(R, dtype=np.float64)),
+                               _p(np.asfortranarray(x, dtype=np.float64)),
+                               _p(np.ascontiguousarray(np.ravel(w), dtype=np.float64)), ctypes.byref(out), _p(fx)))
+    r = out.as_dict(dim)
+    if return_fx:
+        r["fx"] = fx
+    return r
+
+
+def psvd(F):
+    """[U, S, V] = psvd(F) (matrix Fisher/psvd.m)."""
+    F = np.asfortranarray(F, dtype=np.float64)
+    U, S, V = (np.zeros((3, 3), order="F") for _ in range(3))
+    _check(lib().hpup_psvd(_p(F), _p(U), _p(S), _p(V)))
+    return U, S, V
+
+
+def pdf_MF_normal(s, bool_scaled=False):
+    """[c, dc, ddc] = pdf_MF_normal(s, bool_scaled, 1, 1)."""
+    s = np.ascontiguousarray(np.ravel(s), dtype=np.float64)
+    c = np.zeros(1)
+    dc = np.zeros(3)
+    ddc = np.zeros((3, 3), order="F")
+    _check(lib().hpup_pdf_MF_normal(_p(s), int(bool(bool_scaled)), _p(c), _p(dc), _p(ddc)))
+    return float(c[0]), dc, ddc
+
+
+def pdf_MF_M2S(d, s0=None):
+    """[s, FVAL, NITER] = pdf_MF_M2S(d, s0); NITER = -1 if not converged."""
+    d = np.ascontiguousarray(np.ravel(d), dtype=np.float64)
+    s = np.zeros(3)
+    fval = np.zeros(3)
+    it = ctypes.c_int(0)
+    s0p = _p(np.ascontiguousarray(np.ravel(s0), dtype=np.float64)) if s0 is not None else _D()
+    _check(lib().hpup_pdf_MF_M2S(_p(d), s0p, _p(s), _p(fval), ctypes.byref(it)))
+    return s, fval, it.value
+
+
 class Plan:
     """Device-resident propagator.  Coefficients are torch tensors on cuda:0
     in the compact layout [r][k3][k2][k1] (complex128 for fp64, complex64
@@ -405,6 +477,17 @@ class Plan:
         d = max(self.dim, 1)
         return (ER.reshape((3, 3), order="F"), Ex[: self.dim], Varx.reshape((d, d), order="F")[: self.dim, : self.dim],
                 fx.reshape((self.N,) * self.dim, order="F") if self.dim else fx)
+
+    def get_stat(self, f, return_fx=False):
+        """Full get_stat of the device grid f on the plan's grids (dict)."""
+        out = StatOut()
+        fx = np.zeros(self.nx)
+        _check(lib().hpup_plan_get_stat(self._h, self._chk(f, self.ngrid, self.rdtype), ctypes.byref(out), _p(fx),
+                                        self._stream()))
+        r = out.as_dict(self.dim)
+        if return_fx:
+            r["fx"] = fx.reshape((self.N,) * self.dim, order="F")
+        return r
 
     def profile(self, F, nreps=3):
         """Average milliseconds per kernel of an Euler step (advances F by nreps

@@ -75,7 +75,8 @@ int hpup_fftSO3R_backward(const double* F, const int64_t* dims_F, int ndims, con
  * Replaces propagation/mex/sources/pendulum_propagate_reduced/kernel.cu:9-262.
  * F complex (2B-1,2B-1,B,N,N); X complex (N,N,2); MR complex (2B-1,2B-1,B,2);
  * bt real(2); G real 2x2; u complex (2B-1,2B-1,B,3); CG: B*B pointers to the
- * cells CG{l1+1,l2+1} at index l1 + l2*B (only l2 = 1 is read, see DESIGN).
+ * cells CG{l1+1,l2+1} at index l1 + l2*B (only l2 = 1 is read: MR lives on
+ * degree 1, and MR content elsewhere is rejected with HPUP_ERR_UNSUPPORTED).
  * Fnew same shape as F, entries with max(|m|,|n|) > l set 0. */
 int hpup_pendulum_propagate_reduced(const double* F, const int64_t* dims_F, const double* X,
                                     const double* MR, const double* bt, const double* G,
@@ -171,6 +172,50 @@ int hpup_discrete_propagate_nonoise(const double* f, int B, int N, const double*
                                     double* df);
 
 /* ---------------------------------------------------------------------------
+ * Output statistics and the matrix-Fisher fit (host FP64).
+ * ------------------------------------------------------------------------- */
+
+/* get_stat outputs.  3x3 blocks column-major; blocks involving omega have
+ * leading dimension dim (Ex dim, Varx dim x dim, ExvR dim x 3, P dim x 3,
+ * Miu dim, Sigma dim x dim).  S = diag(s) of the fitted proper singular
+ * values.  niter: Newton iterations of the fit, -1 if it did not reach
+ * |residual| <= 1e-12 in 100 (the outputs are then those of the last
+ * iterate, as the reference keeps them after its warning).  fit_ok = 0: a
+ * fit output was not finite, and S, EvR, EvRvR, ExvR, P, Miu, Sigma are NaN
+ * (the reference's catch branch). */
+typedef struct {
+    double ER[9], Ex[3], Varx[9], EvR[3], ExvR[9], EvRvR[9];
+    double U[9], S[9], V[9], P[9], Miu[3], Sigma[9];
+    int niter, fit_ok;
+} hpup_stat;
+
+/* [ER, Ex, Varx, EvR, ExvR, EvRvR, U, S, V, P, Miu, Sigma(, fx)] = get_stat(f, R, x, w)
+ * Replaces the drivers' local function get_stat (propagation/pendulum_reduced.m:485-532,
+ * pendulum.m:547-594, pendulum_reduced_den.m:553-600 with the fx output).
+ * f real (2B,2B,2B,N,N[,N]) with dims/ndims (5 or 6); R real (3,3,2B,2B,2B);
+ * x real (dim,N,..,N); w real (2B).  The grid sums run on the device in one
+ * pass over f; fx (N^dim, may be NULL) receives the omega marginal. */
+int hpup_get_stat(const double* f, const int64_t* dims, int ndims, const double* R, const double* x,
+                  const double* w, hpup_stat* out, double* fx);
+
+/* [U, S, V] = psvd(F): proper SVD, U, V in SO(3), F = U S V^T, S diagonal
+ * with S33 carrying the sign of det F.  Replaces matrix Fisher/psvd.m:1-13. */
+int hpup_psvd(const double* F, double* U, double* S, double* V);
+
+/* [c, dc, ddc] = pdf_MF_normal(s, bool_scaled, 1, 1): normalizing constant of
+ * the matrix Fisher density on SO(3) with proper singular values s, its
+ * gradient (3) and Hessian (3x3); bool_scaled = 1 multiplies all three by
+ * exp(-sum(s)) as the reference's scaled form does (c_bar and its
+ * derivatives).  dc and ddc may be NULL.  Replaces matrix Fisher/pdf_MF_normal.m:1-211. */
+int hpup_pdf_MF_normal(const double* s, int bool_scaled, double* c, double* dc, double* ddc);
+
+/* [s, FVAL, NITER] = pdf_MF_M2S(d, s0): proper singular values whose first
+ * moments E[Q_ii] equal d.  Replaces matrix Fisher/pdf_MF_M2S.m:1-102 (same
+ * root and stopping rule max|FVAL| <= 1e-12, 100 iterations).  s0 may be
+ * NULL (zeros).  *niter = -1 if not converged. */
+int hpup_pdf_MF_M2S(const double* d, const double* s0, double* s, double* fval, int* niter);
+
+/* ---------------------------------------------------------------------------
  * 2. Device-resident plan API.
  * ------------------------------------------------------------------------- */
 
@@ -199,8 +244,9 @@ int64_t hpup_plan_num_coeffs(const hpup_plan* plan);
 /* Number of grid points: (2B)^3 * nx. */
 int64_t hpup_plan_num_grid(const hpup_plan* plan);
 
-/* Device coefficient layout: compact rotation index r (l-block offset
- * l(2l-1)(2l+1)/3, then (n+l)(2l+1) + (m+l)) outermost, omega index
+/* Device coefficient layout: compact rotation index r outermost, m-major:
+ * for m = -(B-1)..B-1, l = |m|..B-1, n = -l..l, r = moff(m) + l^2 - m^2 + (n+l)
+ * with moff(m) the number of (l, m', n) triples with m' < m; omega index
  * k1 + N(k2 + N k3) innermost; complex128 (FP64) or complex64 (FP32, BF16X3).
  * Grid layout: MATLAB (alpha, beta, gamma, x) real, same precision. */
 
@@ -234,6 +280,12 @@ double hpup_plan_profile_bytes(const hpup_plan* plan, int i);
  * host outputs, from the device grid f_dev. */
 int hpup_plan_stat_moments(hpup_plan* plan, const void* f_dev, double* ER, double* Ex,
                            double* Varx, double* fx, void* stream);
+
+/* Full get_stat (moments, proper SVD, matrix-Fisher fit, MFG parameters) of
+ * the device grid f_dev on the plan's own grids (rotations R(alpha, beta,
+ * gamma) = Rz Ry Rz, omega x = -L/2 + L/N * index).  fx may be NULL.
+ * Synchronizes `stream`. */
+int hpup_plan_get_stat(hpup_plan* plan, const void* f_dev, hpup_stat* out, double* fx, void* stream);
 
 #ifdef __cplusplus
 }

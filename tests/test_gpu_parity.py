@@ -181,7 +181,8 @@ def test_plan_derivative_and_mass(precision, tol):
     m0 = float(F[p.mass_index].real)
     p.step(F, 3)
     torch.cuda.synchronize()
-    assert abs(float(F[p.mass_index].real) - m0) <= 1e-12 * abs(m0) if precision == "fp64" else 1e-6 * abs(m0)
+    mass_tol = 1e-12 if precision == "fp64" else 1e-6
+    assert abs(float(F[p.mass_index].real) - m0) <= mass_tol * abs(m0)
 
 
 def test_plan_transforms_roundtrip():
