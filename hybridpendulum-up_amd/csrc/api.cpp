@@ -259,7 +259,7 @@ template <class T> struct PlanT : PlanBase {
     DriftTables<T> dr;
     bool has_drift = false;
     bool bf16x3 = false;  // circulants as 3xBF16 split products (FP32 storage)
-    DevBuf G, Wb, V1, V2, Z3, Fs, ACC, tmp, st_work, st_out;
+    DevBuf G, Wb, V1, V2, Z3, Fs, Fs2, ACC, tmp, st_work, st_out;
 
     void init_xform(const double* d, const double* wv) {
         const hpup_model& m = model;
@@ -322,6 +322,10 @@ template <class T> struct PlanT : PlanBase {
 
     void stage(const void* Fin, const void* F0, void* Fout, const StageCoef<T>& sc, hipStream_t st) {
         DriftDev<T> D = dr.view;
+        // the fused kernel reads neighbouring rotation rows of Fin while other
+        // workgroups write Fout, so the two must not alias (the three-pass
+        // path reads Fin only through its own slab and may run in place)
+        HPUP_REQUIRE(!(fused() && Fin == Fout), HPUP_ERR_ARG, "fused drift stage cannot run in place");
         if (den) den_gravity(Fin, D, st);
         if (fused()) {
             if (D.dim == 3) launch_axis3_mfma<T>(D, Fin, Z3.p, bf16x3, st);
@@ -341,14 +345,36 @@ template <class T> struct PlanT : PlanBase {
     }
     void step(void* Fv, int nsteps, hipStream_t st) override {
         const int meth = model.method;
+        const bool fz = fused();
         ensure_drift_buffers(meth != HPUP_EULER);
         const T dt = (T)model.dt;
         cT<T>* F = (cT<T>*)Fv;
+        if (meth == HPUP_EULER) {
+            const StageCoef<T> sc{dt, 0, 1, 0, 0, 0, 0, 0, 0};
+            if (!fz) {
+                for (int it = 0; it < nsteps; it++) stage(F, nullptr, F, sc, st);
+                return;
+            }
+            // Euler steps alternate between F and Fs; an odd count ends with a copy
+            ensure(Fs, cbytes());
+            cT<T>* cur = F;
+            cT<T>* nxt = Fs.as<cT<T>>();
+            for (int it = 0; it < nsteps; it++) {
+                stage(cur, nullptr, nxt, sc, st);
+                std::swap(cur, nxt);
+            }
+            if (cur != F) HPUP_HIP(hipMemcpyAsync(F, cur, cbytes(), hipMemcpyDeviceToDevice, st));
+            return;
+        }
+        // RK4's middle stages read and write the stage buffer: the fused path
+        // alternates it with Fs2
+        void* Fm = Fs.p;
+        if (meth == HPUP_RK4 && fz) {
+            ensure(Fs2, cbytes());
+            Fm = Fs2.p;
+        }
         for (int it = 0; it < nsteps; it++) {
             switch (meth) {
-            case HPUP_EULER:
-                stage(F, nullptr, F, StageCoef<T>{dt, 0, 1, 0, 0, 0, 0, 0, 0}, st);
-                break;
             case HPUP_MIDPOINT:
                 stage(F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 0, 0, 0, 0, 0}, st);
                 stage(Fs.p, F, F, StageCoef<T>{dt, 0, 1, 0, 0, 1, 0, 0, 0}, st);
@@ -359,8 +385,8 @@ template <class T> struct PlanT : PlanBase {
                 break;
             case HPUP_RK4:
                 stage(F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 1, 0, 0, 1, 0}, st);
-                stage(Fs.p, F, Fs.p, StageCoef<T>{dt / 2, 0, 1, 1, 2, 1, 1, 1, 0}, st);
-                stage(Fs.p, F, Fs.p, StageCoef<T>{dt, 0, 1, 1, 2, 1, 1, 1, 0}, st);
+                stage(Fs.p, F, Fm, StageCoef<T>{dt / 2, 0, 1, 1, 2, 1, 1, 1, 0}, st);
+                stage(Fm, F, Fs.p, StageCoef<T>{dt, 0, 1, 1, 2, 1, 1, 1, 0}, st);
                 stage(Fs.p, F, F, StageCoef<T>{dt / 6, 1, 1, 0, 0, 1, 1, 0, 0}, st);
                 break;
             default:
@@ -381,6 +407,9 @@ template <class T> struct PlanT : PlanBase {
         const StageCoef<T> sc{(T)model.dt, 0, 1, 0, 0, 0, 0, 0, 0};
         const bool fz = fused();
         const int nk = fz ? 2 : 3;
+        if (fz) ensure(Fs, cbytes());
+        void* cur = Fv;
+        void* nxt = Fs.p;
         hipEvent_t ev[4];
         for (auto& e : ev) HPUP_HIP(hipEventCreate(&e));
         double acc[3] = {0, 0, 0};
@@ -388,10 +417,11 @@ template <class T> struct PlanT : PlanBase {
             for (int it = 0; it < nreps; it++) {
                 HPUP_HIP(hipEventRecord(ev[0], st));
                 if (fz) {
-                    if (D.dim == 3) launch_axis3_mfma<T>(D, Fv, Z3.p, bf16x3, st);
+                    if (D.dim == 3) launch_axis3_mfma<T>(D, cur, Z3.p, bf16x3, st);
                     HPUP_HIP(hipEventRecord(ev[1], st));
-                    launch_drift_fused<T>(D, dr.mix, Fv, Z3.p, nullptr, Fv, ACC.p, sc, bf16x3, st);
+                    launch_drift_fused<T>(D, dr.mix, cur, Z3.p, nullptr, nxt, ACC.p, sc, bf16x3, st);
                     HPUP_HIP(hipEventRecord(ev[2], st));
+                    std::swap(cur, nxt);
                 } else {
                     launch_rotmix<T>(D, Fv, Wb.p, V1.p, V2.p, st);
                     HPUP_HIP(hipEventRecord(ev[1], st));
@@ -412,6 +442,7 @@ template <class T> struct PlanT : PlanBase {
             throw;
         }
         for (auto& e : ev) HPUP_HIP(hipEventDestroy(e));
+        if (cur != Fv) HPUP_HIP(hipMemcpyAsync(Fv, cur, cbytes(), hipMemcpyDeviceToDevice, st));
         for (int k = 0; k < nk; k++) ms[k] = acc[k] / nreps;
         return nk;
     }

@@ -81,6 +81,14 @@ def lib():
     """Loads libhpup.so (raises if it is not built)."""
     global _lib
     if _lib is None:
+        # torch bundles its own HIP runtime under the same soname as ROCm's
+        # (libamdhip64.so.7); whichever loads first serves the whole process.
+        # Loading torch first keeps one runtime that both torch and libhpup
+        # initialise; the other order leaves torch without a device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise HpupError("libhpup.so is not built at %s (run __graft_entry__.build())" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)

@@ -185,6 +185,34 @@ def test_plan_derivative_and_mass(precision, tol):
     assert abs(float(F[p.mass_index].real) - m0) <= mass_tol * abs(m0)
 
 
+@pytest.mark.parametrize("method,nsteps", [("euler", 1), ("euler", 2), ("euler", 3), ("midpoint", 2), ("RK2", 2),
+                                            ("RK4", 1), ("RK4", 2)])
+def test_plan_steps_vs_oracle(method, nsteps):
+    """Several steps on the device-resident plan against the oracle
+    integrator, at B = 5 (3-D), where the grid holds more workgroups than run
+    at once: the fused stage reads neighbouring rotation rows of its input, so
+    its stage buffers alternate (an odd Euler count ends with a copy)."""
+    import torch
+    _print("case plan steps", method, nsteps)
+    B, N, L, dt = 5, 6, 10.0, 0.004
+    Jt = np.diag([1.0, 0.93539, 0.24935])
+    Sigma = np.array([[0.8, 0.1, 0.0], [0.1, 0.6, 0.05], [0.0, 0.05, 1.0]])
+    f = density(B, 3, N, L, (10, 5, 1), seed=8, Sigma=Sigma)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    Ffull = O.fft_forward(B, f, d, w, 3, N)
+    X, OJO, MR, u, cells = operators(B, 3, N, L, Jt)
+    bt, G = np.array([0.2, 0.15, 0.25]), np.diag([0.1, 0.2, 0.05])
+    ref = Ffull
+    for _ in range(nsteps):
+        ref = O.integrate(B, 3, N, ref, X, OJO, MR, bt, G, dt, L, u, cells, method)
+    p = hpup.Plan(B, 3, N, L, b=bt, G=G, Jt=Jt, dt=dt, method=method, precision="fp64")
+    F = p.upload_full(Ffull)
+    p.step(F, nsteps)
+    torch.cuda.synchronize()
+    assert rel_l2(p.download_full(F), ref) < 1e-10
+
+
 def test_plan_transforms_roundtrip():
     """forward(backward(F)) == F for bandlimited coefficients (plan API, FP64)."""
     import torch
