@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "fp64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-reps", type=int, default=3)
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="at the end, run a plain copy kernel (torch.neg, 4-byte elements) over this many GiB "
+                         "three times: the known-size reference for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU sharding is "
@@ -188,6 +191,15 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+    if args.copy_probe_gib > 0:
+        n = int(args.copy_probe_gib * 2 ** 30 / 4)
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes read and %d written per launch" % (4 * n, 4 * n))
 
 
 if __name__ == "__main__":
