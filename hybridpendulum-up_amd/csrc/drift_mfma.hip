@@ -1,7 +1,7 @@
 // MFMA drift kernels for omega grids up to 32 points per axis.
 //
-// k_drift_fused evaluates the whole derivative of one (r, k3) slab and applies
-// the integrator epilogue (see drift.hip for the operator split):
+// k_drift_fused evaluates the whole derivative of each (r, k3) slab it visits
+// and applies the integrator epilogue (see drift.hip for the operator split):
 //   P1 = T1 F, P2 = T2 F                                    (MFMA)
 //   W  = -sum_p c_p CG_p[F] + D F,  V1 = -U1 F, V2 = -U2 F   (neighbour rows,
 //        within 2B+1 rows of r in the m-major layout, read through L2)
@@ -20,6 +20,8 @@
 //
 // k_axis3 computes Z3 = T3 F (circulant along the outermost omega axis) as the
 // GEMM C3 [F_re; F_im] over 64-column chunks of the k1,k2 plane.
+#include <algorithm>
+
 #include "drift.h"
 #include "kernels_common.h"
 
@@ -52,48 +54,51 @@ template <> struct Mfma<double> {
 
 constexpr int kPad = 33;  // LDS row stride of the 32x32 planes
 
-// Real-ified circulant entry for GEMM row/column `o` (output index along the
-// axis), reduction index k in [0, 64): (re | im) stacked input, `im_out`
-// selects the imaginary output half:  [[Cr, -Ci], [Ci, Cr]].
-template <class T>
-__device__ __forceinline__ T circ_entry(const cT<T>* kap, int N, int o, int k, bool im_out) {
-    const int a = k & 31;
-    cT<T> c = czero<T>();
-    if (o < N && a < N) c = kap[o >= a ? o - a : o - a + N];
-    const bool re_in = k < 32;
-    return im_out ? (re_in ? c.y : c.x) : (re_in ? c.x : -c.y);
-}
-
 // ------------------------------------------------------------ policies ---
 
-// FP32 / FP64 MFMA, 16 reduction steps of K = 4.
+// Circulant fragments from LDS.  Entry (o, a) of a circulant with kernel
+// kappa is kappa[(o - a) mod N]; the table holds, for d = o - a + 32 in
+// [0, 64), the pair (re, im) of kappa[(o - a) mod N] and the pair (-im, re)
+// used for the imaginary input half.  Rows o >= N or columns a >= N read
+// finite values that meet zero-padded data or are discarded, so the table
+// needs no padding logic.  CircTab<T>::fill builds it for one axis.
+template <class T> struct CircTab {
+    cT<T> re_in[64], im_in[64];
+    __device__ __forceinline__ void fill(const cT<T>* kap, int N, int tid) {
+        if (tid < 64) {
+            const int d = tid - 32;
+            const cT<T> c = kap[((d % N) + N) % N];
+            re_in[tid] = c;
+            im_in[tid] = cmk<T>(-c.y, c.x);
+        }
+    }
+};
+
+// FP32 / FP64 MFMA, 16 reduction steps of K = 4; fragments read from LDS.
 template <class T> struct GemmF {
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
-    T a2re[16], a2im[16], b1re[16], b1im[16];
-    int k2t, k1t, lr, lq;
+    const CircTab<T>* c1;
+    const CircTab<T>* c2;
+    int k2t, k1t, lr, lq, b1, b2;
     static __device__ __forceinline__ int orow(int q, int v) { return MF::orow(q, v); }
-    __device__ __forceinline__ void init(const cT<T>* kap1, const cT<T>* kap2, int N, int k2t_, int k1t_, int lr_,
-                                         int lq_) {
+    __device__ __forceinline__ void init(const CircTab<T>* tab1, const CircTab<T>* tab2, int k2t_, int k1t_,
+                                         int lr_, int lq_) {
+        c1 = tab1; c2 = tab2;
         k2t = k2t_; k1t = k1t_; lr = lr_; lq = lq_;
-#pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const int k = 4 * s + lq;
-            a2re[s] = circ_entry<T>(kap2, N, 16 * k2t + lr, k, false);
-            a2im[s] = circ_entry<T>(kap2, N, 16 * k2t + lr, k, true);
-            b1re[s] = circ_entry<T>(kap1, N, 16 * k1t + lr, k, false);
-            b1im[s] = circ_entry<T>(kap1, N, 16 * k1t + lr, k, true);
-        }
+        b1 = 16 * k1t + lr + 32 - lq;  // d of reduction step s is b - 4 (s & 7)
+        b2 = 16 * k2t + lr + 32 - lq;
     }
     // Y = X C1^T along k1 (slab rows as the A operand)
     __device__ __forceinline__ void t1(const T* pr, const T* pi, cT<T>* out) const {
         acc_t are = MF::zero(), aim = MF::zero();
         const int row = (16 * k2t + lr) * kPad;
-#pragma unroll
+#pragma unroll 4
         for (int s = 0; s < 16; s++) {
             const T av = (s < 8 ? pr : pi)[row + ((4 * s + lq) & 31)];
-            are = MF::fma(av, b1re[s], are);
-            aim = MF::fma(av, b1im[s], aim);
+            const cT<T> f = (s < 8 ? c1->re_in : c1->im_in)[b1 - 4 * (s & 7)];
+            are = MF::fma(av, f.x, are);
+            aim = MF::fma(av, f.y, aim);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) out[v] = cmk<T>(are[v], aim[v]);
@@ -102,11 +107,12 @@ template <class T> struct GemmF {
     __device__ __forceinline__ void t2(const T* pr, const T* pi, cT<T>* out) const {
         acc_t are = MF::zero(), aim = MF::zero();
         const int col = 16 * k1t + lr;
-#pragma unroll
+#pragma unroll 4
         for (int s = 0; s < 16; s++) {
             const T bv = (s < 8 ? pr : pi)[((4 * s + lq) & 31) * kPad + col];
-            are = MF::fma(a2re[s], bv, are);
-            aim = MF::fma(a2im[s], bv, aim);
+            const cT<T> f = (s < 8 ? c2->re_in : c2->im_in)[b2 - 4 * (s & 7)];
+            are = MF::fma(f.x, bv, are);
+            aim = MF::fma(f.y, bv, aim);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) out[v] = cmk<T>(are[v], aim[v]);
@@ -133,19 +139,24 @@ struct GemmBf16x3 {
     bf16x8 a2re_h[2], a2re_l[2], a2im_h[2], a2im_l[2], b1re_h[2], b1re_l[2], b1im_h[2], b1im_l[2];
     int k2t, k1t, lr, lq;
     static __device__ __forceinline__ int orow(int q, int v) { return 4 * q + v; }
-    __device__ __forceinline__ void init(const float2* kap1, const float2* kap2, int N, int k2t_, int k1t_, int lr_,
-                                         int lq_) {
+    __device__ __forceinline__ void init(const CircTab<float>* tab1, const CircTab<float>* tab2, int k2t_, int k1t_,
+                                         int lr_, int lq_) {
         k2t = k2t_; k1t = k1t_; lr = lr_; lq = lq_;
+        // entry (o, k) of the real-ified circulant, as GemmF reads it
+        auto ent = [](const CircTab<float>* t, int o, int k, bool im_out) {
+            const float2 c = (k < 32 ? t->re_in : t->im_in)[o - (k & 31) + 32];
+            return im_out ? c.y : c.x;
+        };
 #pragma unroll
         for (int s = 0; s < 2; s++) {
             float v[4][8];
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 const int k = 32 * s + 8 * lq + t;
-                v[0][t] = circ_entry<float>(kap2, N, 16 * k2t + lr, k, false);
-                v[1][t] = circ_entry<float>(kap2, N, 16 * k2t + lr, k, true);
-                v[2][t] = circ_entry<float>(kap1, N, 16 * k1t + lr, k, false);
-                v[3][t] = circ_entry<float>(kap1, N, 16 * k1t + lr, k, true);
+                v[0][t] = ent(tab2, 16 * k2t + lr, k, false);
+                v[1][t] = ent(tab2, 16 * k2t + lr, k, true);
+                v[2][t] = ent(tab1, 16 * k1t + lr, k, false);
+                v[3][t] = ent(tab1, 16 * k1t + lr, k, true);
             }
             split8(v[0], a2re_h[s], a2re_l[s]);
             split8(v[1], a2im_h[s], a2im_l[s]);
@@ -205,143 +216,199 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
 }
 
 // ------------------------------------------------------- fused kernel ---
+//
+// Persistent: each workgroup builds its circulant fragments once and walks
+// slabs sl = first, first + gridDim.x, ... (sl = k3 * nrc + r), so the chip
+// works on one contiguous window of slabs at a time and the neighbour rows the
+// rotation mixing reads stay in L2.  Three LDS plane pairs (slab, V1, V2) need
+// two barriers per slab: writes to the slab planes follow the previous slab's
+// second barrier, writes to the V planes follow this slab's first barrier.
 
-template <class T, class G>
+template <class T> struct MixBatch;  // source rows whose loads are issued together
+template <> struct MixBatch<float> { static constexpr int n = 4; };
+template <> struct MixBatch<double> { static constexpr int n = 2; };
+
+template <class T>
+__device__ __forceinline__ bool cnz(cT<T> c) { return c.x != T(0) || c.y != T(0); }
+
+template <class T, class G, int DIM>
 __global__ __launch_bounds__(256) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
                                                      const cT<T>* __restrict__ Z3, const cT<T>* F0, cT<T>* Fout,
-                                                     cT<T>* ACC, StageCoef<T> sc) {
-    __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad];
-    __shared__ cT<T> kap[2][32];
+                                                     cT<T>* ACC, StageCoef<T> sc, int64_t nslab) {
+    __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad], pc_re[32 * kPad],
+        pc_im[32 * kPad];
+    __shared__ CircTab<T> tabs[2];
+    constexpr int NC = DIM + 2;
+    constexpr int MB = MixBatch<T>::n;
 
-    const int N = D.N, NN = N * N, dim = D.dim;
-    const int nz = dim == 3 ? N : 1;
-    const int64_t sl = xcd_remap(blockIdx.x, gridDim.x);  // = k3 * nrc + r
-    const int64_t r = sl % D.nrc;
-    const int k3 = (int)(sl / D.nrc);
-    const int64_t base = (r * nz + k3) * NN;
+    const int N = D.N, NN = N * N;
+    const int nz = DIM == 3 ? N : 1;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int k2t = w >> 1, k1t = w & 1, lr = lane & 15, lq = lane >> 4;
-
-    for (int idx = tid; idx < 1024; idx += 256) {
-        const int a2 = idx >> 5, a1 = idx & 31;
-        const cT<T> v = (a2 < N && a1 < N) ? Fin[base + a2 * N + a1] : czero<T>();
-        pa_re[a2 * kPad + a1] = v.x;
-        pa_im[a2 * kPad + a1] = v.y;
-    }
-    if (tid < 64) {
-        const int p = tid >> 5, j = tid & 31;
-        kap[p][j] = j < N ? D.kap[p * N + j] : czero<T>();
-    }
+    tabs[0].fill(D.kap, N, tid);
+    tabs[1].fill(D.kap + N, N, tid);
     __syncthreads();
-
     G gm;
-    gm.init(kap[0], kap[1], N, k2t, k1t, lr, lq);
+    gm.init(&tabs[0], &tabs[1], k2t, k1t, lr, lq);
 
-    cT<T> P1[4], P2[4];
-    gm.t1(pa_re, pa_im, P1);
-    gm.t2(pa_re, pa_im, P2);
-
-    // rotation mixing at this lane's four elements
+    // this lane's four elements (k2v[v], k1) of every slab
     const int k1 = 16 * k1t + lr;
     const bool k1ok = k1 < N;
-    const T kc1 = k1ok ? D.kc[k1] : T(0);
-    const T kd1 = k1ok ? D.kd[k1] : T(0);
-    const T kc3 = dim == 3 ? D.kc[k3] : T(0);
-    const T kd3 = dim == 3 ? D.kd[k3] : T(0);
-    cT<T> Wv[4], V1[4], V2[4], Zv[4];
-    cT<T> ag[4][3];
+    const T kc1 = k1ok ? D.kc[k1] : T(0), kd1 = k1ok ? D.kd[k1] : T(0);
+    int k2v[4];
+    bool ok[4];
+    T kc2v[4], kd2v[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        V1[v] = czero<T>(); V2[v] = czero<T>(); Zv[v] = czero<T>();
-        ag[v][0] = czero<T>(); ag[v][1] = czero<T>(); ag[v][2] = czero<T>();
+        k2v[v] = 16 * k2t + G::orow(lq, v);
+        ok[v] = k1ok && k2v[v] < N;
+        kc2v[v] = ok[v] ? D.kc[k2v[v]] : T(0);
+        kd2v[v] = ok[v] ? D.kd[k2v[v]] : T(0);
     }
-    const int NC = dim + 2;
-    for (int t = 0; t < MX.TS; t++) {
-        const int src = MX.src[r * MX.TS + t];
-        if (src < 0) break;
-        const cT<T>* cf = MX.coef + (r * MX.TS + t) * NC;
-        const cT<T> cu1 = cf[dim], cu2 = cf[dim + 1];
-        const cT<T> cg0 = cf[0], cg1 = cf[1], cg2 = dim == 3 ? cf[2] : czero<T>();
-        const cT<T>* Fs = Fin + ((int64_t)src * nz + k3) * NN;
+    // D(k) = sum_i G_ii kd(k_i) - sum_{i != j} G_ij kc(k_i) kc(k_j)
+    const T g00 = D.G[0], g11 = D.G[DIM + 1], g01 = D.G[1] + D.G[DIM];
+    const T g22 = DIM == 3 ? D.G[8] : T(0), g02 = DIM == 3 ? D.G[2] + D.G[6] : T(0),
+            g12 = DIM == 3 ? D.G[5] + D.G[7] : T(0);
+
+    for (int64_t sl = xcd_remap(blockIdx.x, gridDim.x); sl < nslab; sl += gridDim.x) {
+        const int64_t r = sl % D.nrc;
+        const int k3 = (int)(sl / D.nrc);
+        const int64_t base = (r * nz + k3) * NN;
+        const T kc3 = DIM == 3 ? D.kc[k3] : T(0), kd3 = DIM == 3 ? D.kd[k3] : T(0);
+
+        // slab -> plane A (zero padded to 32 x 32)
+        {
+            cT<T> own[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+                own[i] = (a2 < N && a1 < N) ? Fin[base + a2 * N + a1] : czero<T>();
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+                pa_re[a2 * kPad + a1] = own[i].x;
+                pa_im[a2 * kPad + a1] = own[i].y;
+            }
+        }
+        cT<T> Zv[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) Zv[v] = (DIM == 3 && ok[v]) ? Z3[base + k2v[v] * N + k1] : czero<T>();
+
+        // rotation mixing: sources in batches of MB rows, structurally zero
+        // coefficients skipped (the branches are uniform over the workgroup).
+        // The Clebsch-Gordan terms enter W as -i (sum_p kc_p CG_p) F_src with
+        // the kc of this lane's element, so one accumulator holds all of them.
+        cT<T> V1[4], V2[4], Wv[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int k2 = 16 * k2t + G::orow(lq, v);
-            if (!(k1ok && k2 < N)) continue;
-            const cT<T> fv = Fs[k2 * N + k1];
-            cfma<T>(ag[v][0], cg0, fv);
-            cfma<T>(ag[v][1], cg1, fv);
-            if (dim == 3) cfma<T>(ag[v][2], cg2, fv);
-            cfma<T>(V1[v], cu1, fv);
-            cfma<T>(V2[v], cu2, fv);
+            V1[v] = czero<T>();
+            V2[v] = czero<T>();
+            Wv[v] = czero<T>();
         }
-    }
+        const int TS = MX.TS;
+        for (int t0 = 0; t0 < TS; t0 += MB) {
+            int src[MB];
+            cT<T> fv[MB][4];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int k2 = 16 * k2t + G::orow(lq, v);
-        Wv[v] = czero<T>();
-        if (!(k1ok && k2 < N)) continue;
-        const T kc2 = D.kc[k2], kd2 = D.kd[k2];
-        const T kc[3] = {kc1, kc2, kc3}, kd[3] = {kd1, kd2, kd3};
-        T dd = T(0);
-        for (int i = 0; i < dim; i++)
-            for (int j = 0; j < dim; j++) {
-                const T g = D.G[i + dim * j];
-                dd += (i == j) ? g * kd[i] : -g * kc[i] * kc[j];
+            for (int u = 0; u < MB; u++) {
+                src[u] = t0 + u < TS ? MX.src[r * TS + t0 + u] : -1;
+                const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : src[u]) * nz + k3) * NN;
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    fv[u][v] = (src[u] >= 0 && ok[v]) ? Fs[k2v[v] * N + k1] : czero<T>();
             }
-        const cT<T> fh = cmk<T>(pa_re[k2 * kPad + k1], pa_im[k2 * kPad + k1]);
-        cT<T> wv = cscale<T>(fh, dd);
-        wv = csub<T>(wv, cimul<T>(ag[v][0], kc1));
-        wv = csub<T>(wv, cimul<T>(ag[v][1], kc2));
-        for (int p = 0; p < dim; p++)  // gravity evaluated on the grid (density form)
-            if (D.grav[p]) wv = csub<T>(wv, cimul<T>(D.grav[p][base + k2 * N + k1], kc[p]));
-        if (dim == 3) {
-            wv = csub<T>(wv, cimul<T>(ag[v][2], kc3));
-            Zv[v] = Z3[base + k2 * N + k1];
-            if (D.has_gyro) {
-                V1[v] = csub<T>(V1[v], cimul<T>(Zv[v], kc2 * D.gyro[1]));
+#pragma unroll
+            for (int u = 0; u < MB; u++) {
+                if (src[u] < 0) continue;
+                const cT<T>* cf = MX.coef + (r * TS + t0 + u) * NC;
+                const cT<T> g0 = cf[0], g1 = cf[1], g2 = DIM == 3 ? cf[2] : czero<T>();
+                const bool h0 = cnz<T>(g0) || (DIM == 3 && cnz<T>(g2)), h1 = cnz<T>(g1);
+                if (h0 || h1) {
+                    // kc1 CG_1 + kc3 CG_3 is the same for the lane's four elements
+                    const cT<T> e = cadd<T>(cscale<T>(g0, kc1), cscale<T>(g2, kc3));
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const cT<T> c = cadd<T>(e, cscale<T>(g1, kc2v[v]));
+                        const cT<T> t = cmul<T>(c, fv[u][v]);
+                        Wv[v] = cmk<T>(Wv[v].x + t.y, Wv[v].y - t.x);  // W -= i c f
+                    }
+                }
+                const cT<T> cu1 = cf[DIM], cu2 = cf[DIM + 1];
+                if (cnz<T>(cu1))
+#pragma unroll
+                    for (int v = 0; v < 4; v++) cfma<T>(V1[v], cu1, fv[u][v]);
+                if (cnz<T>(cu2))
+#pragma unroll
+                    for (int v = 0; v < 4; v++) cfma<T>(V2[v], cu2, fv[u][v]);
+            }
+        }
+        // W += D(k) F - sum_p c_p (grid gravity term), at this lane's elements
+        // (the slab value itself re-read through the cache)
+        const T dd0 = g00 * kd1 + g22 * kd3 - g02 * kc1 * kc3;
+        const T dd1 = -g01 * kc1 - g12 * kc3;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            if (!ok[v]) continue;
+            const int64_t o = base + k2v[v] * N + k1;
+            const T dd = dd0 + g11 * kd2v[v] + dd1 * kc2v[v];
+            cT<T> wv = cadd<T>(Wv[v], cscale<T>(Fin[o], dd));
+            const T kcp[3] = {kc1, kc2v[v], kc3};
+#pragma unroll
+            for (int p = 0; p < DIM; p++)  // gravity evaluated on the grid (density form)
+                if (D.grav[p]) wv = csub<T>(wv, cimul<T>(D.grav[p][o], kcp[p]));
+            Wv[v] = wv;
+        }
+        __syncthreads();  // plane A complete; the V planes of the previous slab are consumed
+
+        cT<T> P1[4], P2[4];
+        gm.t1(pa_re, pa_im, P1);
+        gm.t2(pa_re, pa_im, P2);
+        if (DIM == 3 && D.has_gyro) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (!ok[v]) continue;
+                V1[v] = csub<T>(V1[v], cimul<T>(Zv[v], kc2v[v] * D.gyro[1]));
                 V1[v] = csub<T>(V1[v], cimul<T>(P2[v], kc3 * D.gyro[2]));
                 V2[v] = csub<T>(V2[v], cimul<T>(Zv[v], kc1 * D.gyro[0]));
             }
         }
-        Wv[v] = wv;
-    }
-    __syncthreads();  // all reads of the F planes are done
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int k2 = 16 * k2t + G::orow(lq, v);
-        pa_re[k2 * kPad + k1] = V1[v].x;
-        pa_im[k2 * kPad + k1] = V1[v].y;
-        pb_re[k2 * kPad + k1] = V2[v].x;
-        pb_im[k2 * kPad + k1] = V2[v].y;
-    }
-    __syncthreads();
-    cT<T> Q1[4], Q2[4];
-    gm.t1(pa_re, pa_im, Q1);
-    gm.t2(pb_re, pb_im, Q2);
+        for (int v = 0; v < 4; v++) {
+            const int k2 = k2v[v];
+            pb_re[k2 * kPad + k1] = V1[v].x;
+            pb_im[k2 * kPad + k1] = V1[v].y;
+            pc_re[k2 * kPad + k1] = V2[v].x;
+            pc_im[k2 * kPad + k1] = V2[v].y;
+        }
+        __syncthreads();  // V planes complete; plane A is consumed
+        cT<T> Q1[4], Q2[4];
+        gm.t1(pb_re, pb_im, Q1);
+        gm.t2(pc_re, pc_im, Q2);
 
-    const cT<T> e3 = dim == 3 ? D.e3[r] : czero<T>();
+        const cT<T> e3 = DIM == 3 ? D.e3[r] : czero<T>();
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int k2 = 16 * k2t + G::orow(lq, v);
-        if (!(k1ok && k2 < N)) continue;
-        cT<T> q = cadd<T>(Q1[v], Q2[v]);
-        q = cadd<T>(q, Wv[v]);
-        q = cadd<T>(q, cimul<T>(P1[v], D.b[0] * kc1));
-        q = cadd<T>(q, cimul<T>(P2[v], D.b[1] * D.kc[k2]));
-        if (dim == 3) {
-            q = cadd<T>(q, cmul<T>(Zv[v], e3));
-            q = cadd<T>(q, cimul<T>(Zv[v], D.b[2] * kc3));
+        for (int v = 0; v < 4; v++) {
+            if (!ok[v]) continue;
+            cT<T> q = cadd<T>(Q1[v], Q2[v]);
+            q = cadd<T>(q, Wv[v]);
+            q = cadd<T>(q, cimul<T>(P1[v], D.b[0] * kc1));
+            q = cadd<T>(q, cimul<T>(P2[v], D.b[1] * kc2v[v]));
+            if (DIM == 3) {
+                q = cadd<T>(q, cmul<T>(Zv[v], e3));
+                q = cadd<T>(q, cimul<T>(Zv[v], D.b[2] * kc3));
+            }
+            const int64_t o = base + k2v[v] * N + k1;
+            if (sc.deriv_only) {
+                Fout[o] = q;
+                continue;
+            }
+            const cT<T> f0 = sc.read_f0 ? F0[o] : Fin[o];
+            const cT<T> ac = sc.read_acc ? ACC[o] : czero<T>();
+            const cT<T> upd = cadd<T>(cscale<T>(ac, sc.ca), cscale<T>(q, sc.cd));
+            Fout[o] = cadd<T>(f0, cscale<T>(upd, sc.dt));
+            if (sc.write_acc) ACC[o] = cadd<T>(cscale<T>(ac, sc.ea), cscale<T>(q, sc.ed));
         }
-        const int64_t o = base + k2 * N + k1;
-        if (sc.deriv_only) {
-            Fout[o] = q;
-            continue;
-        }
-        const cT<T> f0 = sc.read_f0 ? F0[o] : Fin[o];
-        const cT<T> ac = sc.read_acc ? ACC[o] : czero<T>();
-        const cT<T> upd = cadd<T>(cscale<T>(ac, sc.ca), cscale<T>(q, sc.cd));
-        Fout[o] = cadd<T>(f0, cscale<T>(upd, sc.dt));
-        if (sc.write_acc) ACC[o] = cadd<T>(cscale<T>(ac, sc.ea), cscale<T>(q, sc.ed));
     }
 }
 
@@ -354,7 +421,7 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
     using acc_t = typename MF::acc;
     constexpr int CP = 65;
     __shared__ T sre[32 * CP], sim[32 * CP];
-    __shared__ cT<T> kap[32];
+    __shared__ CircTab<T> tab;
     const int N = D.N;
     const int64_t NN = (int64_t)N * N;
     const int64_t r = blockIdx.y;
@@ -367,18 +434,23 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
         sre[a * CP + jj] = v.x;
         sim[a * CP + jj] = v.y;
     }
-    if (tid < 32) kap[tid] = tid < N ? D.kap[2 * N + tid] : czero<T>();
+    tab.fill(D.kap + 2 * N, N, tid);
     __syncthreads();
+    // acc[h + 2 * im_out]: output rows k3 = 16 h + ..., real / imaginary part
     acc_t acc[4];
 #pragma unroll
     for (int it = 0; it < 4; it++) acc[it] = MF::zero();
-#pragma unroll
+#pragma unroll 4
     for (int s = 0; s < 16; s++) {
         const int k = 4 * s + lq;
         const T bv = (s < 8 ? sre : sim)[(k & 31) * CP + 16 * w + lr];
+        const cT<T>* t = s < 8 ? tab.re_in : tab.im_in;
 #pragma unroll
-        for (int it = 0; it < 4; it++)
-            acc[it] = MF::fma(circ_entry<T>(kap, N, 16 * (it & 1) + lr, k, (it >> 1) != 0), bv, acc[it]);
+        for (int h = 0; h < 2; h++) {
+            const cT<T> f = t[16 * h + lr + 32 - (k & 31)];
+            acc[h] = MF::fma(f.x, bv, acc[h]);
+            acc[h + 2] = MF::fma(f.y, bv, acc[h + 2]);
+        }
     }
     const int64_t j = j0 + 16 * w + lr;
     if (j >= NN) return;
@@ -395,7 +467,7 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
                                                       float2* __restrict__ out) {
     constexpr int CP = 65;
     __shared__ float sre[32 * CP], sim[32 * CP];
-    __shared__ float2 kap[32];
+    __shared__ CircTab<float> tab;
     const int N = D.N;
     const int64_t NN = (int64_t)N * N;
     const int64_t r = blockIdx.y;
@@ -408,7 +480,7 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
         sre[a * CP + jj] = v.x;
         sim[a * CP + jj] = v.y;
     }
-    if (tid < 32) kap[tid] = tid < N ? D.kap[2 * N + tid] : czero<float>();
+    tab.fill(D.kap + 2 * N, N, tid);
     __syncthreads();
     f32x4 acc[4];
 #pragma unroll
@@ -425,8 +497,11 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
         for (int it = 0; it < 4; it++) {
             float av[8];
 #pragma unroll
-            for (int t = 0; t < 8; t++) av[t] = circ_entry<float>(kap, N, 16 * (it & 1) + lr, 32 * s + 8 * lq + t,
-                                                                  (it >> 1) != 0);
+            for (int t = 0; t < 8; t++) {
+                const int k = 32 * s + 8 * lq + t;
+                const float2 c = (k < 32 ? tab.re_in : tab.im_in)[16 * (it & 1) + lr + 32 - (k & 31)];
+                av[t] = (it >> 1) ? c.y : c.x;
+            }
             bf16x8 ah, al;
             split8(av, ah, al);
             acc[it] = mfma_bf16(al, bh, acc[it]);
@@ -447,21 +522,49 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
 
 // ---------------------------------------------------------- launchers ---
 
+// Workgroups of the persistent fused kernel: enough to fill every CU at the
+// kernel's occupancy, never more than there are slabs.
+template <class K> static unsigned persistent_grid(K kernel, int64_t nslab) {
+    static int cached = 0;  // one value per kernel instantiation
+    if (!cached) {
+        int dev = 0, ncu = 0, per_cu = 0;
+        HPUP_HIP(hipGetDevice(&dev));
+        HPUP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HPUP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+        cached = std::max(1, ncu * std::max(per_cu, 1));
+    }
+    return (unsigned)std::min<int64_t>(nslab, cached);
+}
+
+template <class T, class G, int DIM>
+static void launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3,
+                           const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab,
+                           hipStream_t st) {
+    auto kern = k_drift_fused<T, G, DIM>;
+    hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, nslab)), dim3(256), 0, st, D, MX, (const cT<T>*)Fin,
+                       (const cT<T>*)Z3, (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
+}
+
 template <class T>
 void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
                         void* Fout, void* ACC, const StageCoef<T>& sc, bool bf16x3, hipStream_t st) {
     HPUP_REQUIRE(D.N <= 32, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs N <= 32");
+    HPUP_REQUIRE(D.dim == 2 || D.dim == 3, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs dim 2 or 3");
     const int64_t nslab = D.nrc * (D.dim == 3 ? D.N : 1);
     if constexpr (sizeof(T) == 4) {
         if (bf16x3) {
-            hipLaunchKernelGGL((k_drift_fused<float, GemmBf16x3>), dim3((unsigned)nslab), dim3(256), 0, st, D, MX,
-                               (const cT<T>*)Fin, (const cT<T>*)Z3, (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc);
+            if (D.dim == 3)
+                launch_fused_t<float, GemmBf16x3, 3>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+            else
+                launch_fused_t<float, GemmBf16x3, 2>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
             return;
         }
     }
     HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 needs FP32 storage");
-    hipLaunchKernelGGL((k_drift_fused<T, GemmF<T>>), dim3((unsigned)nslab), dim3(256), 0, st, D, MX,
-                       (const cT<T>*)Fin, (const cT<T>*)Z3, (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc);
+    if (D.dim == 3)
+        launch_fused_t<T, GemmF<T>, 3>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+    else
+        launch_fused_t<T, GemmF<T>, 2>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
 }
 
 template <class T>
