@@ -52,7 +52,11 @@ template <> struct Mfma<double> {
     }
 };
 
-constexpr int kPad = 33;  // LDS row stride of the 32x32 planes
+// LDS row stride of the 32x32 planes.  With 34 (= 2 mod 32) the row-wise
+// operand reads of T1 (lane (lr, lq) reads row lr, column 4s + lq) and the
+// column-wise reads of T2 (lane reads row 8 lq + t, column lr) each hit 32
+// distinct banks per half-wave.
+constexpr int kPad = 34;
 
 // ------------------------------------------------------------ policies ---
 
@@ -86,8 +90,11 @@ template <class T> struct GemmF {
                                          int lr_, int lq_) {
         c1 = tab1; c2 = tab2;
         k2t = k2t_; k1t = k1t_; lr = lr_; lq = lq_;
-        b1 = 16 * k1t + lr + 32 - lq;  // d of reduction step s is b - 4 (s & 7)
-        b2 = 16 * k2t + lr + 32 - lq;
+        // T1 reduces over k = 4 (s & 7) + lq, T2 over k = 8 lq + (s & 7) (in each
+        // re/im half): the order of a GEMM's reduction is free, and these
+        // orders make the data reads conflict-free.  d = o - k + 32.
+        b1 = 16 * k1t + lr + 32 - lq;
+        b2 = 16 * k2t + lr + 32 - 8 * lq;
     }
     // Y = X C1^T along k1 (slab rows as the A operand)
     __device__ __forceinline__ void t1(const T* pr, const T* pi, cT<T>* out) const {
@@ -109,8 +116,8 @@ template <class T> struct GemmF {
         const int col = 16 * k1t + lr;
 #pragma unroll 4
         for (int s = 0; s < 16; s++) {
-            const T bv = (s < 8 ? pr : pi)[((4 * s + lq) & 31) * kPad + col];
-            const cT<T> f = (s < 8 ? c2->re_in : c2->im_in)[b2 - 4 * (s & 7)];
+            const T bv = (s < 8 ? pr : pi)[(8 * lq + (s & 7)) * kPad + col];
+            const cT<T> f = (s < 8 ? c2->re_in : c2->im_in)[b2 - (s & 7)];
             are = MF::fma(f.x, bv, are);
             aim = MF::fma(f.y, bv, aim);
         }
@@ -225,14 +232,22 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
 // second barrier, writes to the V planes follow this slab's first barrier.
 
 template <class T> struct MixBatch;  // source rows whose loads are issued together
-template <> struct MixBatch<float> { static constexpr int n = 4; };
+template <> struct MixBatch<float> { static constexpr int n = 3; };
 template <> struct MixBatch<double> { static constexpr int n = 2; };
 
 template <class T>
 __device__ __forceinline__ bool cnz(cT<T> c) { return c.x != T(0) || c.y != T(0); }
 
-template <class T, class G, int DIM>
-__global__ __launch_bounds__(256) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
+// Waves per SIMD the fused kernel is compiled for (the register budget):
+// 3 for FP32 (at most 168 VGPRs), 2 for FP64 and 3xBF16.
+template <class T, class G> struct FusedWaves { static constexpr int n = 2; };
+template <> struct FusedWaves<float, GemmF<float>> { static constexpr int n = 3; };
+
+// GRAV: the gravity term comes from the grid (density form, D.grav) instead
+// of the Clebsch-Gordan table; a separate instantiation so that the common
+// case carries none of its addressing.
+template <class T, class G, int DIM, bool GRAV>
+__global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
                                                      const cT<T>* __restrict__ Z3, const cT<T>* F0, cT<T>* Fout,
                                                      cT<T>* ACC, StageCoef<T> sc, int64_t nslab) {
     __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad], pc_re[32 * kPad],
@@ -251,17 +266,22 @@ __global__ __launch_bounds__(256) void k_drift_fused(DriftDev<T> D, MixDev<T> MX
     G gm;
     gm.init(&tabs[0], &tabs[1], k2t, k1t, lr, lq);
 
-    // this lane's four elements (k2v[v], k1) of every slab
+    // this lane's four elements (k2v[v], k1) of every slab; out-of-range
+    // elements (k >= N in the 32 x 32 padding) load a clamped in-range address
+    // and select zero, so no load is predicated
     const int k1 = 16 * k1t + lr;
     const bool k1ok = k1 < N;
+    const int k1c = k1ok ? k1 : N - 1;
     const T kc1 = k1ok ? D.kc[k1] : T(0), kd1 = k1ok ? D.kd[k1] : T(0);
     int k2v[4];
+    unsigned off[4];  // element offset inside a slab (uniform slab base + 32-bit offset)
     bool ok[4];
     T kc2v[4], kd2v[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         k2v[v] = 16 * k2t + G::orow(lq, v);
         ok[v] = k1ok && k2v[v] < N;
+        off[v] = (unsigned)((k2v[v] < N ? k2v[v] : N - 1) * N + k1c);
         kc2v[v] = ok[v] ? D.kc[k2v[v]] : T(0);
         kd2v[v] = ok[v] ? D.kd[k2v[v]] : T(0);
     }
@@ -270,30 +290,40 @@ __global__ __launch_bounds__(256) void k_drift_fused(DriftDev<T> D, MixDev<T> MX
     const T g22 = DIM == 3 ? D.G[8] : T(0), g02 = DIM == 3 ? D.G[2] + D.G[6] : T(0),
             g12 = DIM == 3 ? D.G[5] + D.G[7] : T(0);
 
-    for (int64_t sl = xcd_remap(blockIdx.x, gridDim.x); sl < nslab; sl += gridDim.x) {
-        const int64_t r = sl % D.nrc;
-        const int k3 = (int)(sl / D.nrc);
+    // slab sl = k3 * nrc + r, advanced by gridDim.x without dividing
+    const int64_t nrc = D.nrc, sl0 = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t dr = gridDim.x % nrc;
+    const int dk = (int)(gridDim.x / nrc);
+    int64_t r = sl0 % nrc;
+    int k3 = (int)(sl0 / nrc);
+    while (k3 < nz) {
         const int64_t base = (r * nz + k3) * NN;
+        const cT<T>* Fb = Fin + base;  // this slab
         const T kc3 = DIM == 3 ? D.kc[k3] : T(0), kd3 = DIM == 3 ? D.kd[k3] : T(0);
 
-        // slab -> plane A (zero padded to 32 x 32)
-        {
-            cT<T> own[4];
+        // the slab, this lane's elements of F and of T3 F: issued together
+        cT<T> own[4], fo[4], Zv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-                own[i] = (a2 < N && a1 < N) ? Fin[base + a2 * N + a1] : czero<T>();
-            }
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-                pa_re[a2 * kPad + a1] = own[i].x;
-                pa_im[a2 * kPad + a1] = own[i].y;
-            }
+        for (int i = 0; i < 4; i++) {
+            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+            const bool in = a2 < N && a1 < N;
+            const cT<T> x = Fb[in ? (unsigned)(a2 * N + a1) : 0u];
+            own[i] = in ? x : czero<T>();
         }
-        cT<T> Zv[4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) Zv[v] = (DIM == 3 && ok[v]) ? Z3[base + k2v[v] * N + k1] : czero<T>();
+        for (int v = 0; v < 4; v++) {
+            // padded elements (ok false) read a clamped in-range element: they
+            // only feed outputs that are discarded, and the V planes are zeroed
+            // there before the second GEMM pair
+            fo[v] = Fb[off[v]];
+            Zv[v] = DIM == 3 ? (Z3 + base)[off[v]] : czero<T>();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {  // slab -> plane A, zero padded to 32 x 32
+            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+            pa_re[a2 * kPad + a1] = own[i].x;
+            pa_im[a2 * kPad + a1] = own[i].y;
+        }
 
         // rotation mixing: sources in batches of MB rows, structurally zero
         // coefficients skipped (the branches are uniform over the workgroup).
@@ -315,8 +345,7 @@ __global__ __launch_bounds__(256) void k_drift_fused(DriftDev<T> D, MixDev<T> MX
                 src[u] = t0 + u < TS ? MX.src[r * TS + t0 + u] : -1;
                 const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : src[u]) * nz + k3) * NN;
 #pragma unroll
-                for (int v = 0; v < 4; v++)
-                    fv[u][v] = (src[u] >= 0 && ok[v]) ? Fs[k2v[v] * N + k1] : czero<T>();
+                for (int v = 0; v < 4; v++) fv[u][v] = Fs[off[v]];
             }
 #pragma unroll
             for (int u = 0; u < MB; u++) {
@@ -343,20 +372,18 @@ __global__ __launch_bounds__(256) void k_drift_fused(DriftDev<T> D, MixDev<T> MX
                     for (int v = 0; v < 4; v++) cfma<T>(V2[v], cu2, fv[u][v]);
             }
         }
-        // W += D(k) F - sum_p c_p (grid gravity term), at this lane's elements
-        // (the slab value itself re-read through the cache)
+        // W += D(k) F - sum_p c_p (grid gravity term)
         const T dd0 = g00 * kd1 + g22 * kd3 - g02 * kc1 * kc3;
         const T dd1 = -g01 * kc1 - g12 * kc3;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            if (!ok[v]) continue;
-            const int64_t o = base + k2v[v] * N + k1;
             const T dd = dd0 + g11 * kd2v[v] + dd1 * kc2v[v];
-            cT<T> wv = cadd<T>(Wv[v], cscale<T>(Fin[o], dd));
+            cT<T> wv = cadd<T>(Wv[v], cscale<T>(fo[v], dd));
             const T kcp[3] = {kc1, kc2v[v], kc3};
+            if constexpr (GRAV)
 #pragma unroll
-            for (int p = 0; p < DIM; p++)  // gravity evaluated on the grid (density form)
-                if (D.grav[p]) wv = csub<T>(wv, cimul<T>(D.grav[p][o], kcp[p]));
+                for (int p = 0; p < DIM; p++)  // gravity evaluated on the grid (density form)
+                    wv = csub<T>(wv, cimul<T>((D.grav[p] + base)[off[v]], kcp[p]));
             Wv[v] = wv;
         }
         __syncthreads();  // plane A complete; the V planes of the previous slab are consumed
@@ -367,47 +394,59 @@ __global__ __launch_bounds__(256) void k_drift_fused(DriftDev<T> D, MixDev<T> MX
         if (DIM == 3 && D.has_gyro) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                if (!ok[v]) continue;
                 V1[v] = csub<T>(V1[v], cimul<T>(Zv[v], kc2v[v] * D.gyro[1]));
                 V1[v] = csub<T>(V1[v], cimul<T>(P2[v], kc3 * D.gyro[2]));
                 V2[v] = csub<T>(V2[v], cimul<T>(Zv[v], kc1 * D.gyro[0]));
             }
         }
+        // every term but T1 V1 + T2 V2, summed now so that P, Z and W retire:
+        // S = W + b1 c1 P1 + b2 c2 P2 + (e3 + b3 c3) Z3
+        const cT<T> e3 = DIM == 3 ? D.e3[r] : czero<T>();
+        cT<T> S[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
+            cT<T> q = cadd<T>(Wv[v], cimul<T>(P1[v], D.b[0] * kc1));
+            q = cadd<T>(q, cimul<T>(P2[v], D.b[1] * kc2v[v]));
+            if (DIM == 3) {
+                q = cadd<T>(q, cmul<T>(Zv[v], e3));
+                q = cadd<T>(q, cimul<T>(Zv[v], D.b[2] * kc3));
+            }
+            S[v] = q;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {  // V planes, zero in the padding
             const int k2 = k2v[v];
-            pb_re[k2 * kPad + k1] = V1[v].x;
-            pb_im[k2 * kPad + k1] = V1[v].y;
-            pc_re[k2 * kPad + k1] = V2[v].x;
-            pc_im[k2 * kPad + k1] = V2[v].y;
+            pb_re[k2 * kPad + k1] = ok[v] ? V1[v].x : T(0);
+            pb_im[k2 * kPad + k1] = ok[v] ? V1[v].y : T(0);
+            pc_re[k2 * kPad + k1] = ok[v] ? V2[v].x : T(0);
+            pc_im[k2 * kPad + k1] = ok[v] ? V2[v].y : T(0);
         }
         __syncthreads();  // V planes complete; plane A is consumed
         cT<T> Q1[4], Q2[4];
         gm.t1(pb_re, pb_im, Q1);
         gm.t2(pc_re, pc_im, Q2);
 
-        const cT<T> e3 = DIM == 3 ? D.e3[r] : czero<T>();
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             if (!ok[v]) continue;
-            cT<T> q = cadd<T>(Q1[v], Q2[v]);
-            q = cadd<T>(q, Wv[v]);
-            q = cadd<T>(q, cimul<T>(P1[v], D.b[0] * kc1));
-            q = cadd<T>(q, cimul<T>(P2[v], D.b[1] * kc2v[v]));
-            if (DIM == 3) {
-                q = cadd<T>(q, cmul<T>(Zv[v], e3));
-                q = cadd<T>(q, cimul<T>(Zv[v], D.b[2] * kc3));
-            }
-            const int64_t o = base + k2v[v] * N + k1;
+            const cT<T> q = cadd<T>(cadd<T>(Q1[v], Q2[v]), S[v]);
+            const unsigned o = off[v];
             if (sc.deriv_only) {
-                Fout[o] = q;
+                (Fout + base)[o] = q;
                 continue;
             }
-            const cT<T> f0 = sc.read_f0 ? F0[o] : Fin[o];
-            const cT<T> ac = sc.read_acc ? ACC[o] : czero<T>();
-            const cT<T> upd = cadd<T>(cscale<T>(ac, sc.ca), cscale<T>(q, sc.cd));
-            Fout[o] = cadd<T>(f0, cscale<T>(upd, sc.dt));
-            if (sc.write_acc) ACC[o] = cadd<T>(cscale<T>(ac, sc.ea), cscale<T>(q, sc.ed));
+            // RK stages read F0 and ACC here; an Euler step reuses fo
+            const cT<T> f0v = sc.read_f0 ? (F0 + base)[o] : fo[v];
+            const cT<T> acv = sc.read_acc ? (ACC + base)[o] : czero<T>();
+            const cT<T> upd = cadd<T>(cscale<T>(acv, sc.ca), cscale<T>(q, sc.cd));
+            (Fout + base)[o] = cadd<T>(f0v, cscale<T>(upd, sc.dt));
+            if (sc.write_acc) (ACC + base)[o] = cadd<T>(cscale<T>(acv, sc.ea), cscale<T>(q, sc.ed));
+        }
+        r += dr;
+        k3 += dk;
+        if (r >= nrc) {
+            r -= nrc;
+            k3++;
         }
     }
 }
@@ -536,11 +575,11 @@ template <class K> static unsigned persistent_grid(K kernel, int64_t nslab) {
     return (unsigned)std::min<int64_t>(nslab, cached);
 }
 
-template <class T, class G, int DIM>
+template <class T, class G, int DIM, bool GRAV>
 static void launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3,
                            const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab,
                            hipStream_t st) {
-    auto kern = k_drift_fused<T, G, DIM>;
+    auto kern = k_drift_fused<T, G, DIM, GRAV>;
     hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, nslab)), dim3(256), 0, st, D, MX, (const cT<T>*)Fin,
                        (const cT<T>*)Z3, (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
 }
@@ -551,20 +590,25 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     HPUP_REQUIRE(D.N <= 32, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs N <= 32");
     HPUP_REQUIRE(D.dim == 2 || D.dim == 3, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs dim 2 or 3");
     const int64_t nslab = D.nrc * (D.dim == 3 ? D.N : 1);
+    const bool grav = D.grav[0] || D.grav[1] || D.grav[2];
     if constexpr (sizeof(T) == 4) {
         if (bf16x3) {
+            HPUP_REQUIRE(!grav, HPUP_ERR_UNSUPPORTED, "3xBF16 drift has no density-form gravity");
             if (D.dim == 3)
-                launch_fused_t<float, GemmBf16x3, 3>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+                launch_fused_t<float, GemmBf16x3, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
             else
-                launch_fused_t<float, GemmBf16x3, 2>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+                launch_fused_t<float, GemmBf16x3, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
             return;
         }
     }
     HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 needs FP32 storage");
+    HPUP_REQUIRE(!grav || D.dim == 2, HPUP_ERR_UNSUPPORTED, "density-form gravity is a 2-D path");
     if (D.dim == 3)
-        launch_fused_t<T, GemmF<T>, 3>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        launch_fused_t<T, GemmF<T>, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+    else if (grav)
+        launch_fused_t<T, GemmF<T>, 2, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
     else
-        launch_fused_t<T, GemmF<T>, 2>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        launch_fused_t<T, GemmF<T>, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
 }
 
 template <class T>
