@@ -239,9 +239,9 @@ template <class T>
 __device__ __forceinline__ bool cnz(cT<T> c) { return c.x != T(0) || c.y != T(0); }
 
 // Waves per SIMD the fused kernel is compiled for (the register budget):
-// 3 for FP32 (at most 168 VGPRs), 2 for FP64 and 3xBF16.
+// 4 for FP32 (at most 128 VGPRs), 2 for FP64 and 3xBF16.
 template <class T, class G> struct FusedWaves { static constexpr int n = 2; };
-template <> struct FusedWaves<float, GemmF<float>> { static constexpr int n = 3; };
+template <> struct FusedWaves<float, GemmF<float>> { static constexpr int n = 4; };
 
 // GRAV: the gravity term comes from the grid (density form, D.grav) instead
 // of the Clebsch-Gordan table; a separate instantiation so that the common
@@ -297,6 +297,21 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
     int64_t r = sl0 % nrc;
     int k3 = (int)(sl0 / nrc);
     while (k3 < nz) {
+        // zo is 0, produced by an opaque move each iteration: indices derived
+        // from it are recomputed per slab instead of being hoisted out of the
+        // loop (one register each for every address), which keeps the kernel
+        // within the register budget of its occupancy
+        int zo;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
+        const int tidz = tid + zo;
+        int k2w[4];
+        unsigned offw[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            k2w[v] = k2v[v] + zo;
+            offw[v] = off[v] + (unsigned)zo;
+        }
+        const int k1w = k1 + zo;
         const int64_t base = (r * nz + k3) * NN;
         const cT<T>* Fb = Fin + base;  // this slab
         const T kc3 = DIM == 3 ? D.kc[k3] : T(0), kd3 = DIM == 3 ? D.kd[k3] : T(0);
@@ -305,7 +320,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         cT<T> own[4], fo[4], Zv[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+            const int idx = tidz + 256 * i, a2 = idx >> 5, a1 = idx & 31;
             const bool in = a2 < N && a1 < N;
             const cT<T> x = Fb[in ? (unsigned)(a2 * N + a1) : 0u];
             own[i] = in ? x : czero<T>();
@@ -315,12 +330,12 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             // padded elements (ok false) read a clamped in-range element: they
             // only feed outputs that are discarded, and the V planes are zeroed
             // there before the second GEMM pair
-            fo[v] = Fb[off[v]];
-            Zv[v] = DIM == 3 ? (Z3 + base)[off[v]] : czero<T>();
+            fo[v] = Fb[offw[v]];
+            Zv[v] = DIM == 3 ? (Z3 + base)[offw[v]] : czero<T>();
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {  // slab -> plane A, zero padded to 32 x 32
-            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+            const int idx = tidz + 256 * i, a2 = idx >> 5, a1 = idx & 31;
             pa_re[a2 * kPad + a1] = own[i].x;
             pa_im[a2 * kPad + a1] = own[i].y;
         }
@@ -345,7 +360,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
                 src[u] = t0 + u < TS ? MX.src[r * TS + t0 + u] : -1;
                 const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : src[u]) * nz + k3) * NN;
 #pragma unroll
-                for (int v = 0; v < 4; v++) fv[u][v] = Fs[off[v]];
+                for (int v = 0; v < 4; v++) fv[u][v] = Fs[offw[v]];
             }
 #pragma unroll
             for (int u = 0; u < MB; u++) {
@@ -383,7 +398,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             if constexpr (GRAV)
 #pragma unroll
                 for (int p = 0; p < DIM; p++)  // gravity evaluated on the grid (density form)
-                    wv = csub<T>(wv, cimul<T>((D.grav[p] + base)[off[v]], kcp[p]));
+                    wv = csub<T>(wv, cimul<T>((D.grav[p] + base)[offw[v]], kcp[p]));
             Wv[v] = wv;
         }
         __syncthreads();  // plane A complete; the V planes of the previous slab are consumed
@@ -415,11 +430,11 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {  // V planes, zero in the padding
-            const int k2 = k2v[v];
-            pb_re[k2 * kPad + k1] = ok[v] ? V1[v].x : T(0);
-            pb_im[k2 * kPad + k1] = ok[v] ? V1[v].y : T(0);
-            pc_re[k2 * kPad + k1] = ok[v] ? V2[v].x : T(0);
-            pc_im[k2 * kPad + k1] = ok[v] ? V2[v].y : T(0);
+            const int k2 = k2w[v];
+            pb_re[k2 * kPad + k1w] = ok[v] ? V1[v].x : T(0);
+            pb_im[k2 * kPad + k1w] = ok[v] ? V1[v].y : T(0);
+            pc_re[k2 * kPad + k1w] = ok[v] ? V2[v].x : T(0);
+            pc_im[k2 * kPad + k1w] = ok[v] ? V2[v].y : T(0);
         }
         __syncthreads();  // V planes complete; plane A is consumed
         cT<T> Q1[4], Q2[4];
@@ -430,7 +445,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         for (int v = 0; v < 4; v++) {
             if (!ok[v]) continue;
             const cT<T> q = cadd<T>(cadd<T>(Q1[v], Q2[v]), S[v]);
-            const unsigned o = off[v];
+            const unsigned o = offw[v];
             if (sc.deriv_only) {
                 (Fout + base)[o] = q;
                 continue;
