@@ -320,21 +320,97 @@ template <class T> struct PlanT : PlanBase {
         }
     }
 
-    void stage(const void* Fin, const void* F0, void* Fout, const StageCoef<T>& sc, hipStream_t st) {
+    // One derivative evaluation with its integrator epilogue.  ev (nullable)
+    // receives an event before and after each kernel (nk + 1 events).
+    void stage(const void* Fin, const void* F0, void* Fout, const StageCoef<T>& sc, hipStream_t st,
+               hipEvent_t* ev = nullptr) {
         DriftDev<T> D = dr.view;
         // the fused kernel reads neighbouring rotation rows of Fin while other
         // workgroups write Fout, so the two must not alias (the three-pass
         // path reads Fin only through its own slab and may run in place)
         HPUP_REQUIRE(!(fused() && Fin == Fout), HPUP_ERR_ARG, "fused drift stage cannot run in place");
+        auto mark = [&](int k) {
+            if (ev) HPUP_HIP(hipEventRecord(ev[k], st));
+        };
         if (den) den_gravity(Fin, D, st);
         if (fused()) {
+            mark(0);
             if (D.dim == 3) launch_axis3_mfma<T>(D, Fin, Z3.p, bf16x3, st);
+            mark(1);
             launch_drift_fused<T>(D, dr.mix, Fin, Z3.p, F0, Fout, ACC.p, sc, bf16x3, st);
+            mark(2);
             return;
         }
+        mark(0);
         launch_rotmix<T>(D, Fin, Wb.p, V1.p, V2.p, st);
+        mark(1);
         if (D.dim == 3) launch_axis_outer<T>(Fin, Z3.p, dr.M3.p, D.N, (int64_t)D.N * D.N, D.nrc, T(1), st);
+        mark(2);
         launch_drift_slab<T>(D, Fin, Z3.p, Wb.p, V1.p, V2.p, F0, Fout, ACC.p, sc, st);
+        mark(3);
+    }
+    struct StageRef {
+        const void* in;
+        const void* f0;
+        void* out;
+        StageCoef<T> sc;
+    };
+    // The stages of one step of a Runge-Kutta method on the state F
+    // (pendulum.m:346-385); no fused stage writes its own input, so RK4's
+    // middle stages alternate Fs and Fs2 on the fused path.
+    int rk_stages(cT<T>* F, StageRef* sr) {
+        const T dt = (T)model.dt;
+        void* Fm = Fs.p;
+        if (model.method == HPUP_RK4 && fused()) {
+            ensure(Fs2, cbytes());
+            Fm = Fs2.p;
+        }
+        switch (model.method) {
+        case HPUP_MIDPOINT:
+            sr[0] = {F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 0, 0, 0, 0, 0}};
+            sr[1] = {Fs.p, F, F, StageCoef<T>{dt, 0, 1, 0, 0, 1, 0, 0, 0}};
+            return 2;
+        case HPUP_RK2:
+            sr[0] = {F, nullptr, Fs.p, StageCoef<T>{dt, 0, 1, 0, 1, 0, 0, 1, 0}};
+            sr[1] = {Fs.p, F, F, StageCoef<T>{dt / 2, 1, 1, 0, 0, 1, 1, 0, 0}};
+            return 2;
+        case HPUP_RK4:
+            sr[0] = {F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 1, 0, 0, 1, 0}};
+            sr[1] = {Fs.p, F, Fm, StageCoef<T>{dt / 2, 0, 1, 1, 2, 1, 1, 1, 0}};
+            sr[2] = {Fm, F, Fs.p, StageCoef<T>{dt, 0, 1, 1, 2, 1, 1, 1, 0}};
+            sr[3] = {Fs.p, F, F, StageCoef<T>{dt / 6, 1, 1, 0, 0, 1, 1, 0, 0}};
+            return 4;
+        default:
+            throw Error{HPUP_ERR_METHOD, "unknown integrator"};
+        }
+    }
+    // Runs nsteps steps of the plan's method; ev (nullable) receives nk + 1
+    // events per stage.  Euler steps alternate between F and Fs on the fused
+    // path, and an odd count ends with a copy back into F.
+    void run_steps(cT<T>* F, int nsteps, hipStream_t st, hipEvent_t* ev, int nk) {
+        const bool fz = fused();
+        if (model.method == HPUP_EULER) {
+            const StageCoef<T> sc{(T)model.dt, 0, 1, 0, 0, 0, 0, 0, 0};
+            cT<T>* cur = F;
+            cT<T>* nxt = fz ? Fs.as<cT<T>>() : F;
+            for (int it = 0; it < nsteps; it++) {
+                stage(cur, nullptr, nxt, sc, st, ev ? ev + (size_t)it * (nk + 1) : nullptr);
+                std::swap(cur, nxt);
+            }
+            if (fz && cur != F) HPUP_HIP(hipMemcpyAsync(F, cur, cbytes(), hipMemcpyDeviceToDevice, st));
+            return;
+        }
+        StageRef sr[4];
+        const int ns = rk_stages(F, sr);
+        for (int it = 0; it < nsteps; it++)
+            for (int k = 0; k < ns; k++)
+                stage(sr[k].in, sr[k].f0, sr[k].out, sr[k].sc, st,
+                      ev ? ev + ((size_t)it * ns + k) * (nk + 1) : nullptr);
+    }
+    int stages_per_step() {
+        if (model.method == HPUP_EULER) return 1;
+        StageRef sr[4];
+        return rk_stages(nullptr, sr);
     }
     void ensure_drift_buffers(bool rk) {
         HPUP_REQUIRE(has_drift, HPUP_ERR_ARG, "plan has no drift operator (dim must be 2 or 3)");
@@ -344,55 +420,8 @@ template <class T> struct PlanT : PlanBase {
         if (rk) { ensure(Fs, cbytes()); ensure(ACC, cbytes()); }
     }
     void step(void* Fv, int nsteps, hipStream_t st) override {
-        const int meth = model.method;
-        const bool fz = fused();
-        ensure_drift_buffers(meth != HPUP_EULER);
-        const T dt = (T)model.dt;
-        cT<T>* F = (cT<T>*)Fv;
-        if (meth == HPUP_EULER) {
-            const StageCoef<T> sc{dt, 0, 1, 0, 0, 0, 0, 0, 0};
-            if (!fz) {
-                for (int it = 0; it < nsteps; it++) stage(F, nullptr, F, sc, st);
-                return;
-            }
-            // Euler steps alternate between F and Fs; an odd count ends with a copy
-            ensure(Fs, cbytes());
-            cT<T>* cur = F;
-            cT<T>* nxt = Fs.as<cT<T>>();
-            for (int it = 0; it < nsteps; it++) {
-                stage(cur, nullptr, nxt, sc, st);
-                std::swap(cur, nxt);
-            }
-            if (cur != F) HPUP_HIP(hipMemcpyAsync(F, cur, cbytes(), hipMemcpyDeviceToDevice, st));
-            return;
-        }
-        // RK4's middle stages read and write the stage buffer: the fused path
-        // alternates it with Fs2
-        void* Fm = Fs.p;
-        if (meth == HPUP_RK4 && fz) {
-            ensure(Fs2, cbytes());
-            Fm = Fs2.p;
-        }
-        for (int it = 0; it < nsteps; it++) {
-            switch (meth) {
-            case HPUP_MIDPOINT:
-                stage(F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 0, 0, 0, 0, 0}, st);
-                stage(Fs.p, F, F, StageCoef<T>{dt, 0, 1, 0, 0, 1, 0, 0, 0}, st);
-                break;
-            case HPUP_RK2:
-                stage(F, nullptr, Fs.p, StageCoef<T>{dt, 0, 1, 0, 1, 0, 0, 1, 0}, st);
-                stage(Fs.p, F, F, StageCoef<T>{dt / 2, 1, 1, 0, 0, 1, 1, 0, 0}, st);
-                break;
-            case HPUP_RK4:
-                stage(F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 1, 0, 0, 1, 0}, st);
-                stage(Fs.p, F, Fm, StageCoef<T>{dt / 2, 0, 1, 1, 2, 1, 1, 1, 0}, st);
-                stage(Fm, F, Fs.p, StageCoef<T>{dt, 0, 1, 1, 2, 1, 1, 1, 0}, st);
-                stage(Fs.p, F, F, StageCoef<T>{dt / 6, 1, 1, 0, 0, 1, 1, 0, 0}, st);
-                break;
-            default:
-                throw Error{HPUP_ERR_METHOD, "unknown integrator"};
-            }
-        }
+        ensure_drift_buffers(true);
+        run_steps((cT<T>*)Fv, nsteps, st, nullptr, 0);
         HPUP_HIP(hipGetLastError());
     }
     void derivative(const void* F, void* dF, hipStream_t st) override {
@@ -400,50 +429,37 @@ template <class T> struct PlanT : PlanBase {
         stage(F, nullptr, dF, StageCoef<T>{1, 0, 1, 0, 0, 0, 0, 0, 1}, st);
         HPUP_HIP(hipGetLastError());
     }
+    // Per-kernel timing of nreps steps of the plan's method: an event before
+    // and after every kernel launch, all steps back to back after a few
+    // untimed ones (the device clocks up under sustained load), one
+    // synchronization at the end; ms[k] is the average per launch.
     int profile(void* Fv, int nreps, double* ms, int maxk, hipStream_t st) override {
-        ensure_drift_buffers(false);
+        ensure_drift_buffers(true);
         HPUP_REQUIRE(nreps >= 1 && maxk >= 3, HPUP_ERR_ARG, "profile: need nreps >= 1 and room for 3 kernels");
-        const DriftDev<T>& D = dr.view;
-        const StageCoef<T> sc{(T)model.dt, 0, 1, 0, 0, 0, 0, 0, 0};
-        const bool fz = fused();
-        const int nk = fz ? 2 : 3;
-        if (fz) ensure(Fs, cbytes());
-        void* cur = Fv;
-        void* nxt = Fs.p;
-        hipEvent_t ev[4];
-        for (auto& e : ev) HPUP_HIP(hipEventCreate(&e));
-        double acc[3] = {0, 0, 0};
+        HPUP_REQUIRE(!den, HPUP_ERR_UNSUPPORTED, "profile: density-form plans are not profiled");
+        const int nk = fused() ? 2 : 3;
+        const int ns = stages_per_step();
+        cT<T>* F = (cT<T>*)Fv;
+        std::vector<hipEvent_t> ev((size_t)nreps * ns * (nk + 1), nullptr);
         try {
-            for (int it = 0; it < nreps; it++) {
-                HPUP_HIP(hipEventRecord(ev[0], st));
-                if (fz) {
-                    if (D.dim == 3) launch_axis3_mfma<T>(D, cur, Z3.p, bf16x3, st);
-                    HPUP_HIP(hipEventRecord(ev[1], st));
-                    launch_drift_fused<T>(D, dr.mix, cur, Z3.p, nullptr, nxt, ACC.p, sc, bf16x3, st);
-                    HPUP_HIP(hipEventRecord(ev[2], st));
-                    std::swap(cur, nxt);
-                } else {
-                    launch_rotmix<T>(D, Fv, Wb.p, V1.p, V2.p, st);
-                    HPUP_HIP(hipEventRecord(ev[1], st));
-                    if (D.dim == 3) launch_axis_outer<T>(Fv, Z3.p, dr.M3.p, D.N, (int64_t)D.N * D.N, D.nrc, T(1), st);
-                    HPUP_HIP(hipEventRecord(ev[2], st));
-                    launch_drift_slab<T>(D, Fv, Z3.p, Wb.p, V1.p, V2.p, nullptr, Fv, ACC.p, sc, st);
-                    HPUP_HIP(hipEventRecord(ev[3], st));
-                }
-                HPUP_HIP(hipEventSynchronize(ev[nk]));
+            for (auto& e : ev) HPUP_HIP(hipEventCreate(&e));
+            run_steps(F, 3, st, nullptr, nk);
+            run_steps(F, nreps, st, ev.data(), nk);
+            HPUP_HIP(hipEventSynchronize(ev.back()));
+            double acc[3] = {0, 0, 0};
+            for (size_t l = 0; l < (size_t)nreps * ns; l++)
                 for (int k = 0; k < nk; k++) {
                     float t = 0.f;
-                    HPUP_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                    HPUP_HIP(hipEventElapsedTime(&t, ev[l * (nk + 1) + k], ev[l * (nk + 1) + k + 1]));
                     acc[k] += t;
                 }
-            }
+            for (int k = 0; k < nk; k++) ms[k] = acc[k] / ((double)nreps * ns);
         } catch (...) {
-            for (auto& e : ev) (void)hipEventDestroy(e);
+            for (auto& e : ev)
+                if (e) (void)hipEventDestroy(e);
             throw;
         }
         for (auto& e : ev) HPUP_HIP(hipEventDestroy(e));
-        if (cur != Fv) HPUP_HIP(hipMemcpyAsync(Fv, cur, cbytes(), hipMemcpyDeviceToDevice, st));
-        for (int k = 0; k < nk; k++) ms[k] = acc[k] / nreps;
         return nk;
     }
     const char* profile_name(int i) const override {
@@ -452,22 +468,33 @@ template <class T> struct PlanT : PlanBase {
         if (fused()) return (i >= 0 && i < 2) ? fz[i] : "";
         return (i >= 0 && i < 3) ? names[i] : "";
     }
+    // Algorithmic HBM bytes per launch of kernel i, averaged over the stages
+    // of the plan's method (each input read once, each output written once).
     double profile_bytes(int i) const override {
         const double c = (double)sizeof(cT<T>) * (double)ncoef;
         const bool d3 = model.dim == 3;
-        if (fused()) {
-            switch (i) {
-            case 0: return d3 ? 2 * c : 0.0;   // read F; write T3 F
-            case 1: return (d3 ? 3 : 2) * c;   // read F (+ T3 F); write F
-            default: return 0.0;
+        auto* self = const_cast<PlanT<T>*>(this);
+        StageRef sr[4];
+        int ns = 1;
+        if (model.method == HPUP_EULER)
+            sr[0].sc = StageCoef<T>{(T)model.dt, 0, 1, 0, 0, 0, 0, 0, 0};
+        else
+            ns = self->rk_stages(nullptr, sr);
+        double sum = 0.0;
+        for (int k = 0; k < ns; k++) {
+            const StageCoef<T>& sc = sr[k].sc;
+            // streams of the stage epilogue: F0, ACC read, ACC written
+            const double epi = (sc.read_f0 ? 1 : 0) + (sc.read_acc ? 1 : 0) + (sc.write_acc ? 1 : 0);
+            if (fused()) {
+                if (i == 0) sum += d3 ? 2 * c : 0.0;         // read F; write T3 F
+                else if (i == 1) sum += (d3 ? 3 + epi : 2 + epi) * c;  // read F (+ T3 F); write F
+            } else {
+                if (i == 0) sum += 4 * c;                    // read F; write W, -U1 F, -U2 F
+                else if (i == 1) sum += d3 ? 2 * c : 0.0;    // read F; write T3 F
+                else if (i == 2) sum += ((d3 ? 6 : 5) + epi) * c;  // read F, T3F, W, V1, V2; write F
             }
         }
-        switch (i) {
-        case 0: return 4 * c;                  // read F; write W, -U1 F, -U2 F
-        case 1: return d3 ? 2 * c : 0.0;       // read F; write T3 F
-        case 2: return (d3 ? 6 : 5) * c;       // read F, T3F, W, V1, V2; write F
-        default: return 0.0;
-        }
+        return sum / ns;
     }
     void full_to_compact(const void* a, void* b, hipStream_t st) override {
         launch_full_to_compact<T>(model.B, nx, a, b, st);

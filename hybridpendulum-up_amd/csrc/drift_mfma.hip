@@ -222,12 +222,23 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
     return x * q + (x < rem ? x : rem) + i;
 }
 
+// Data the fused kernel touches once (T3 F, F0, ACC, the output).  FP64
+// streams them non-temporally, which keeps the re-read neighbour rows in L2;
+// for FP32 the plain forms measured faster (the non-temporal forms cost
+// registers at the kernel's 4-waves budget).
+template <class T> __device__ __forceinline__ cT<T> ld_once(const cT<T>* p) {
+    if constexpr (sizeof(T) == 8) return ld_stream<T>(p);
+    else return *p;
+}
+template <class T> __device__ __forceinline__ void st_once(cT<T>* p, cT<T> v) {
+    if constexpr (sizeof(T) == 8) st_stream<T>(p, v);
+    else *p = v;
+}
+
 // ------------------------------------------------------- fused kernel ---
 //
-// Persistent: each workgroup builds its circulant fragments once and walks
-// slabs sl = first, first + gridDim.x, ... (sl = k3 * nrc + r), so the chip
-// works on one contiguous window of slabs at a time and the neighbour rows the
-// rotation mixing reads stay in L2.  Three LDS plane pairs (slab, V1, V2) need
+// Persistent: each workgroup builds its circulant tables once and walks a
+// strided sequence of slabs inside its XCD's share (see the slab order below).  Three LDS plane pairs (slab, V1, V2) need
 // two barriers per slab: writes to the slab planes follow the previous slab's
 // second barrier, writes to the V planes follow this slab's first barrier.
 
@@ -290,13 +301,24 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
     const T g22 = DIM == 3 ? D.G[8] : T(0), g02 = DIM == 3 ? D.G[2] + D.G[6] : T(0),
             g12 = DIM == 3 ? D.G[5] + D.G[7] : T(0);
 
-    // slab sl = k3 * nrc + r, advanced by gridDim.x without dividing
-    const int64_t nrc = D.nrc, sl0 = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t dr = gridDim.x % nrc;
-    const int dk = (int)(gridDim.x / nrc);
-    int64_t r = sl0 % nrc;
-    int k3 = (int)(sl0 / nrc);
-    while (k3 < nz) {
+    // Slab order.  Workgroups are dealt round robin over the 8 XCDs, each with
+    // its own L2: XCD x walks its own contiguous eighth of the slabs (sl =
+    // k3 * nrc + r), its workgroups side by side, so the neighbour rows (within
+    // 2B + 1 of r) that the mixing reads were loaded into that XCD's L2 moments
+    // earlier.  The launch uses a multiple of 8 workgroups (or fewer than 8).
+    // slab counts fit 32 bits (the launcher checks): 32-bit bookkeeping keeps
+    // the loop state in few scalar registers
+    const int nrc = (int)D.nrc;
+    const int nxcd = gridDim.x >= 8 ? 8 : 1;
+    const int per = (int)gridDim.x / nxcd;  // workgroups per XCD
+    const int xcd = (int)blockIdx.x % nxcd;
+    const int hi = (int)(nslab * (xcd + 1) / nxcd);
+    int sl = (int)(nslab * xcd / nxcd) + (int)blockIdx.x / nxcd;
+    const int dr = per % nrc;
+    const int dk = per / nrc;
+    int r = sl % nrc;
+    int k3 = sl / nrc;
+    while (sl < hi) {
         // zo is 0, produced by an opaque move each iteration: indices derived
         // from it are recomputed per slab instead of being hoisted out of the
         // loop (one register each for every address), which keeps the kernel
@@ -312,7 +334,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             offw[v] = off[v] + (unsigned)zo;
         }
         const int k1w = k1 + zo;
-        const int64_t base = (r * nz + k3) * NN;
+        const int64_t base = ((int64_t)r * nz + k3) * NN;
         const cT<T>* Fb = Fin + base;  // this slab
         const T kc3 = DIM == 3 ? D.kc[k3] : T(0), kd3 = DIM == 3 ? D.kd[k3] : T(0);
 
@@ -331,7 +353,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             // only feed outputs that are discarded, and the V planes are zeroed
             // there before the second GEMM pair
             fo[v] = Fb[offw[v]];
-            Zv[v] = DIM == 3 ? (Z3 + base)[offw[v]] : czero<T>();
+            Zv[v] = DIM == 3 ? ld_once<T>(Z3 + base + offw[v]) : czero<T>();
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {  // slab -> plane A, zero padded to 32 x 32
@@ -357,7 +379,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             cT<T> fv[MB][4];
 #pragma unroll
             for (int u = 0; u < MB; u++) {
-                src[u] = t0 + u < TS ? MX.src[r * TS + t0 + u] : -1;
+                src[u] = t0 + u < TS ? MX.src[(int64_t)r * TS + t0 + u] : -1;
                 const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : src[u]) * nz + k3) * NN;
 #pragma unroll
                 for (int v = 0; v < 4; v++) fv[u][v] = Fs[offw[v]];
@@ -365,7 +387,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
 #pragma unroll
             for (int u = 0; u < MB; u++) {
                 if (src[u] < 0) continue;
-                const cT<T>* cf = MX.coef + (r * TS + t0 + u) * NC;
+                const cT<T>* cf = MX.coef + ((int64_t)r * TS + t0 + u) * NC;
                 const cT<T> g0 = cf[0], g1 = cf[1], g2 = DIM == 3 ? cf[2] : czero<T>();
                 const bool h0 = cnz<T>(g0) || (DIM == 3 && cnz<T>(g2)), h1 = cnz<T>(g1);
                 if (h0 || h1) {
@@ -447,16 +469,18 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             const cT<T> q = cadd<T>(cadd<T>(Q1[v], Q2[v]), S[v]);
             const unsigned o = offw[v];
             if (sc.deriv_only) {
-                (Fout + base)[o] = q;
+                st_once<T>(Fout + base + o, q);
                 continue;
             }
-            // RK stages read F0 and ACC here; an Euler step reuses fo
-            const cT<T> f0v = sc.read_f0 ? (F0 + base)[o] : fo[v];
-            const cT<T> acv = sc.read_acc ? (ACC + base)[o] : czero<T>();
+            // RK stages read F0 and ACC here; an Euler step reuses fo.  These
+            // streams and the outputs are touched once: non-temporal.
+            const cT<T> f0v = sc.read_f0 ? ld_once<T>(F0 + base + o) : fo[v];
+            const cT<T> acv = sc.read_acc ? ld_once<T>(ACC + base + o) : czero<T>();
             const cT<T> upd = cadd<T>(cscale<T>(acv, sc.ca), cscale<T>(q, sc.cd));
-            (Fout + base)[o] = cadd<T>(f0v, cscale<T>(upd, sc.dt));
-            if (sc.write_acc) (ACC + base)[o] = cadd<T>(cscale<T>(acv, sc.ea), cscale<T>(q, sc.ed));
+            st_once<T>(Fout + base + o, cadd<T>(f0v, cscale<T>(upd, sc.dt)));
+            if (sc.write_acc) st_once<T>(ACC + base + o, cadd<T>(cscale<T>(acv, sc.ea), cscale<T>(q, sc.ed)));
         }
+        sl += per;
         r += dr;
         k3 += dk;
         if (r >= nrc) {
@@ -513,7 +537,7 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int k3 = 16 * h + MF::orow(lq, v);
-            if (k3 < N) out[(r * N + k3) * NN + j] = cmk<T>(acc[h][v], acc[h + 2][v]);
+            if (k3 < N) st_stream<T>(out + (r * N + k3) * NN + j, cmk<T>(acc[h][v], acc[h + 2][v]));
         }
 }
 
@@ -587,7 +611,9 @@ template <class K> static unsigned persistent_grid(K kernel, int64_t nslab) {
         HPUP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
         cached = std::max(1, ncu * std::max(per_cu, 1));
     }
-    return (unsigned)std::min<int64_t>(nslab, cached);
+    int64_t g = std::min<int64_t>(nslab, cached);
+    if (g >= 8) g -= g % 8;  // whole workgroups per XCD
+    return (unsigned)g;
 }
 
 template <class T, class G, int DIM, bool GRAV>
@@ -605,6 +631,7 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     HPUP_REQUIRE(D.N <= 32, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs N <= 32");
     HPUP_REQUIRE(D.dim == 2 || D.dim == 3, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs dim 2 or 3");
     const int64_t nslab = D.nrc * (D.dim == 3 ? D.N : 1);
+    HPUP_REQUIRE(nslab < ((int64_t)1 << 30), HPUP_ERR_UNSUPPORTED, "fused MFMA drift: too many slabs");
     const bool grav = D.grav[0] || D.grav[1] || D.grav[2];
     if constexpr (sizeof(T) == 4) {
         if (bf16x3) {

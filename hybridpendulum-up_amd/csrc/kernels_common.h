@@ -21,6 +21,20 @@ template <class T> __device__ __forceinline__ cT<T> cmul(cT<T> a, cT<T> b) {
 }
 template <class T> __device__ __forceinline__ cT<T> cconj(cT<T> a) { return cmk<T>(a.x, -a.y); }
 // acc += a * b
+// Streaming (non-temporal) access for data touched once per kernel, so that
+// it does not evict the rows other workgroups still re-read from L2.
+template <class T> __device__ __forceinline__ cT<T> ld_stream(const cT<T>* p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return cmk<T>(v.x, v.y);
+}
+template <class T> __device__ __forceinline__ void st_stream(cT<T>* p, cT<T> a) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
+}
 template <class T> __device__ __forceinline__ void cfma(cT<T>& acc, cT<T> a, cT<T> b) {
     acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
     acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
