@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int k3 = 16 * h + MF::orow(lq, v);
-            if (k3 < N) st_stream<T>(out + (r * N + k3) * NN + j, cmk<T>(acc[h][v], acc[h + 2][v]));
+            if (k3 < N) out[(r * N + k3) * NN + j] = cmk<T>(acc[h][v], acc[h + 2][v]);
         }
 }
 
