@@ -171,7 +171,8 @@ template <class T> struct DriftTables {
         v.e3 = e3.as<cT<T>>(); v.kap = kap.as<cT<T>>();
         v.kc = kc.as<T>(); v.kd = kd.as<T>();
         for (int i = 0; i < 9; i++) v.G[i] = (T)s.G[i];
-        for (int i = 0; i < 3; i++) { v.b[i] = (T)s.b[i]; v.gyro[i] = (T)s.gyro[i]; }
+        for (int i = 0; i < 3; i++) { v.b[i] = (T)s.b[i]; v.gyro[i] = (T)s.gyro[i]; v.c0[i] = (T)s.c0[i]; }
+        v.ramp = s.ramp ? 1 : 0;
         v.has_gyro = s.has_gyro ? 1 : 0;
         v.grav[0] = v.grav[1] = v.grav[2] = nullptr;
     }

@@ -33,6 +33,9 @@ template <class T> struct DriftDev {
     // density-form gravity: transforms of f mR_p on the coefficient layout
     // (nullptr when the gravity term comes from the Clebsch-Gordan table)
     const cT<T>* grav[3];
+    // every kap_p = c0[p] + i (real circulant): the real-circulant path
+    int ramp;
+    T c0[3];
 };
 
 // Integrator epilogue of the slab kernel:

@@ -68,18 +68,31 @@ constexpr int kPad = 34;
 // needs no padding logic.  CircTab<T>::fill builds it for one axis.
 template <class T> struct CircTab {
     cT<T> re_in[64], im_in[64];
+    cT<T> rim[64];  // (-Im kappa, Im kappa): the real circulant of the ramp path
     __device__ __forceinline__ void fill(const cT<T>* kap, int N, int tid) {
         if (tid < 64) {
             const int d = tid - 32;
             const cT<T> c = kap[((d % N) + N) % N];
             re_in[tid] = c;
             im_in[tid] = cmk<T>(-c.y, c.x);
+            rim[tid] = cmk<T>(-c.y, c.y);
         }
     }
 };
 
-// FP32 / FP64 MFMA, 16 reduction steps of K = 4; fragments read from LDS.
-template <class T> struct GemmF {
+// Sum over the four lanes l, l ^ 16, l ^ 32, l ^ 48 (the lq groups).
+template <class T> __device__ __forceinline__ T sum_lq(T x) {
+    x += __shfl_xor(x, 16, 64);
+    x += __shfl_xor(x, 32, 64);
+    return x;
+}
+
+// FP32 / FP64 MFMA; fragments read from LDS.  General circulants: 16
+// reduction steps of K = 4 over the (re | im) stacked slab.  RAMP (kappa =
+// c0 + i (real circulant)): 8 steps of two MFMAs (-Im C . X_im, Im C . X_re)
+// plus c0 times the axis sums of X, which the operand loads accumulate on the
+// way (the four lq lanes of a row or column hold a quarter each).
+template <class T, bool RAMP = false> struct GemmF {
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
     const CircTab<T>* c1;
@@ -96,10 +109,33 @@ template <class T> struct GemmF {
         b1 = 16 * k1t + lr + 32 - lq;
         b2 = 16 * k2t + lr + 32 - 8 * lq;
     }
-    // Y = X C1^T along k1 (slab rows as the A operand)
-    __device__ __forceinline__ void t1(const T* pr, const T* pi, cT<T>* out) const {
+    // Y = X C1^T along k1 (slab rows as the A operand); c0: Re kappa_1 (RAMP)
+    __device__ __forceinline__ void t1(const T* pr, const T* pi, cT<T>* out, T c0 = T(0)) const {
         acc_t are = MF::zero(), aim = MF::zero();
         const int row = (16 * k2t + lr) * kPad;
+        if constexpr (RAMP) {
+            T sr = T(0), si = T(0);
+#pragma unroll 1
+            for (int s = 0; s < 8; s++) {
+                const int col = 4 * s + lq;
+                const T xr = pr[row + col], xi = pi[row + col];
+                const cT<T> f = c1->rim[b1 - 4 * s];
+                are = MF::fma(xi, f.x, are);
+                aim = MF::fma(xr, f.y, aim);
+                sr += xr;
+                si += xi;
+            }
+            // row 16 k2t + lr summed over its 32 columns; output row orow(lq, v)
+            // is held by the lanes with lr = orow(lq, v)
+            sr = sum_lq<T>(sr);
+            si = sum_lq<T>(si);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int src = orow(lq, v);
+                out[v] = cmk<T>(are[v] + c0 * __shfl(sr, src, 64), aim[v] + c0 * __shfl(si, src, 64));
+            }
+            return;
+        }
 #pragma unroll 4
         for (int s = 0; s < 16; s++) {
             const T av = (s < 8 ? pr : pi)[row + ((4 * s + lq) & 31)];
@@ -110,10 +146,29 @@ template <class T> struct GemmF {
 #pragma unroll
         for (int v = 0; v < 4; v++) out[v] = cmk<T>(are[v], aim[v]);
     }
-    // Y = C2 X along k2 (slab columns as the B operand)
-    __device__ __forceinline__ void t2(const T* pr, const T* pi, cT<T>* out) const {
+    // Y = C2 X along k2 (slab columns as the B operand); c0: Re kappa_2 (RAMP)
+    __device__ __forceinline__ void t2(const T* pr, const T* pi, cT<T>* out, T c0 = T(0)) const {
         acc_t are = MF::zero(), aim = MF::zero();
         const int col = 16 * k1t + lr;
+        if constexpr (RAMP) {
+            T sr = T(0), si = T(0);
+#pragma unroll 1
+            for (int s = 0; s < 8; s++) {
+                const int rw = (8 * lq + s) * kPad + col;
+                const T xr = pr[rw], xi = pi[rw];
+                const cT<T> f = c2->rim[b2 - s];
+                are = MF::fma(f.x, xi, are);
+                aim = MF::fma(f.y, xr, aim);
+                sr += xr;
+                si += xi;
+            }
+            // column 16 k1t + lr summed over its 32 rows: this lane's own column
+            sr = sum_lq<T>(sr);
+            si = sum_lq<T>(si);
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[v] = cmk<T>(are[v] + c0 * sr, aim[v] + c0 * si);
+            return;
+        }
 #pragma unroll 4
         for (int s = 0; s < 16; s++) {
             const T bv = (s < 8 ? pr : pi)[(8 * lq + (s & 7)) * kPad + col];
@@ -171,7 +226,7 @@ struct GemmBf16x3 {
             split8(v[3], b1im_h[s], b1im_l[s]);
         }
     }
-    __device__ __forceinline__ void t1(const float* pr, const float* pi, float2* out) const {
+    __device__ __forceinline__ void t1(const float* pr, const float* pi, float2* out, float = 0.f) const {
         f32x4 are = {0.f, 0.f, 0.f, 0.f}, aim = {0.f, 0.f, 0.f, 0.f};
         const int row = (16 * k2t + lr) * kPad + 8 * lq;
 #pragma unroll
@@ -192,7 +247,7 @@ struct GemmBf16x3 {
 #pragma unroll
         for (int v = 0; v < 4; v++) out[v] = cmk<float>(are[v], aim[v]);
     }
-    __device__ __forceinline__ void t2(const float* pr, const float* pi, float2* out) const {
+    __device__ __forceinline__ void t2(const float* pr, const float* pi, float2* out, float = 0.f) const {
         f32x4 are = {0.f, 0.f, 0.f, 0.f}, aim = {0.f, 0.f, 0.f, 0.f};
         const int col = 16 * k1t + lr;
 #pragma unroll
@@ -252,7 +307,7 @@ __device__ __forceinline__ bool cnz(cT<T> c) { return c.x != T(0) || c.y != T(0)
 // Waves per SIMD the fused kernel is compiled for (the register budget):
 // 4 for FP32 (at most 128 VGPRs), 2 for FP64 and 3xBF16.
 template <class T, class G> struct FusedWaves { static constexpr int n = 2; };
-template <> struct FusedWaves<float, GemmF<float>> { static constexpr int n = 4; };
+template <bool R> struct FusedWaves<float, GemmF<float, R>> { static constexpr int n = 4; };
 
 // GRAV: the gravity term comes from the grid (density form, D.grav) instead
 // of the Clebsch-Gordan table; a separate instantiation so that the common
@@ -426,8 +481,8 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         __syncthreads();  // plane A complete; the V planes of the previous slab are consumed
 
         cT<T> P1[4], P2[4];
-        gm.t1(pa_re, pa_im, P1);
-        gm.t2(pa_re, pa_im, P2);
+        gm.t1(pa_re, pa_im, P1, D.c0[0]);
+        gm.t2(pa_re, pa_im, P2, D.c0[1]);
         if (DIM == 3 && D.has_gyro) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -460,8 +515,8 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         }
         __syncthreads();  // V planes complete; plane A is consumed
         cT<T> Q1[4], Q2[4];
-        gm.t1(pb_re, pb_im, Q1);
-        gm.t2(pc_re, pc_im, Q2);
+        gm.t1(pb_re, pb_im, Q1, D.c0[0]);
+        gm.t2(pc_re, pc_im, Q2, D.c0[1]);
 
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -518,16 +573,44 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
     acc_t acc[4];
 #pragma unroll
     for (int it = 0; it < 4; it++) acc[it] = MF::zero();
+    if (D.ramp) {
+        // kappa_3 = c0 + i (real circulant): 8 steps, plus c0 times the k3 sum
+        // of this lane's column (its four lq lanes hold a quarter each)
+        T sr = T(0), si = T(0);
 #pragma unroll 4
-    for (int s = 0; s < 16; s++) {
-        const int k = 4 * s + lq;
-        const T bv = (s < 8 ? sre : sim)[(k & 31) * CP + 16 * w + lr];
-        const cT<T>* t = s < 8 ? tab.re_in : tab.im_in;
+        for (int s = 0; s < 8; s++) {
+            const int k = 4 * s + lq;
+            const T xr = sre[k * CP + 16 * w + lr], xi = sim[k * CP + 16 * w + lr];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const cT<T> f = t[16 * h + lr + 32 - (k & 31)];
-            acc[h] = MF::fma(f.x, bv, acc[h]);
-            acc[h + 2] = MF::fma(f.y, bv, acc[h + 2]);
+            for (int h = 0; h < 2; h++) {
+                const cT<T> f = tab.rim[16 * h + lr + 32 - k];
+                acc[h] = MF::fma(f.x, xi, acc[h]);
+                acc[h + 2] = MF::fma(f.y, xr, acc[h + 2]);
+            }
+            sr += xr;
+            si += xi;
+        }
+        sr = sum_lq<T>(sr) * D.c0[2];
+        si = sum_lq<T>(si) * D.c0[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                acc[h][v] += sr;
+                acc[h + 2][v] += si;
+            }
+    } else {
+#pragma unroll 4
+        for (int s = 0; s < 16; s++) {
+            const int k = 4 * s + lq;
+            const T bv = (s < 8 ? sre : sim)[(k & 31) * CP + 16 * w + lr];
+            const cT<T>* t = s < 8 ? tab.re_in : tab.im_in;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const cT<T> f = t[16 * h + lr + 32 - (k & 31)];
+                acc[h] = MF::fma(f.x, bv, acc[h]);
+                acc[h + 2] = MF::fma(f.y, bv, acc[h + 2]);
+            }
         }
     }
     const int64_t j = j0 + 16 * w + lr;
@@ -645,6 +728,15 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     }
     HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 needs FP32 storage");
     HPUP_REQUIRE(!grav || D.dim == 2, HPUP_ERR_UNSUPPORTED, "density-form gravity is a 2-D path");
+    if (D.ramp) {
+        if (D.dim == 3)
+            launch_fused_t<T, GemmF<T, true>, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        else if (grav)
+            launch_fused_t<T, GemmF<T, true>, 2, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        else
+            launch_fused_t<T, GemmF<T, true>, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        return;
+    }
     if (D.dim == 3)
         launch_fused_t<T, GemmF<T>, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
     else if (grav)

@@ -110,6 +110,12 @@ struct DriftSpec {
     bool has_gyro = false;
     // circulant kernels kappa_p[j] (complex), T_p F[k] = sum_a kappa_p[(k_p - a_p) mod N] F[..a_p..]
     std::vector<double> kappa;  // 2 * N * dim
+    // Every kappa_p has a constant real part c0[p] (the DFT of a coordinate
+    // ramp does: Re kappa = -h/2 at every index), so T_p F = c0_p (sum of F
+    // along axis p) + i (Im kappa_p circulant) F: one real circulant instead of
+    // a complex one.
+    bool ramp = false;
+    double c0[3] = {0, 0, 0};
     // Clebsch-Gordan gravity terms per compact output r: up to TG terms (src, coef[dim] complex)
     int TG = 0;
     std::vector<int32_t> cg_src;   // nRc * TG, -1 padded

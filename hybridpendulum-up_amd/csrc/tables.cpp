@@ -200,6 +200,15 @@ DriftSpec build_drift_spec_from_arrays(int B, int dim, int N, double L, const do
         }
     }
     auto kap = [&](int p, int j) { return cd(s.kappa[2 * (N * p + j)], s.kappa[2 * (N * p + j) + 1]); };
+    // constant real part on every axis -> the kernels' real-circulant path
+    s.ramp = true;
+    for (int p = 0; p < dim; p++) {
+        double mx = 0.0, dev = 0.0;
+        for (int j = 0; j < N; j++) mx = std::max(mx, std::abs(kap(p, j)));
+        for (int j = 0; j < N; j++) dev = std::max(dev, std::abs(kap(p, j).real() - kap(p, 0).real()));
+        s.c0[p] = kap(p, 0).real();
+        if (!(dev <= 1e-3 * tol * mx)) s.ramp = false;
+    }
 
     // ---- OJO: separable gyroscopic coefficients
     if (dim == 3 && OJOr) {
