@@ -37,13 +37,15 @@ MEASURED_TRAFFIC = {
 
 
 def kernel_flops(name, B, N):
-    """Algorithmic FLOPs per launch: every circulant along an omega axis costs
-    N^3 complex MACs (8 real FLOPs each) per (r, k3) slab of N^2 points; the
-    fused kernel applies four per slab (T1 F, T2 F, T1 V1, T2 V2), the axis-3
-    kernel one per slab.  nRc = B(2B-1)(2B+1)/3 rotation rows, N slabs each."""
+    """Algorithmic MFMA FLOPs per launch.  Each circulant along an omega axis
+    acts on a (r, k3) slab of N^2 points; kappa_p = c0 + i (real circulant) for
+    the coordinate ramps of this workload, so one application is two real
+    N^3-MAC products (4 N^3 FLOPs) plus an axis sum.  The fused kernel applies
+    four per slab (T1 F, T2 F, T1 V1, T2 V2), the axis-3 kernel one; nRc =
+    B(2B-1)(2B+1)/3 rotation rows, N slabs each."""
     nrc = B * (2 * B - 1) * (2 * B + 1) // 3
     per = {"k_drift_fused": 4, "k_axis3_mfma": 1}.get(name, 0)
-    return per * 8.0 * N ** 3 * nrc * N
+    return per * 4.0 * N ** 3 * nrc * N
 
 CFG = dict(B=15, dim=3, N=30, L=10.0, b=(0.2, 0.2, 0.2), G=0.1, Jt=(1.0, 0.93539, 0.24935), dt=0.004,
            s=(10.0, 5.0, 1.0), seed=2026)
