@@ -42,7 +42,7 @@ template <class T> std::vector<T> cast_vec(const std::vector<double>& v) { retur
 
 // Transform tables from a Wigner-d table d (M,M,B,2B) and weights w (2B).
 template <class T> struct XformTables {
-    DevBuf Wf, Wi, twB, dftN, idftN, twNf, twNi;
+    DevBuf Wf, Wi, twB, twMB, dftN, idftN, twNf, twNi;
     XformDev<T> view{};
     void build(int B, int dim, int N, const double* d, const double* w) {
         const int M = 2 * B - 1, n2 = 2 * B;
@@ -60,6 +60,16 @@ template <class T> struct XformTables {
         std::vector<T> tb(2 * n2), dn, idn, tf, ti;
         for (int t = 0; t < n2; t++) { tb[2 * t] = (T)std::cos(kPi * t / B); tb[2 * t + 1] = (T)std::sin(kPi * t / B); }
         twB = upload(tb);
+        if (n2 <= 32) {  // row r = frequency r - (B-1), column t = grid angle pi t / B, zero padded to 32 x 32
+            std::vector<T> tm(2 * 32 * 32, T(0));
+            for (int r = 0; r < M; r++)
+                for (int t = 0; t < n2; t++) {
+                    const int e = (((r - (B - 1)) * t) % n2 + n2) % n2;
+                    tm[2 * (r * 32 + t)] = tb[2 * e];
+                    tm[2 * (r * 32 + t) + 1] = tb[2 * e + 1];
+                }
+            twMB = upload(tm);
+        }
         const int Nn = std::max(N, 1);
         dn.resize(2 * (size_t)Nn * Nn); idn.resize(2 * (size_t)Nn * Nn); tf.resize(2 * Nn); ti.resize(2 * Nn);
         for (int k = 0; k < Nn; k++) {
@@ -78,6 +88,7 @@ template <class T> struct XformTables {
         view.B = B; view.dim = dim; view.N = Nn;
         view.nx = ipow64(Nn, dim); view.nrc = nrc_of(B);
         view.Wf = Wf.as<T>(); view.Wi = Wi.as<T>(); view.twB = twB.as<cT<T>>();
+        view.twMB = n2 <= 32 ? twMB.as<cT<T>>() : nullptr;
         view.dftN = dftN.as<cT<T>>(); view.idftN = idftN.as<cT<T>>();
         view.twNf = twNf.as<cT<T>>(); view.twNi = twNi.as<cT<T>>();
     }

@@ -9,6 +9,8 @@
 //   3. omega DFT of every compact row (slab kernel on axes 1,2; axis kernel on 3)
 // Inverse: the same stages reversed (1/nx on omega, unnormalized over alpha,
 // gamma, real part), Nyquist bins of alpha and gamma implicitly zero.
+#include <algorithm>
+
 #include "drift.h"
 #include "kernels_common.h"
 
@@ -145,32 +147,49 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
     }
 }
 
-template <class T>
+// Inverse over (alpha, gamma) for one beta_j and XC omega points:
+//   stage 1: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}: a task per (m, x,
+//            chunk of GC gammas) streams its G row once, the GC outputs in
+//            registers, twiddle indices advanced incrementally;
+//   stage 2: f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a} from LDS.
+template <class T, int GC>
 __global__ __launch_bounds__(256) void k_inv_ag(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f, int XC) {
     extern __shared__ char smem_raw[];
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);      // [m][g][xl]
-    cT<T>* tw = sH + (size_t)M * n2 * XC;                 // n2
+    cT<T>* tw = sH + (size_t)M * n2 * XC;                 // n2: e^{i pi t / B}
     const int j = blockIdx.y;
     const int64_t x0 = (int64_t)blockIdx.x * XC;
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
     for (int t = tid; t < n2; t += blockDim.x) tw[t] = X.twB[t];
     __syncthreads();
-    // over n -> gamma: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}  (G rows read through the cache)
-    for (int i = tid; i < M * n2 * XC; i += blockDim.x) {
-        const int xl = i % XC, g = (i / XC) % n2, mi = i / (XC * n2);
+    const int ngc = (n2 + GC - 1) / GC;
+    for (int task = tid; task < M * ngc * XC; task += blockDim.x) {
+        const int xl = task % XC, gc = (task / XC) % ngc, mi = task / (XC * ngc);
         const int64_t x = x0 + xl;
-        cT<T> acc = czero<T>();
+        const int g0 = gc * GC;
+        cT<T> acc[GC];
+#pragma unroll
+        for (int k = 0; k < GC; k++) acc[k] = czero<T>();
         if (x < nx) {
+            const cT<T>* Gp = G + (int64_t)mi * n2 * nx + (int64_t)j * nx + x;
             for (int ni = 0; ni < M; ni++) {
+                const cT<T> gv = Gp[(int64_t)M * ni * n2 * nx];
                 const int n = ni - lmax;
-                const int t = (int)((((int64_t)n * g) % n2 + n2) % n2);
-                const cT<T> gv = G[((int64_t)mi + (int64_t)M * ni) * n2 * nx + (int64_t)j * nx + x];
-                cfma<T>(acc, gv, cconj<T>(tw[t]));
+                const int dn = ((n % n2) + n2) % n2;  // (n g) mod n2 advances by dn per gamma
+                int t = (int)(((int64_t)dn * g0) % n2);
+#pragma unroll
+                for (int k = 0; k < GC; k++) {
+                    cfma<T>(acc[k], gv, cconj<T>(tw[t]));
+                    t += dn;
+                    if (t >= n2) t -= n2;
+                }
             }
         }
-        sH[i] = acc;
+#pragma unroll
+        for (int k = 0; k < GC; k++)
+            if (g0 + k < n2) sH[((size_t)mi * n2 + g0 + k) * XC + xl] = acc[k];
     }
     __syncthreads();
     // over m -> alpha, real part
@@ -178,13 +197,14 @@ __global__ __launch_bounds__(256) void k_inv_ag(XformDev<T> X, const cT<T>* __re
         const int a = i % n2, g = (i / n2) % n2, xl = i / (n2 * n2);
         const int64_t x = x0 + xl;
         if (x >= nx) continue;
+        int t = ((-lmax * a) % n2 + n2) % n2;  // (m a) mod n2 for m = -lmax, then + a per m
         T acc = T(0);
         for (int mi = 0; mi < M; mi++) {
-            const int m = mi - lmax;
-            const int t = (int)((((int64_t)m * a) % n2 + n2) % n2);
             const cT<T> h = sH[((size_t)mi * n2 + g) * XC + xl];
             const cT<T> w = tw[t];  // e^{-i m alpha} = conj(w): Re(h conj(w)) = h.x w.x + h.y w.y
             acc = fma(h.x, w.x, fma(h.y, w.y, acc));
+            t += a;
+            if (t >= n2) t -= n2;
         }
         f[a + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x))] = acc;
     }
@@ -277,6 +297,205 @@ __global__ void k_compact_to_full(int B, int64_t nx, const cT<T>* __restrict__ c
     }
 }
 
+// ------------------------------------- (alpha, gamma) stages for B <= 16 --
+//
+// Work items hold one row of the stage's input in registers and sweep a
+// frequency x angle twiddle table E[r][t] = e^{i pi (r - lmax) t / B}, zero
+// padded to 32 x 32, so every loop runs a fixed 32 (or 8) steps without
+// guards.  The table row and column of each step depend only on loop
+// counters and on the wave's part of the block, so the table is read through
+// the scalar cache (uniform addresses) instead of LDS.  A stage whose work
+// items fill only part of the block splits its output range into chunks of
+// 8 over wave-aligned parts.  Blocks are numbered so that consecutive omega
+// chunks of one beta_j land on the same XCD (the hardware deals blocks to the
+// 8 XCDs round-robin): a cache line shared by neighbouring chunks is then
+// fetched into one L2 only.
+
+constexpr int kAgMax = 32;  // 2B limit of the register-row kernels
+
+// table in the constant address space: uniform indices become scalar loads
+template <class T> using cTab = __attribute__((address_space(4))) const T*;
+template <class T> __device__ __forceinline__ cT<T> etw(cTab<T> e, int i) { return cmk<T>(e[2 * i], e[2 * i + 1]); }
+
+// block b of nb -> work index, a bijection with XCD k = b % 8 owning a
+// contiguous range of work indices (the last nb % 8 blocks keep their index)
+__device__ __forceinline__ int xcd_major(int b, int nb) {
+    const int per = nb >> 3;
+    if (b >= per << 3) return b;
+    return (b & 7) * per + (b >> 3);
+}
+
+// tasks of a stage -> parts of the block: <= 64 -> 4, <= 128 -> 2, else 1
+__device__ __forceinline__ int stage_parts(int T) { return T <= 64 ? 4 : (T <= 128 ? 2 : 1); }
+
+template <class T, int XC>
+__global__ __launch_bounds__(256) void k_fwd_ag_r(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
+                                                  int wstride, cT<T>* __restrict__ G, int nxc) {
+    extern __shared__ char smem_raw[];
+    const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
+    const int rp = n2 + 1;                                   // padded row of sf
+    T* sf = reinterpret_cast<T*>(smem_raw);                  // [xl][g][a]  XC*n2*rp
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);          // [m>=0][g][xl], aliases sf
+    const cTab<T> E = (cTab<T>)(const T*)X.twMB;
+    const int w = xcd_major(blockIdx.x, gridDim.x);
+    const int j = w / nxc;
+    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    const int64_t nx = X.nx;
+    const int tid = threadIdx.x;
+    const int64_t n2c = (int64_t)n2 * n2 * n2;
+    for (int i = tid; i < XC * n2 * n2; i += 256) {
+        const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
+        const int64_t x = x0 + xl;
+        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
+        T v = x < nx ? f[iR + n2c * x] : T(0);
+        if (wR) v *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
+        sf[(xl * n2 + g) * rp + a] = v;
+    }
+    __syncthreads();
+    // alpha DFT, m >= 0: H[m][g] = sum_a f[a] e^{+i m alpha_a}; item (g, xl)
+    {
+        const int TA = n2 * XC, SA = stage_parts(TA), PA = 256 / SA;
+        const int part = __builtin_amdgcn_readfirstlane(tid / PA), task = tid % PA;
+        const bool act = task < TA;
+        const int g = task / XC, xl = task % XC;
+        T row[kAgMax];
+#pragma unroll
+        for (int a = 0; a < kAgMax; a++) row[a] = (act && a < n2) ? sf[(xl * n2 + g) * rp + a] : T(0);
+        __syncthreads();  // sH overwrites sf
+        for (int c = part; c * 8 < B; c += SA) {
+            const int m0 = c * 8;
+            cT<T> acc[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const cTab<T> e = E + 2 * (m0 + k + lmax) * kAgMax;
+                acc[k] = czero<T>();
+#pragma unroll
+                for (int a = 0; a < kAgMax; a++) {
+                    const cT<T> tw = etw<T>(e, a);
+                    acc[k].x = fma(row[a], tw.x, acc[k].x);
+                    acc[k].y = fma(row[a], tw.y, acc[k].y);
+                }
+            }
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (m0 + k < B) sH[((m0 + k) * n2 + g) * XC + xl] = acc[k];
+            }
+        }
+    }
+    __syncthreads();
+    // gamma DFT: G[m][n] = sum_g H[m][g] e^{+i n gamma_g}, G[-m][-n] = conj; item (m, xl)
+    {
+        const int TB = B * XC, SB = stage_parts(TB), PB = 256 / SB;
+        const int part = __builtin_amdgcn_readfirstlane(tid / PB), task = tid % PB;
+        const bool act = task < TB;
+        const int m = task / XC, xl = task % XC;
+        const int64_t x = x0 + xl;
+        cT<T> h[kAgMax];
+#pragma unroll
+        for (int g = 0; g < kAgMax; g++) h[g] = (act && g < n2) ? sH[(m * n2 + g) * XC + xl] : czero<T>();
+        const bool st = act && x < nx;
+        for (int c = part; c * 8 < M; c += SB) {
+            const int ni0 = c * 8;
+            cT<T> acc[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const cTab<T> e = E + 2 * (ni0 + k) * kAgMax;
+                acc[k] = czero<T>();
+#pragma unroll
+                for (int g = 0; g < kAgMax; g++) cfma<T>(acc[k], h[g], etw<T>(e, g));
+            }
+            if (st) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int ni = ni0 + k;
+                    if (ni < M) {
+                        G[((int64_t)(m + lmax) + M * ni) * n2 * nx + (int64_t)j * nx + x] = acc[k];
+                        if (m > 0)
+                            G[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * nx + (int64_t)j * nx + x] =
+                                cconj<T>(acc[k]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class T, int XC>
+__global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
+                                                  int nxc) {
+    extern __shared__ char smem_raw[];
+    const int B = X.B, n2 = 2 * B, M = 2 * B - 1;
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl]
+    const cTab<T> E = (cTab<T>)(const T*)X.twMB;
+    const int w = xcd_major(blockIdx.x, gridDim.x);
+    const int j = w / nxc;
+    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    const int64_t nx = X.nx;
+    const int tid = threadIdx.x;
+    // H[m][g] = sum_n G[m][n] e^{-i n gamma_g}; item (m, xl) holds its G row
+    {
+        const int T1 = M * XC, S1 = stage_parts(T1), P1 = 256 / S1;
+        const int part = __builtin_amdgcn_readfirstlane(tid / P1), task = tid % P1;
+        const bool act = task < T1;
+        const int mi = task / XC, xl = task % XC;
+        const int64_t x = x0 + xl;
+        const bool ld = act && x < nx;
+        cT<T> gr[kAgMax];
+#pragma unroll
+        for (int ni = 0; ni < kAgMax; ni++)
+            gr[ni] = (ld && ni < M) ? G[((int64_t)mi + (int64_t)M * ni) * n2 * nx + (int64_t)j * nx + x] : czero<T>();
+        for (int c = part; c * 8 < n2; c += S1) {
+            const int g0 = c * 8;
+            cT<T> acc[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc[k] = czero<T>();
+#pragma unroll
+            for (int ni = 0; ni < kAgMax; ni++) {
+                const cTab<T> e = E + 2 * (ni * kAgMax + g0);
+#pragma unroll
+                for (int k = 0; k < 8; k++) cfma<T>(acc[k], gr[ni], cconj<T>(etw<T>(e, k)));
+            }
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (g0 + k < n2) sH[(mi * n2 + g0 + k) * XC + xl] = acc[k];
+            }
+        }
+    }
+    __syncthreads();
+    // f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a}; item (g, xl)
+    {
+        const int T2 = n2 * XC, S2 = stage_parts(T2), P2 = 256 / S2;
+        const int part = __builtin_amdgcn_readfirstlane(tid / P2), task = tid % P2;
+        const bool act = task < T2;
+        const int g = task / XC, xl = task % XC;
+        const int64_t x = x0 + xl;
+        const bool st = act && x < nx;
+        for (int c = part; c * 8 < n2; c += S2) {
+            const int a0 = c * 8;
+            T acc[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc[k] = T(0);
+            for (int mi = 0; mi < M; mi++) {
+                const cT<T> hv = act ? sH[(mi * n2 + g) * XC + xl] : czero<T>();
+                const cTab<T> e = E + 2 * (mi * kAgMax + a0);
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const cT<T> tw = etw<T>(e, k);  // Re(h conj(tw))
+                    acc[k] = fma(hv.x, tw.x, fma(hv.y, tw.y, acc[k]));
+                }
+            }
+            if (st) {
+                T* fo = f + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x));
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (a0 + k < n2) fo[a0 + k] = acc[k];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------- launchers -----
 
 static int pick_xc(size_t per_x, size_t budget, int64_t nx) {
@@ -286,9 +505,23 @@ static int pick_xc(size_t per_x, size_t budget, int64_t nx) {
     return std::max(xc, 1);
 }
 
+// omega points per block of the register-row (alpha, gamma) kernels
+constexpr int kFwdXC = 8;
+template <class T> constexpr int inv_xc() { return sizeof(T) == 8 ? 4 : 8; }
+
 template <class T>
 void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
     const int n2 = 2 * X.B;
+    if (n2 <= kAgMax) {
+        constexpr int XC = kFwdXC;
+        const int nxc = (int)((X.nx + XC - 1) / XC);
+        HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+        const size_t sh = std::max(sizeof(T) * XC * n2 * (n2 + 1), sizeof(cT<T>) * X.B * n2 * XC);
+        allow_lds((const void*)k_fwd_ag_r<T, XC>, sh);
+        hipLaunchKernelGGL((k_fwd_ag_r<T, XC>), dim3((unsigned)(nxc * n2)), dim3(256), sh, st, X, (const T*)f,
+                           (const T*)wR, wstride, (cT<T>*)G, nxc);
+        return;
+    }
     const size_t per_x = sizeof(T) * n2 * n2 + sizeof(cT<T>) * n2 * X.B;
     const int XC = pick_xc(per_x, kLdsTile - sizeof(cT<T>) * n2, X.nx);
     const size_t sh = per_x * XC + sizeof(cT<T>) * n2;
@@ -316,12 +549,26 @@ void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t s
 template <class T>
 void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
     const int n2 = 2 * X.B, M = 2 * X.B - 1;
+    if (n2 <= kAgMax) {
+        constexpr int XC = inv_xc<T>();
+        const int nxc = (int)((X.nx + XC - 1) / XC);
+        HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+        const size_t sh = sizeof(cT<T>) * M * n2 * XC;
+        allow_lds((const void*)k_inv_ag_r<T, XC>, sh);
+        hipLaunchKernelGGL((k_inv_ag_r<T, XC>), dim3((unsigned)(nxc * n2)), dim3(256), sh, st, X, (const cT<T>*)G,
+                           (T*)f, nxc);
+        return;
+    }
+    // gamma chunks of 16 for 2B > 32
+    constexpr int GC = 16;
+    const int ngc = (n2 + GC - 1) / GC;
     const size_t per_x = sizeof(cT<T>) * ((size_t)M * n2);
-    const int XC = pick_xc(per_x, kLdsTile - sizeof(cT<T>) * n2, X.nx);
+    int XC = std::max(1, 256 / (M * ngc));
+    XC = std::min<int>(XC, pick_xc(per_x, kLdsTile - sizeof(cT<T>) * n2, X.nx));
     const size_t sh = per_x * XC + sizeof(cT<T>) * n2;
-    allow_lds((const void*)k_inv_ag<T>, sh);
     dim3 grid((unsigned)((X.nx + XC - 1) / XC), (unsigned)n2);
-    hipLaunchKernelGGL(k_inv_ag<T>, grid, dim3(256), sh, st, X, (const cT<T>*)G, (T*)f, XC);
+    allow_lds((const void*)k_inv_ag<T, GC>, sh);
+    hipLaunchKernelGGL((k_inv_ag<T, GC>), grid, dim3(256), sh, st, X, (const cT<T>*)G, (T*)f, XC);
 }
 
 template <class T>
