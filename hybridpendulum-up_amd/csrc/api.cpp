@@ -60,13 +60,15 @@ template <class T> struct XformTables {
         std::vector<T> tb(2 * n2), dn, idn, tf, ti;
         for (int t = 0; t < n2; t++) { tb[2 * t] = (T)std::cos(kPi * t / B); tb[2 * t + 1] = (T)std::sin(kPi * t / B); }
         twB = upload(tb);
-        if (n2 <= 32) {  // row r = frequency r - (B-1), column t = grid angle pi t / B, zero padded to 32 x 32
-            std::vector<T> tm(2 * 32 * 32, T(0));
+        if (n2 <= 32) {  // [r][t] then [t][r]: frequency r - (B-1), grid angle pi t / B, zero padded to 32 x 32
+            std::vector<T> tm(2 * 2 * 32 * 32, T(0));
             for (int r = 0; r < M; r++)
                 for (int t = 0; t < n2; t++) {
                     const int e = (((r - (B - 1)) * t) % n2 + n2) % n2;
-                    tm[2 * (r * 32 + t)] = tb[2 * e];
-                    tm[2 * (r * 32 + t) + 1] = tb[2 * e + 1];
+                    for (int c = 0; c < 2; c++) {
+                        tm[2 * (r * 32 + t) + c] = tb[2 * e + c];
+                        tm[2 * (32 * 32 + t * 32 + r) + c] = tb[2 * e + c];
+                    }
                 }
             twMB = upload(tm);
         }

@@ -79,7 +79,8 @@ template <class T> struct XformDev {
     const T* Wf;        // forward beta weights  [mn][l][j] = w_j d^l_mn(beta_j)
     const T* Wi;        // inverse beta weights  [mn][j][l] = (2l+1) d^l_mn(beta_j)
     const cT<T>* twB;   // e^{i pi t / B}, t in [0, 2B)
-    const cT<T>* twMB;  // [32][32]: e^{i pi (r - (B-1)) t / B} for r < 2B-1, t < 2B, zero elsewhere (B <= 16)
+    const cT<T>* twMB;  // [32][32] e^{i pi (r - (B-1)) t / B} for r < 2B-1, t < 2B, zero elsewhere, then its
+                        // transpose [t][r] (B <= 16)
     const cT<T>* dftN;  // forward DFT matrix N x N (row-major [k][a]) e^{-2 pi i k a/N}
     const cT<T>* idftN; // inverse DFT matrix (1/N scaled) e^{+2 pi i k a/N}/N
     const cT<T>* twNf;  // e^{-2 pi i t/N}, t in [0, N)

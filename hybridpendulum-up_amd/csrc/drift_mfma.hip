@@ -304,6 +304,16 @@ template <> struct MixBatch<double> { static constexpr int n = 2; };
 template <class T>
 __device__ __forceinline__ bool cnz(cT<T> c) { return c.x != T(0) || c.y != T(0); }
 
+// Tables read at workgroup-uniform indices (mixing sources and coefficients,
+// per-row and per-k3 constants) are read through the constant address space:
+// nothing writes them during a launch, so they become scalar loads, which do
+// not wait behind the vector loads of the slab data.
+template <class T> using kc_ptr = __attribute__((address_space(4))) const T*;
+template <class T> __device__ __forceinline__ kc_ptr<T> as_const(const T* p) { return (kc_ptr<T>)p; }
+template <class T> __device__ __forceinline__ cT<T> kc_cplx(kc_ptr<T> p, int64_t i) {
+    return cmk<T>(p[2 * i], p[2 * i + 1]);
+}
+
 // Waves per SIMD the fused kernel is compiled for (the register budget):
 // 4 for FP32 (at most 128 VGPRs), 2 for FP64 and 3xBF16.
 template <class T, class G> struct FusedWaves { static constexpr int n = 2; };
@@ -391,7 +401,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         const int k1w = k1 + zo;
         const int64_t base = ((int64_t)r * nz + k3) * NN;
         const cT<T>* Fb = Fin + base;  // this slab
-        const T kc3 = DIM == 3 ? D.kc[k3] : T(0), kd3 = DIM == 3 ? D.kd[k3] : T(0);
+        const T kc3 = DIM == 3 ? as_const<T>(D.kc)[k3] : T(0), kd3 = DIM == 3 ? as_const<T>(D.kd)[k3] : T(0);
 
         // the slab, this lane's elements of F and of T3 F: issued together
         cT<T> own[4], fo[4], Zv[4];
@@ -429,12 +439,15 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             Wv[v] = czero<T>();
         }
         const int TS = MX.TS;
+        const kc_ptr<int32_t> srcT = as_const<int32_t>(MX.src) + (int64_t)r * TS;
+        const kc_ptr<T> coefT = as_const<T>((const T*)MX.coef) + 2 * (int64_t)r * TS * NC;
         for (int t0 = 0; t0 < TS; t0 += MB) {
             int src[MB];
             cT<T> fv[MB][4];
 #pragma unroll
+            for (int u = 0; u < MB; u++) src[u] = t0 + u < TS ? srcT[t0 + u] : -1;
+#pragma unroll
             for (int u = 0; u < MB; u++) {
-                src[u] = t0 + u < TS ? MX.src[(int64_t)r * TS + t0 + u] : -1;
                 const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : src[u]) * nz + k3) * NN;
 #pragma unroll
                 for (int v = 0; v < 4; v++) fv[u][v] = Fs[offw[v]];
@@ -442,8 +455,9 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
 #pragma unroll
             for (int u = 0; u < MB; u++) {
                 if (src[u] < 0) continue;
-                const cT<T>* cf = MX.coef + ((int64_t)r * TS + t0 + u) * NC;
-                const cT<T> g0 = cf[0], g1 = cf[1], g2 = DIM == 3 ? cf[2] : czero<T>();
+                const int64_t cb = (int64_t)(t0 + u) * NC;
+                const cT<T> g0 = kc_cplx<T>(coefT, cb), g1 = kc_cplx<T>(coefT, cb + 1),
+                            g2 = DIM == 3 ? kc_cplx<T>(coefT, cb + 2) : czero<T>();
                 const bool h0 = cnz<T>(g0) || (DIM == 3 && cnz<T>(g2)), h1 = cnz<T>(g1);
                 if (h0 || h1) {
                     // kc1 CG_1 + kc3 CG_3 is the same for the lane's four elements
@@ -455,7 +469,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
                         Wv[v] = cmk<T>(Wv[v].x + t.y, Wv[v].y - t.x);  // W -= i c f
                     }
                 }
-                const cT<T> cu1 = cf[DIM], cu2 = cf[DIM + 1];
+                const cT<T> cu1 = kc_cplx<T>(coefT, cb + DIM), cu2 = kc_cplx<T>(coefT, cb + DIM + 1);
                 if (cnz<T>(cu1))
 #pragma unroll
                     for (int v = 0; v < 4; v++) cfma<T>(V1[v], cu1, fv[u][v]);
@@ -493,7 +507,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         }
         // every term but T1 V1 + T2 V2, summed now so that P, Z and W retire:
         // S = W + b1 c1 P1 + b2 c2 P2 + (e3 + b3 c3) Z3
-        const cT<T> e3 = DIM == 3 ? D.e3[r] : czero<T>();
+        const cT<T> e3 = DIM == 3 ? kc_cplx<T>(as_const<T>((const T*)D.e3), r) : czero<T>();
         cT<T> S[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {

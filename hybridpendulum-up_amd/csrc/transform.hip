@@ -362,25 +362,16 @@ __global__ __launch_bounds__(256) void k_fwd_ag_r(XformDev<T> X, const T* __rest
 #pragma unroll
         for (int a = 0; a < kAgMax; a++) row[a] = (act && a < n2) ? sf[(xl * n2 + g) * rp + a] : T(0);
         __syncthreads();  // sH overwrites sf
-        for (int c = part; c * 8 < B; c += SA) {
-            const int m0 = c * 8;
-            cT<T> acc[8];
+        for (int m = part; m < B; m += SA) {
+            const cTab<T> e = E + 2 * (m + lmax) * kAgMax;
+            cT<T> acc = czero<T>();
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const cTab<T> e = E + 2 * (m0 + k + lmax) * kAgMax;
-                acc[k] = czero<T>();
-#pragma unroll
-                for (int a = 0; a < kAgMax; a++) {
-                    const cT<T> tw = etw<T>(e, a);
-                    acc[k].x = fma(row[a], tw.x, acc[k].x);
-                    acc[k].y = fma(row[a], tw.y, acc[k].y);
-                }
+            for (int a = 0; a < kAgMax; a++) {
+                const cT<T> tw = etw<T>(e, a);
+                acc.x = fma(row[a], tw.x, acc.x);
+                acc.y = fma(row[a], tw.y, acc.y);
             }
-            if (act) {
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (m0 + k < B) sH[((m0 + k) * n2 + g) * XC + xl] = acc[k];
-            }
+            if (act) sH[(m * n2 + g) * XC + xl] = acc;
         }
     }
     __syncthreads();
@@ -395,27 +386,15 @@ __global__ __launch_bounds__(256) void k_fwd_ag_r(XformDev<T> X, const T* __rest
 #pragma unroll
         for (int g = 0; g < kAgMax; g++) h[g] = (act && g < n2) ? sH[(m * n2 + g) * XC + xl] : czero<T>();
         const bool st = act && x < nx;
-        for (int c = part; c * 8 < M; c += SB) {
-            const int ni0 = c * 8;
-            cT<T> acc[8];
+        cT<T>* Gx = G + (int64_t)j * nx + min<int64_t>(x, nx - 1);
+        for (int ni = part; ni < M; ni += SB) {
+            const cTab<T> e = E + 2 * ni * kAgMax;
+            cT<T> acc = czero<T>();
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const cTab<T> e = E + 2 * (ni0 + k) * kAgMax;
-                acc[k] = czero<T>();
-#pragma unroll
-                for (int g = 0; g < kAgMax; g++) cfma<T>(acc[k], h[g], etw<T>(e, g));
-            }
+            for (int g = 0; g < kAgMax; g++) cfma<T>(acc, h[g], etw<T>(e, g));
             if (st) {
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int ni = ni0 + k;
-                    if (ni < M) {
-                        G[((int64_t)(m + lmax) + M * ni) * n2 * nx + (int64_t)j * nx + x] = acc[k];
-                        if (m > 0)
-                            G[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * nx + (int64_t)j * nx + x] =
-                                cconj<T>(acc[k]);
-                    }
-                }
+                Gx[((int64_t)(m + lmax) + M * ni) * n2 * nx] = acc;
+                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * nx] = cconj<T>(acc);
             }
         }
     }
@@ -427,7 +406,7 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
     extern __shared__ char smem_raw[];
     const int B = X.B, n2 = 2 * B, M = 2 * B - 1;
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl]
-    const cTab<T> E = (cTab<T>)(const T*)X.twMB;
+    const cTab<T> ET = (cTab<T>)(const T*)X.twMB + 2 * kAgMax * kAgMax;  // [t][r]
     const int w = xcd_major(blockIdx.x, gridDim.x);
     const int j = w / nxc;
     const int64_t x0 = (int64_t)(w % nxc) * XC;
@@ -441,30 +420,24 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
         const int mi = task / XC, xl = task % XC;
         const int64_t x = x0 + xl;
         const bool ld = act && x < nx;
+        // clamped, branch-free loads; invalid entries are zeroed after the load
+        const cT<T>* Gp = G + (int64_t)min(mi, M - 1) * n2 * nx + (int64_t)j * nx + min<int64_t>(x, nx - 1);
         cT<T> gr[kAgMax];
 #pragma unroll
-        for (int ni = 0; ni < kAgMax; ni++)
-            gr[ni] = (ld && ni < M) ? G[((int64_t)mi + (int64_t)M * ni) * n2 * nx + (int64_t)j * nx + x] : czero<T>();
-        for (int c = part; c * 8 < n2; c += S1) {
-            const int g0 = c * 8;
-            cT<T> acc[8];
+        for (int ni = 0; ni < kAgMax; ni++) {
+            const cT<T> v = Gp[(int64_t)M * min(ni, M - 1) * n2 * nx];
+            gr[ni] = (ld && ni < M) ? v : czero<T>();
+        }
+        for (int g = part; g < n2; g += S1) {
+            const cTab<T> e = ET + 2 * g * kAgMax;
+            cT<T> acc = czero<T>();
 #pragma unroll
-            for (int k = 0; k < 8; k++) acc[k] = czero<T>();
-#pragma unroll
-            for (int ni = 0; ni < kAgMax; ni++) {
-                const cTab<T> e = E + 2 * (ni * kAgMax + g0);
-#pragma unroll
-                for (int k = 0; k < 8; k++) cfma<T>(acc[k], gr[ni], cconj<T>(etw<T>(e, k)));
-            }
-            if (act) {
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (g0 + k < n2) sH[(mi * n2 + g0 + k) * XC + xl] = acc[k];
-            }
+            for (int ni = 0; ni < kAgMax; ni++) cfma<T>(acc, gr[ni], cconj<T>(etw<T>(e, ni)));
+            if (act) sH[(mi * n2 + g) * XC + xl] = acc;
         }
     }
     __syncthreads();
-    // f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a}; item (g, xl)
+    // f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a}; item (g, xl) holds its H column
     {
         const int T2 = n2 * XC, S2 = stage_parts(T2), P2 = 256 / S2;
         const int part = __builtin_amdgcn_readfirstlane(tid / P2), task = tid % P2;
@@ -472,26 +445,19 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
         const int g = task / XC, xl = task % XC;
         const int64_t x = x0 + xl;
         const bool st = act && x < nx;
-        for (int c = part; c * 8 < n2; c += S2) {
-            const int a0 = c * 8;
-            T acc[8];
+        cT<T> hr[kAgMax];
 #pragma unroll
-            for (int k = 0; k < 8; k++) acc[k] = T(0);
-            for (int mi = 0; mi < M; mi++) {
-                const cT<T> hv = act ? sH[(mi * n2 + g) * XC + xl] : czero<T>();
-                const cTab<T> e = E + 2 * (mi * kAgMax + a0);
+        for (int mi = 0; mi < kAgMax; mi++) hr[mi] = (act && mi < M) ? sH[(mi * n2 + g) * XC + xl] : czero<T>();
+        T* fo = f + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * min<int64_t>(x, nx - 1)));
+        for (int a = part; a < n2; a += S2) {
+            const cTab<T> e = ET + 2 * a * kAgMax;
+            T acc = T(0);
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const cT<T> tw = etw<T>(e, k);  // Re(h conj(tw))
-                    acc[k] = fma(hv.x, tw.x, fma(hv.y, tw.y, acc[k]));
-                }
+            for (int mi = 0; mi < kAgMax; mi++) {
+                const cT<T> tw = etw<T>(e, mi);  // Re(h conj(tw))
+                acc = fma(hr[mi].x, tw.x, fma(hr[mi].y, tw.y, acc));
             }
-            if (st) {
-                T* fo = f + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x));
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (a0 + k < n2) fo[a0 + k] = acc[k];
-            }
+            if (st) fo[a] = acc;
         }
     }
 }
