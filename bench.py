@@ -98,6 +98,41 @@ def timed_steps(plan, F, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
+def row_timings(plan, F, reps=3):
+    """The other rows of the path on this workload, timed with HIP events on
+    torch's current stream (the plan's stream): inverse transform (coefficients
+    -> grid), forward transform (grid -> coefficients) and get_stat (one device
+    pass over the grid plus the host fit).  Bytes are the algorithmic floor of
+    a transform: the real grid once plus the compact coefficients once."""
+    import torch
+    f = plan.backward(F)
+    F2 = plan.forward(f)
+    st = plan.get_stat(f)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tb = tf = 0.0
+    for _ in range(reps):
+        ev[0].record()
+        plan.backward(F, f)
+        ev[1].record()
+        plan.forward(f, F2)
+        ev[2].record()
+        torch.cuda.synchronize()
+        tb += ev[0].elapsed_time(ev[1]) / reps
+        tf += ev[1].elapsed_time(ev[2]) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = plan.get_stat(f)
+    ts = (time.perf_counter() - t0) * 1e3 / reps
+    rt = float(torch.linalg.vector_norm((F2 - F).to(torch.complex128)) / torch.linalg.vector_norm(F.to(torch.complex128)))
+    nbytes = f.numel() * f.element_size() + F.numel() * F.element_size()
+    del f, F2
+    row = lambda ms: {"ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9, "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "algorithmic_bytes": nbytes}
+    return {"transform_backward": row(tb), "transform_forward": row(tf), "roundtrip_rel_l2": rt,
+            "get_stat_ms": ts, "get_stat_fit_ok": bool(st["fit_ok"])}
+
+
 def cpu_baseline(plan64, F0, cfg, stages, budget_s=25.0):
     """The reference's dense derivative (oracle, FP64) on a bounded sample of
     output omega points of this workload, extrapolated linearly in the number
@@ -150,6 +185,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "fp64"])
     ap.add_argument("--method", default="RK4", choices=["euler", "midpoint", "RK2", "RK4"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-rows", action="store_true", help="skip the transform and get_stat timings")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="at the end, run a plain copy kernel (torch.neg, 4-byte elements) over this many GiB "
@@ -208,6 +244,15 @@ def main():
                  "hbm_achieved_GBs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
                  "ideal_ms": {"mfma": t_mfma * 1e3, "hbm": t_hbm * 1e3}})
 
+    rows = {}
+    if not args.no_rows:
+        for tag, pl, Fr in (("fp64", p64, F0), (args.precision, pm, Fm)):
+            if tag not in rows:
+                rows[tag] = row_timings(pl, Fr if tag != "fp64" else F0)
+                log("rows %s: backward %.2f ms, forward %.2f ms, get_stat %.2f ms"
+                    % (tag, rows[tag]["transform_backward"]["ms"], rows[tag]["transform_forward"]["ms"],
+                       rows[tag]["get_stat_ms"]))
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -240,6 +285,7 @@ def main():
         "fp64_mass_drift": mass_drift,
         "roofline": roof,
         "kernels_ms": {n: t for n, t, _ in prof},
+        "rows": rows,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -532,22 +532,41 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         gm.t1(pb_re, pb_im, Q1, D.c0[0]);
         gm.t2(pc_re, pc_im, Q2, D.c0[1]);
 
+        // epilogue: the RK stage streams F0 and ACC are loaded for all four
+        // elements at once (padded elements read their clamped in-range
+        // address), so their latency is paid once; stores are masked per element
+        cT<T> q[4], f0v[4], acv[4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            if (!ok[v]) continue;
-            const cT<T> q = cadd<T>(cadd<T>(Q1[v], Q2[v]), S[v]);
-            const unsigned o = offw[v];
-            if (sc.deriv_only) {
-                st_once<T>(Fout + base + o, q);
-                continue;
-            }
-            // RK stages read F0 and ACC here; an Euler step reuses fo.  These
+        for (int v = 0; v < 4; v++) q[v] = cadd<T>(cadd<T>(Q1[v], Q2[v]), S[v]);
+        if (sc.deriv_only) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (ok[v]) st_once<T>(Fout + base + offw[v], q[v]);
+        } else {
+            // RK stages read F0 and ACC here; a stage without F0 takes the
+            // slab itself, and one without ACC reads F in its place and
+            // discards it, so the eight loads are unconditional.  These
             // streams and the outputs are touched once: non-temporal.
-            const cT<T> f0v = sc.read_f0 ? ld_once<T>(F0 + base + o) : fo[v];
-            const cT<T> acv = sc.read_acc ? ld_once<T>(ACC + base + o) : czero<T>();
-            const cT<T> upd = cadd<T>(cscale<T>(acv, sc.ca), cscale<T>(q, sc.cd));
-            st_once<T>(Fout + base + o, cadd<T>(f0v, cscale<T>(upd, sc.dt)));
-            if (sc.write_acc) st_once<T>(ACC + base + o, cadd<T>(cscale<T>(acv, sc.ea), cscale<T>(q, sc.ed)));
+            const cT<T>* f0p = (sc.read_f0 ? F0 : Fin) + base;
+            const cT<T>* acp = (sc.read_acc ? ACC : Fin) + base;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                f0v[v] = ld_once<T>(f0p + offw[v]);
+                acv[v] = ld_once<T>(acp + offw[v]);
+            }
+            if (!sc.read_acc)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acv[v] = czero<T>();
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const cT<T> upd = cadd<T>(cscale<T>(acv[v], sc.ca), cscale<T>(q[v], sc.cd));
+                const cT<T> fn = cadd<T>(f0v[v], cscale<T>(upd, sc.dt));
+                const cT<T> an = cadd<T>(cscale<T>(acv[v], sc.ea), cscale<T>(q[v], sc.ed));
+                if (ok[v]) {
+                    st_once<T>(Fout + base + offw[v], fn);
+                    if (sc.write_acc) st_once<T>(ACC + base + offw[v], an);
+                }
+            }
         }
         sl += per;
         r += dr;
