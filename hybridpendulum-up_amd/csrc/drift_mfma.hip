@@ -342,24 +342,19 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
     G gm;
     gm.init(&tabs[0], &tabs[1], k2t, k1t, lr, lq);
 
-    // this lane's four elements (k2v[v], k1) of every slab; out-of-range
-    // elements (k >= N in the 32 x 32 padding) load a clamped in-range address
-    // and select zero, so no load is predicated
+    // this lane's four elements (k2, k1) of every slab: k1 = 16 k1t + lr and
+    // k2 = 16 k2t + orow(lq, v); their omega factors stay in registers, their
+    // indices are recomputed per slab
     const int k1 = 16 * k1t + lr;
     const bool k1ok = k1 < N;
-    const int k1c = k1ok ? k1 : N - 1;
     const T kc1 = k1ok ? D.kc[k1] : T(0), kd1 = k1ok ? D.kd[k1] : T(0);
-    int k2v[4];
-    unsigned off[4];  // element offset inside a slab (uniform slab base + 32-bit offset)
-    bool ok[4];
     T kc2v[4], kd2v[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        k2v[v] = 16 * k2t + G::orow(lq, v);
-        ok[v] = k1ok && k2v[v] < N;
-        off[v] = (unsigned)((k2v[v] < N ? k2v[v] : N - 1) * N + k1c);
-        kc2v[v] = ok[v] ? D.kc[k2v[v]] : T(0);
-        kd2v[v] = ok[v] ? D.kd[k2v[v]] : T(0);
+        const int k2 = 16 * k2t + G::orow(lq, v);
+        const bool ok = k1ok && k2 < N;
+        kc2v[v] = ok ? D.kc[k2] : T(0);
+        kd2v[v] = ok ? D.kd[k2] : T(0);
     }
     // D(k) = sum_i G_ii kd(k_i) - sum_{i != j} G_ij kc(k_i) kc(k_j)
     const T g00 = D.G[0], g11 = D.G[DIM + 1], g01 = D.G[1] + D.G[DIM];
@@ -391,14 +386,19 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         int zo;
         asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
         const int tidz = tid + zo;
+        // the lane's elements: out-of-range ones (ok false, in the 32 x 32
+        // padding) use a clamped in-range offset and are discarded
+        const int lanez = tidz & 63, wz = tidz >> 6;
+        const int k1w = 16 * (wz & 1) + (lanez & 15), lqz = lanez >> 4;
         int k2w[4];
         unsigned offw[4];
+        bool ok[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            k2w[v] = k2v[v] + zo;
-            offw[v] = off[v] + (unsigned)zo;
+            k2w[v] = 16 * (wz >> 1) + G::orow(lqz, v);
+            ok[v] = k1w < N && k2w[v] < N;
+            offw[v] = (unsigned)((k2w[v] < N ? k2w[v] : N - 1) * N + (k1w < N ? k1w : N - 1));
         }
-        const int k1w = k1 + zo;
         const int64_t base = ((int64_t)r * nz + k3) * NN;
         const cT<T>* Fb = Fin + base;  // this slab
         const T kc3 = DIM == 3 ? as_const<T>(D.kc)[k3] : T(0), kd3 = DIM == 3 ? as_const<T>(D.kd)[k3] : T(0);
@@ -414,9 +414,6 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            // padded elements (ok false) read a clamped in-range element: they
-            // only feed outputs that are discarded, and the V planes are zeroed
-            // there before the second GEMM pair
             fo[v] = Fb[offw[v]];
             Zv[v] = DIM == 3 ? ld_once<T>(Z3 + base + offw[v]) : czero<T>();
         }
@@ -543,20 +540,25 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             for (int v = 0; v < 4; v++)
                 if (ok[v]) st_once<T>(Fout + base + offw[v], q[v]);
         } else {
-            // RK stages read F0 and ACC here; a stage without F0 takes the
-            // slab itself, and one without ACC reads F in its place and
-            // discards it, so the eight loads are unconditional.  These
-            // streams and the outputs are touched once: non-temporal.
+            // RK stages read F0 and ACC here (a stage without F0 takes the
+            // slab itself): the loads of the four elements are issued
+            // together, under workgroup-uniform conditions only, and every
+            // loaded value is consumed.  These streams and the outputs are
+            // touched once: non-temporal.
             const cT<T>* f0p = (sc.read_f0 ? F0 : Fin) + base;
-            const cT<T>* acp = (sc.read_acc ? ACC : Fin) + base;
+            if (sc.read_acc) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                f0v[v] = ld_once<T>(f0p + offw[v]);
-                acv[v] = ld_once<T>(acp + offw[v]);
+                for (int v = 0; v < 4; v++) {
+                    f0v[v] = ld_once<T>(f0p + offw[v]);
+                    acv[v] = ld_once<T>(ACC + base + offw[v]);
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    f0v[v] = ld_once<T>(f0p + offw[v]);
+                    acv[v] = czero<T>();
+                }
             }
-            if (!sc.read_acc)
-#pragma unroll
-                for (int v = 0; v < 4; v++) acv[v] = czero<T>();
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const cT<T> upd = cadd<T>(cscale<T>(acv[v], sc.ca), cscale<T>(q[v], sc.cd));
