@@ -42,7 +42,7 @@ template <class T> std::vector<T> cast_vec(const std::vector<double>& v) { retur
 
 // Transform tables from a Wigner-d table d (M,M,B,2B) and weights w (2B).
 template <class T> struct XformTables {
-    DevBuf Wf, Wi, twB, twMB, dftN, idftN, twNf, twNi;
+    DevBuf Wf, Wi, twB, twMB, dftN, idftN, twNf, twNi, bsF, bsI;
     XformDev<T> view{};
     void build(int B, int dim, int N, const double* d, const double* w) {
         const int M = 2 * B - 1, n2 = 2 * B;
@@ -87,12 +87,29 @@ template <class T> struct XformTables {
             ti[2 * k] = (T)std::cos(ang); ti[2 * k + 1] = (T)std::sin(ang);
         }
         dftN = upload(dn); idftN = upload(idn); twNf = upload(tf); twNi = upload(ti);
+        if (Nn <= 32) {  // chirp-Toeplitz tables: c_d = e^{+i pi d^2/N} (d in [-32, 32)), p_k = e^{-i pi k^2/N}
+            std::vector<T> bf(2 * 96, T(0)), bi(2 * 96, T(0));
+            auto ang = [&](int64_t q) { return kPi * (double)(((q % (2 * Nn)) + 2 * Nn) % (2 * Nn)) / Nn; };
+            for (int d = -32; d < 32; d++) {
+                const double a = ang((int64_t)d * d);
+                bf[2 * (d + 32)] = (T)std::cos(a); bf[2 * (d + 32) + 1] = (T)std::sin(a);
+                bi[2 * (d + 32)] = (T)std::cos(a); bi[2 * (d + 32) + 1] = (T)-std::sin(a);
+            }
+            for (int k = 0; k < Nn; k++) {
+                const double a = ang((int64_t)k * k);
+                bf[2 * (64 + k)] = (T)std::cos(a); bf[2 * (64 + k) + 1] = (T)-std::sin(a);
+                bi[2 * (64 + k)] = (T)std::cos(a); bi[2 * (64 + k) + 1] = (T)std::sin(a);
+            }
+            bsF = upload(bf); bsI = upload(bi);
+        }
         view.B = B; view.dim = dim; view.N = Nn;
         view.nx = ipow64(Nn, dim); view.nrc = nrc_of(B);
         view.Wf = Wf.as<T>(); view.Wi = Wi.as<T>(); view.twB = twB.as<cT<T>>();
         view.twMB = n2 <= 32 ? twMB.as<cT<T>>() : nullptr;
         view.dftN = dftN.as<cT<T>>(); view.idftN = idftN.as<cT<T>>();
         view.twNf = twNf.as<cT<T>>(); view.twNi = twNi.as<cT<T>>();
+        view.bsF = Nn <= 32 ? bsF.as<cT<T>>() : nullptr;
+        view.bsI = Nn <= 32 ? bsI.as<cT<T>>() : nullptr;
     }
 };
 

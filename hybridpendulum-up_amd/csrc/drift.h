@@ -61,6 +61,15 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
 template <class T>
 void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf16x3, hipStream_t st);
 
+// Omega DFTs as chirp-Toeplitz MFMA products (N <= 32).  bs: 96 entries, the
+// Toeplitz kernel c_d at [d + 32] for d in [-32, 32) and then the chirp p_k
+// (zero for k >= N); see k_slab_dft_mfma.
+template <class T>
+void launch_slab_dft_mfma(void* F, int64_t nslab, int N, const void* bs, T scale, hipStream_t st);
+template <class T>
+void launch_axis_dft_mfma(const void* in, void* out, int N, int64_t inner, int64_t nlines, const void* bs, T scale,
+                          hipStream_t st);
+
 template <class T>
 void launch_rotmix(const DriftDev<T>& D, const void* F, void* W, void* V1, void* V2, hipStream_t st);
 template <class T>
@@ -85,6 +94,8 @@ template <class T> struct XformDev {
     const cT<T>* idftN; // inverse DFT matrix (1/N scaled) e^{+2 pi i k a/N}/N
     const cT<T>* twNf;  // e^{-2 pi i t/N}, t in [0, N)
     const cT<T>* twNi;  // e^{+2 pi i t/N}
+    const cT<T>* bsF;   // chirp-Toeplitz tables of the forward / inverse omega DFT (N <= 32),
+    const cT<T>* bsI;   // 96 entries each (launch_slab_dft_mfma)
 };
 
 template <class T>

@@ -795,6 +795,157 @@ void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf1
     hipLaunchKernelGGL(k_axis3_mfma<T>, grid, dim3(256), 0, st, D, (const cT<T>*)in, (cT<T>*)out);
 }
 
+// ------------------------------------------------- omega DFTs (chirp) --
+//
+// The length-N DFT as a chirp-Toeplitz product (Bluestein's identity
+// -2 k a = (k - a)^2 - k^2 - a^2):
+//   X_k = p_k sum_a c_{k-a} (p_a x_a),  p_k = e^{-+ i pi k^2 / N}, c_d = e^{+- i pi d^2 / N}.
+// The Toeplitz c_{k-a} is exactly what the circulant tables hold when they are
+// filled with c_d for d = -32 .. 31 directly, so the transform runs on the
+// same MFMA GEMMs as the drift circulants, with the chirps applied on the way
+// into and out of LDS.  bs holds c_d at [d + 32] (64 entries) and then p_k
+// (zero for k >= N), 96 entries; the inverse tables are the conjugates.
+
+template <class T>
+__device__ __forceinline__ void fill_toeplitz(CircTab<T>& t, const cT<T>* bs, int tid) {
+    if (tid < 64) {
+        const cT<T> c = bs[tid];
+        t.re_in[tid] = c;
+        t.im_in[tid] = cmk<T>(-c.y, c.x);
+        t.rim[tid] = cmk<T>(-c.y, c.y);
+    }
+}
+
+// 2-D DFT over (k1, k2) of every N x N slab, in place, times `scale`.
+template <class T>
+__global__ __launch_bounds__(256) void k_slab_dft_mfma(cT<T>* F, const cT<T>* __restrict__ bs, int N, T scale,
+                                                       int nslab) {
+    __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad];
+    __shared__ CircTab<T> tab;
+    __shared__ cT<T> pk[32];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int k2t = w >> 1, k1t = w & 1, lr = lane & 15, lq = lane >> 4;
+    fill_toeplitz<T>(tab, bs, tid);
+    if (tid < 32) pk[tid] = bs[64 + tid];
+    __syncthreads();
+    GemmF<T, false> gm;
+    gm.init(&tab, &tab, k2t, k1t, lr, lq);
+    const int NN = N * N;
+    const int k1 = 16 * k1t + lr;
+    const cT<T> p1 = pk[k1];
+    for (int sl = blockIdx.x; sl < nslab; sl += gridDim.x) {
+        cT<T>* Fb = F + (int64_t)sl * NN;
+        cT<T> x[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {  // clamped in-range loads, zero outside
+            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+            const bool in = a2 < N && a1 < N;
+            const cT<T> v = Fb[in ? a2 * N + a1 : 0];
+            x[i] = in ? cmul<T>(v, pk[a1]) : czero<T>();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+            pa_re[a2 * kPad + a1] = x[i].x;
+            pa_im[a2 * kPad + a1] = x[i].y;
+        }
+        __syncthreads();  // plane A complete; plane B of the previous slab consumed
+        cT<T> R[4];
+        gm.t1(pa_re, pa_im, R);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {  // along k1 done; pre-chirp along k2 (p = 0 in the padding)
+            const int k2 = 16 * k2t + GemmF<T, false>::orow(lq, v);
+            const cT<T> y = cmul<T>(cmul<T>(R[v], p1), pk[k2]);
+            pb_re[k2 * kPad + k1] = y.x;
+            pb_im[k2 * kPad + k1] = y.y;
+        }
+        __syncthreads();  // plane B complete; plane A consumed
+        gm.t2(pb_re, pb_im, R);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int k2 = 16 * k2t + GemmF<T, false>::orow(lq, v);
+            if (k1 < N && k2 < N) Fb[k2 * N + k1] = cscale<T>(cmul<T>(R[v], pk[k2]), scale);
+        }
+    }
+}
+
+// DFT along the outermost axis: lines of N points at stride `inner`, one per
+// (line block L, column j); 64-column chunks as the B operand of the Toeplitz
+// GEMM (the k_axis3_mfma layout), times `scale`.  Out of place or in place.
+template <class T>
+__global__ __launch_bounds__(256) void k_axis_dft_mfma(const cT<T>* in, cT<T>* out, const cT<T>* __restrict__ bs,
+                                                       int N, int64_t inner, T scale) {
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc;
+    constexpr int CP = 65;
+    __shared__ T sre[32 * CP], sim[32 * CP];
+    __shared__ CircTab<T> tab;
+    __shared__ cT<T> pk[32];
+    const int64_t L = blockIdx.y;
+    const int64_t j0 = (int64_t)blockIdx.x * 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
+    fill_toeplitz<T>(tab, bs, tid);
+    if (tid < 32) pk[tid] = bs[64 + tid];
+    __syncthreads();
+    for (int idx = tid; idx < 32 * 64; idx += 256) {
+        const int a = idx >> 6, jj = idx & 63;
+        const int64_t j = j0 + jj;
+        const cT<T> v = (a < N && j < inner) ? cmul<T>(in[(L * N + a) * inner + j], pk[a]) : czero<T>();
+        sre[a * CP + jj] = v.x;
+        sim[a * CP + jj] = v.y;
+    }
+    __syncthreads();
+    acc_t acc[4];
+#pragma unroll
+    for (int it = 0; it < 4; it++) acc[it] = MF::zero();
+#pragma unroll 4
+    for (int s = 0; s < 16; s++) {
+        const int k = 4 * s + lq;
+        const T bv = (s < 8 ? sre : sim)[(k & 31) * CP + 16 * w + lr];
+        const cT<T>* t = s < 8 ? tab.re_in : tab.im_in;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const cT<T> f = t[16 * h + lr + 32 - (k & 31)];
+            acc[h] = MF::fma(f.x, bv, acc[h]);
+            acc[h + 2] = MF::fma(f.y, bv, acc[h + 2]);
+        }
+    }
+    const int64_t j = j0 + 16 * w + lr;
+    if (j >= inner) return;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int k = 16 * h + MF::orow(lq, v);
+            if (k < N)
+                out[(L * N + k) * inner + j] = cscale<T>(cmul<T>(cmk<T>(acc[h][v], acc[h + 2][v]), pk[k]), scale);
+        }
+}
+
+template <class T>
+void launch_slab_dft_mfma(void* F, int64_t nslab, int N, const void* bs, T scale, hipStream_t st) {
+    HPUP_REQUIRE(N <= 32, HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT needs N <= 32");
+    HPUP_REQUIRE(nslab < ((int64_t)1 << 31), HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT: too many slabs");
+    const unsigned grid = (unsigned)std::min<int64_t>(nslab, 4096);
+    hipLaunchKernelGGL(k_slab_dft_mfma<T>, dim3(grid), dim3(256), 0, st, (cT<T>*)F, (const cT<T>*)bs, N, scale,
+                       (int)nslab);
+}
+
+template <class T>
+void launch_axis_dft_mfma(const void* in, void* out, int N, int64_t inner, int64_t nlines, const void* bs, T scale,
+                          hipStream_t st) {
+    HPUP_REQUIRE(N <= 32, HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT needs N <= 32");
+    dim3 grid((unsigned)((inner + 63) / 64), (unsigned)nlines);
+    hipLaunchKernelGGL(k_axis_dft_mfma<T>, grid, dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out,
+                       (const cT<T>*)bs, N, inner, scale);
+}
+
+template void launch_slab_dft_mfma<float>(void*, int64_t, int, const void*, float, hipStream_t);
+template void launch_slab_dft_mfma<double>(void*, int64_t, int, const void*, double, hipStream_t);
+template void launch_axis_dft_mfma<float>(const void*, void*, int, int64_t, int64_t, const void*, float, hipStream_t);
+template void launch_axis_dft_mfma<double>(const void*, void*, int, int64_t, int64_t, const void*, double,
+                                           hipStream_t);
+
 template void launch_drift_fused<float>(const DriftDev<float>&, const MixDev<float>&, const void*, const void*,
                                         const void*, void*, void*, const StageCoef<float>&, bool, hipStream_t);
 template void launch_drift_fused<double>(const DriftDev<double>&, const MixDev<double>&, const void*, const void*,

@@ -543,6 +543,13 @@ void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hi
     if (X.dim == 0) return;
     const int N = X.N, NN = N * N;
     HPUP_REQUIRE(NN <= 4096, HPUP_ERR_UNSUPPORTED, "omega grid larger than 64 per axis");
+    if (N <= 32 && X.bsF) {  // chirp-Toeplitz products on MFMA
+        const cT<T>* bs = inverse ? X.bsI : X.bsF;
+        const int64_t nslab = X.nrc * (X.dim == 3 ? N : 1);
+        launch_slab_dft_mfma<T>(F, nslab, N, bs, inverse ? T(1) / T(NN) : T(1), st);
+        if (X.dim == 3) launch_axis_dft_mfma<T>(F, F, N, (int64_t)NN, X.nrc, bs, inverse ? T(1) / T(N) : T(1), st);
+        return;
+    }
     const cT<T>* Mat = inverse ? X.idftN : X.dftN;
     // twiddles: row 1 of the matrix (unscaled)
     const T sc2 = inverse ? T(1) / T(NN) : T(1);
