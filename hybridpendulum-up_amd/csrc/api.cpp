@@ -10,6 +10,7 @@
 #include <strings.h>
 
 #include "drift.h"
+#include "getc.h"
 #include "hpup_internal.h"
 #include "matfisher.h"
 #include "stats.h"
@@ -730,6 +731,62 @@ int hpup_get_stat(const double* f, const int64_t* dims, int ndims, const double*
         auto Rof = [R](int64_t iR, double* r) { std::memcpy(r, R + 9 * iR, 9 * sizeof(double)); };
         mf::finish_stat(dim, nR, nx, (int)n2, w, hR.data(), hG.data(), hx.data(), xh.data(), dv, Rof, true, *out);
         if (fx) std::memcpy(fx, hx.data(), sizeof(double) * nx);
+    });
+}
+
+int hpup_pendulum_plot_getc(const double* f, const int64_t* dims, int ndims, const double* e, const int64_t* edims,
+                            int nedims, double L, double* c) {
+    return guard([&] {
+        HPUP_REQUIRE(f && dims && e && edims && c, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(ndims == 5 || ndims == 6, HPUP_ERR_ARG, "f must have rank 5 or 6");
+        HPUP_REQUIRE(dims[0] % 2 == 0 && dims[0] > 0 && dims[0] == dims[1] && dims[1] == dims[2], HPUP_ERR_ARG,
+                     "f must be (2B,2B,2B,N,...)");
+        HPUP_REQUIRE(nedims == 4 && edims[0] == 3 && edims[3] == 3 && edims[1] >= 0 && edims[2] >= 0, HPUP_ERR_ARG,
+                     "e must be (3, ns, na, 3)");
+        const int B = (int)(dims[0] / 2), dim = ndims - 3, N = (int)dims[3];
+        HPUP_REQUIRE(B <= 64, HPUP_ERR_UNSUPPORTED, "bandlimit above 64");
+        for (int i = 4; i < ndims; i++) HPUP_REQUIRE(dims[i] == N, HPUP_ERR_ARG, "omega axes must be equal");
+        const int64_t nx = ipow64(N, dim), n2 = 2 * B, nR = n2 * n2 * n2, ns = edims[1], na = edims[2];
+        const int64_t P = ns * na * 3;
+        // rotation marginal, gridsize L / N per omega axis (getc.cpp:42-61)
+        DevBuf fd(sizeof(double) * nR * nx), fR(sizeof(double) * nR);
+        HPUP_HIP(hipMemcpy(fd.p, f, fd.bytes, hipMemcpyHostToDevice));
+        launch_rot_marginal(fd.as<double>(), nR, nx, std::pow(L / N, dim), fR.as<double>(), nullptr);
+        HPUP_HIP(hipGetLastError());
+        // its SO(3) transform (nx = 1), tables of fftSO3R.m
+        hpup_model m{};
+        m.B = B;
+        m.dim = 0;
+        m.N = 1;
+        m.precision = HPUP_FP64;
+        const int M = 2 * B - 1, lmax = B - 1;
+        std::vector<double> beta, w, d((size_t)M * M * B * n2);
+        host_grid(B, beta, w);
+        for (int j = 0; j < n2; j++) host_wigner_d(beta[j], lmax, d.data() + (size_t)j * M * M * B);
+        std::unique_ptr<PlanBase> p(make_plan(m, d.data(), w.data(), nullptr));
+        DevBuf Fc(sizeof(double) * 2 * p->ncoef);
+        p->forward(fR.p, Fc.p, nullptr);
+        // wedge seeds sqrt((2l)! / ((l+m)! (l-m)!)) (Wigner_d.m / getc.cpp:215-217)
+        std::vector<double> sq((size_t)B * M, 0.0), fact(2 * B + 1, 1.0);
+        for (int k = 1; k <= 2 * B; k++) fact[k] = fact[k - 1] * k;
+        for (int l = 0; l < B; l++)
+            for (int mm = -l; mm <= l; mm++) sq[(size_t)l * M + mm + l] = std::sqrt(fact[2 * l] / fact[l + mm] / fact[l - mm]);
+        DevBuf sqd = upload(sq);
+        std::vector<double> ph(P > 0 ? P : 1);
+        if (P > 0) {
+            DevBuf ed(sizeof(double) * 3 * P), od(sizeof(double) * P);
+            HPUP_HIP(hipMemcpy(ed.p, e, ed.bytes, hipMemcpyHostToDevice));
+            launch_point_eval(Fc.p, B, sqd.as<double>(), ed.as<double>(), P, od.as<double>(), nullptr);
+            HPUP_HIP(hipGetLastError());
+            HPUP_HIP(hipMemcpy(ph.data(), od.p, od.bytes, hipMemcpyDeviceToHost));
+        }
+        // c(s, k) = sum over the na angles (getc.cpp:24-35)
+        for (int64_t k = 0; k < 3; k++)
+            for (int64_t s = 0; s < ns; s++) {
+                double acc = 0.0;
+                for (int64_t a = 0; a < na; a++) acc += ph[s + ns * (a + na * k)];
+                c[s + ns * k] = acc;
+            }
     });
 }
 

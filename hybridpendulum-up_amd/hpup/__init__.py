@@ -74,6 +74,7 @@ EXPORTS = [
     "hpup_plan_full_to_compact", "hpup_plan_compact_to_full", "hpup_plan_stat_moments",
     "hpup_plan_profile", "hpup_plan_profile_name", "hpup_plan_profile_bytes",
     "hpup_get_stat", "hpup_psvd", "hpup_pdf_MF_normal", "hpup_pdf_MF_M2S", "hpup_plan_get_stat",
+    "hpup_pendulum_plot_getc",
 ]
 
 
@@ -335,6 +336,19 @@ def get_stat(f, R, x, w, return_fx=False):
     if return_fx:
         r["fx"] = fx
     return r
+
+
+def pendulum_plot_getc(f, e, L):
+    """c = pendulum_plot_getc(f, e, L) (pendulum_plot_getc/src/mexFunction.cpp:5-33): the
+    rotation marginal of the grid density f, SO(3)-transformed and evaluated at the Euler
+    triples e (3, ns, na, 3); c (ns, 3) sums the na evaluations."""
+    f = np.asfortranarray(f, dtype=np.float64)
+    e = np.asfortranarray(e, dtype=np.float64)
+    if e.ndim != 4:
+        raise HpupError("e must be (3, ns, na, 3)")
+    c = np.zeros((e.shape[1], 3), order="F")
+    _check(lib().hpup_pendulum_plot_getc(_p(f), _dims(f), f.ndim, _p(e), _dims(e), e.ndim, ctypes.c_double(L), _p(c)))
+    return c
 
 
 def psvd(F):

@@ -189,6 +189,17 @@ typedef struct {
     int niter, fit_ok;
 } hpup_stat;
 
+/* c = pendulum_plot_getc(f, e, L)
+ * Replaces pendulum_plot_getc (propagation/mex/sources/pendulum_plot_getc/src/mexFunction.cpp:5-33,
+ * getc.cpp:7-40 get_color, 42-61 get_margin, 63-143 fftSO3, 145-175 ifftSO3_single), called by
+ * pendulum_reduced_den.m:230.  f real (2B,2B,2B,N,N[,N]) with dims/ndims (5 or 6); e real Euler
+ * angles (alpha, beta, gamma) with edims = (3, ns, na, 3), nedims = 4; L the omega box length.
+ * The rotation marginal fR = (L/N)^(ndims-3) sum_x f is transformed (SO(3), FP64) and its inverse
+ * transform is evaluated at every Euler triple; c (ns, 3) receives
+ * c[s + ns k] = sum_a fR(e(:, s, a, k)). */
+int hpup_pendulum_plot_getc(const double* f, const int64_t* dims, int ndims, const double* e, const int64_t* edims,
+                            int nedims, double L, double* c);
+
 /* [ER, Ex, Varx, EvR, ExvR, EvRvR, U, S, V, P, Miu, Sigma(, fx)] = get_stat(f, R, x, w)
  * Replaces the drivers' local function get_stat (propagation/pendulum_reduced.m:485-532,
  * pendulum.m:547-594, pendulum_reduced_den.m:553-600 with the fx output).

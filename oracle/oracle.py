@@ -282,3 +282,37 @@ def grid_mR(B):
     """mR = [R32; -R31] on the rotation grid, (2, 2B, 2B, 2B) (pendulum_reduced_hybrid.m:115)."""
     R = rotations(B)
     return np.asfortranarray(np.stack([R[2, 1], -R[2, 0]], axis=0))
+
+
+def getc(f, e, L):
+    """Plot colours of the attitude marginal, restating pendulum_plot_getc
+    (propagation/mex/sources/pendulum_plot_getc/src/getc.cpp:7-40 get_color, 42-61
+    get_margin, 145-175 ifftSO3_single with wigner_D 256-296): the rotation
+    marginal (L/N)^(ndims-3) sum_x f, its SO(3) transform (fftSO3R.m with nx = 1,
+    the refcpu forward), and sum_l (2l+1) sum_{m,n} Re(F^l_mn d^l_mn(beta)
+    e^{-i m alpha} e^{-i n gamma}) at every Euler triple of e (3, ns, na, 3);
+    c[s, k] = sum_a of the values.  Pure Python over the points: small cases."""
+    f = np.asarray(f, dtype=np.float64)
+    B = f.shape[0] // 2
+    N = f.shape[3]
+    dim = f.ndim - 3
+    nR = (2 * B) ** 3
+    fR = f.reshape(nR, -1, order="F").sum(axis=1) * (L / N) ** dim
+    d = wigner_table(B)
+    w = grid(B)[3]
+    F = fft_forward(B, fR.reshape((2 * B,) * 3, order="F"), d, w, 0, 1)
+    lmax = B - 1
+    ms = np.arange(-lmax, lmax + 1)
+    ns, na = e.shape[1], e.shape[2]
+    c = np.zeros((ns, 3))
+    for k in range(3):
+        for s in range(ns):
+            for a in range(na):
+                al, be, ga = e[:, s, a, k]
+                dd = wigner_d(be, lmax)
+                ph = np.exp(-1j * ms[:, None] * al) * np.exp(-1j * ms[None, :] * ga)
+                val = 0.0
+                for l in range(B):
+                    val += (2 * l + 1) * np.real(np.sum(F[:, :, l] * dd[:, :, l] * ph))
+                c[s, k] += val
+    return c
