@@ -936,6 +936,89 @@ static void propagate_gateway(int dim, const T* F, const int64_t* dims, const T*
     HPUP_HIP(hipMemcpy(Fnew, Ff.p, Ff.bytes, hipMemcpyDeviceToHost));
 }
 
+// gyrobias_propagate (gyrobias_propagate/kernel.cu:8-120, integrate.cu:241-443; gyro_bias.m:235-298):
+// the attitude kinematics with the gyro bias as the R^3 variable (the pendulum's -(F (*) X_p / nx) u_p^T
+// term), the gyro random walk sum_ij G1_ij F u_i^T u_j^T and the bias random walk sum_ij G2_ij c_ij(k) F
+// on the bias box L = 2; one Euler step.  The first and third terms are the pendulum drift with no
+// damping, gyroscopic or gravity term; the second is a banded mixing of the n index of each l block,
+// F M_l^T with M_l = sum_ij G1_ij u_j u_i, added from the step's input.
+static void gyrobias_gateway(const double* F, const int64_t* dims, const double* X, double dt, const double* u,
+                             const double* G1, const double* G2, double* Fnew) {
+    HPUP_REQUIRE(F && dims && X && u && G1 && G2 && Fnew, HPUP_ERR_ARG, "null argument");
+    const int B = (int)dims[2], M = 2 * B - 1, N = (int)dims[3], lmax = B - 1;
+    HPUP_REQUIRE(dims[0] == M && dims[1] == M && dims[4] == N && dims[5] == N, HPUP_ERR_ARG,
+                 "F must be (2B-1,2B-1,B,N,N,N)");
+    const int64_t nx = ipow64(N, 3), nrot = (int64_t)M * M * B;
+    std::vector<double> zO(2 * nx * 3, 0.0), zMR(2 * nrot * 3, 0.0);
+    const double bt[3] = {0, 0, 0};
+    std::vector<std::vector<double>> cells((size_t)B * B);
+    std::vector<const double*> cptr((size_t)B * B, nullptr);
+    for (int l1 = 0; l1 < B && B >= 2; l1++) {
+        host_clebsch_gordan(l1, 1, cells[l1 + (size_t)B]);
+        cptr[l1 + (size_t)B] = cells[l1 + (size_t)B].data();
+    }
+    const double L = 2.0;  // gyro_bias.m:64 (the bias box; c_ij of integrate.cu uses pi = 2 pi / L)
+    DriftSpec spec = build_drift_spec_from_arrays(B, 3, N, L, X, zO.data(), zMR.data(), bt, G2, u, cptr.data(), 1e-9);
+    hpup_model m{};
+    m.B = B; m.dim = 3; m.N = N; m.L = L; m.dt = dt; m.method = 0;  // Euler
+    m.precision = HPUP_FP64;
+    for (int i = 0; i < 9; i++) m.G[i] = G2[i];
+    std::vector<double> beta, w;
+    host_grid(B, beta, w);
+    std::vector<double> d((size_t)M * M * B * 2 * B);
+    for (int j = 0; j < 2 * B; j++) host_wigner_d(beta[j], lmax, d.data() + (size_t)j * M * M * B);
+    std::unique_ptr<PlanBase> p(make_plan(m, d.data(), w.data(), &spec));
+    // M_l = sum_ij G1_ij u_j u_i on each l block, as a banded row table (compact m-major rows)
+    auto U = [&](int i, int l, int a, int b) {  // u_i[a][b] of block l (a, b in [-l, l])
+        const size_t k = (size_t)(a + lmax) + M * ((size_t)(b + lmax) + M * ((size_t)l + (size_t)B * i));
+        return std::complex<double>(u[2 * k], u[2 * k + 1]);
+    };
+    const int64_t nrc = p->nrc;
+    const int TS = 5;  // u_1, u_2 bidiagonal, u_3 diagonal: bandwidth 2
+    std::vector<int32_t> src(nrc * TS, -1);
+    std::vector<double> coef(2 * nrc * TS, 0.0);
+    for (int l = 0; l < B; l++) {
+        const int S = 2 * l + 1;
+        std::vector<std::complex<double>> Ml((size_t)S * S, 0.0);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double g = G1[i + 3 * j];
+                if (g == 0.0) continue;
+                for (int a = -l; a <= l; a++)
+                    for (int c = -l; c <= l; c++) {
+                        std::complex<double> s = 0.0;
+                        for (int k = -l; k <= l; k++) s += U(j, l, a, k) * U(i, l, k, c);
+                        Ml[(a + l) + (size_t)S * (c + l)] += g * s;
+                    }
+            }
+        for (int mm = -l; mm <= l; mm++)
+            for (int n = -l; n <= l; n++) {
+                const int64_t r = ridx_B(B, l, mm, n);
+                int t = 0;
+                for (int np = -l; np <= l; np++) {
+                    const std::complex<double> c = Ml[(n + l) + (size_t)S * (np + l)];
+                    if (std::abs(c) == 0.0) continue;
+                    HPUP_REQUIRE(t < TS, HPUP_ERR_UNSUPPORTED, "u_i u_j wider than the n +- 2 band");
+                    src[r * TS + t] = (int32_t)ridx_B(B, l, mm, np);
+                    coef[2 * (r * TS + t)] = c.real();
+                    coef[2 * (r * TS + t) + 1] = c.imag();
+                    t++;
+                }
+            }
+    }
+    DevBuf srcd = upload(src), coefd = upload(coef);
+    const int64_t nfull = nrot * nx;
+    DevBuf Ff(sizeof(double) * 2 * nfull), Fc(sizeof(double) * 2 * p->ncoef), F0(sizeof(double) * 2 * p->ncoef);
+    HPUP_HIP(hipMemcpy(Ff.p, F, Ff.bytes, hipMemcpyHostToDevice));
+    p->full_to_compact(Ff.p, Fc.p, nullptr);
+    HPUP_HIP(hipMemcpy(F0.p, Fc.p, Fc.bytes, hipMemcpyDeviceToDevice));
+    p->step(Fc.p, 1, nullptr);  // F + dt (kinematics + bias random walk)
+    launch_row_mix_add<double>(F0.p, Fc.p, srcd.as<int32_t>(), coefd.p, TS, nrc, nx, dt, nullptr);
+    HPUP_HIP(hipGetLastError());
+    p->compact_to_full(Fc.p, Ff.p, nullptr);
+    HPUP_HIP(hipMemcpy(Fnew, Ff.p, Ff.bytes, hipMemcpyDeviceToHost));
+}
+
 }  // namespace hpup
 
 extern "C" {
@@ -985,6 +1068,10 @@ int hpup_pendulum_propagate_reduced32(const float* F, const int64_t* dims, const
                                       const float* bt, const float* G, float dtt, float L, const float* u,
                                       const float* const* CG, const char* method, float* Fnew) {
     return guard([&] { propagate_gateway<float>(2, F, dims, X, nullptr, MR, bt, G, dtt, L, u, CG, method, Fnew); });
+}
+int hpup_gyrobias_propagate(const double* F, const int64_t* dims, const double* X, double dt, const double* u,
+                            const double* G1, const double* G2, double* Fnew) {
+    return guard([&] { gyrobias_gateway(F, dims, X, dt, u, G1, G2, Fnew); });
 }
 int hpup_pendulum_propagate(const double* F, const int64_t* dims, const double* X, const double* OJO,
                             const double* MR, const double* bt, const double* G, double dtt, double L,

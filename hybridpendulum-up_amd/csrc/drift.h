@@ -72,6 +72,10 @@ void launch_axis_dft_mfma(const void* in, void* out, int N, int64_t inner, int64
 
 template <class T>
 void launch_rotmix(const DriftDev<T>& D, const void* F, void* W, void* V1, void* V2, hipStream_t st);
+// out[r][x] += scale sum_t coef[r][t] in[src[r][t]][x] over rows of nx points (src -1 padded, TS per row)
+template <class T>
+void launch_row_mix_add(const void* in, void* out, const int32_t* src, const void* coef, int TS, int64_t nrows,
+                        int64_t nx, T scale, hipStream_t st);
 template <class T>
 void launch_axis_outer(const void* in, void* out, const void* Mat, int N, int64_t inner, int64_t nlines,
                        T scale, hipStream_t st);

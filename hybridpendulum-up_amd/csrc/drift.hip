@@ -234,6 +234,35 @@ void launch_drift_slab(const DriftDev<T>& D, const void* Fin, const void* Z3, co
     else go(k_drift_slab<T, 16>);
 }
 
+// out[r][x] += scale sum_t coef[r][t] in[src[r][t]][x]  (rows of nx points; src -1 padded)
+template <class T>
+__global__ __launch_bounds__(256) void k_row_mix_add(const cT<T>* __restrict__ in, cT<T>* __restrict__ out,
+                                                     const int32_t* __restrict__ src, const cT<T>* __restrict__ coef,
+                                                     int TS, int64_t nx, T scale) {
+    const int64_t r = blockIdx.y;
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    cT<T> acc = czero<T>();
+    for (int t = 0; t < TS; t++) {
+        const int32_t sr = src[r * TS + t];
+        if (sr < 0) break;
+        cfma<T>(acc, coef[r * TS + t], in[(int64_t)sr * nx + x]);
+    }
+    out[r * nx + x] = cadd<T>(out[r * nx + x], cscale<T>(acc, scale));
+}
+
+template <class T>
+void launch_row_mix_add(const void* in, void* out, const int32_t* src, const void* coef, int TS, int64_t nrows,
+                        int64_t nx, T scale, hipStream_t st) {
+    HPUP_REQUIRE(nrows <= 65535, HPUP_ERR_UNSUPPORTED, "too many rows for one launch");
+    dim3 grid((unsigned)((nx + 255) / 256), (unsigned)nrows);
+    hipLaunchKernelGGL(k_row_mix_add<T>, grid, dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out, src,
+                       (const cT<T>*)coef, TS, nx, scale);
+}
+
+template void launch_row_mix_add<double>(const void*, void*, const int32_t*, const void*, int, int64_t, int64_t, double,
+                                         hipStream_t);
+
 template void launch_rotmix<double>(const DriftDev<double>&, const void*, void*, void*, void*, hipStream_t);
 template void launch_rotmix<float>(const DriftDev<float>&, const void*, void*, void*, void*, hipStream_t);
 template void launch_axis_outer<double>(const void*, void*, const void*, int, int64_t, int64_t, double, hipStream_t);

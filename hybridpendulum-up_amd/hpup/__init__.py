@@ -74,7 +74,7 @@ EXPORTS = [
     "hpup_plan_full_to_compact", "hpup_plan_compact_to_full", "hpup_plan_stat_moments",
     "hpup_plan_profile", "hpup_plan_profile_name", "hpup_plan_profile_bytes",
     "hpup_get_stat", "hpup_psvd", "hpup_pdf_MF_normal", "hpup_pdf_MF_M2S", "hpup_plan_get_stat",
-    "hpup_pendulum_plot_getc",
+    "hpup_pendulum_plot_getc", "hpup_gyrobias_propagate",
 ]
 
 
@@ -336,6 +336,18 @@ def get_stat(f, R, x, w, return_fx=False):
     if return_fx:
         r["fx"] = fx
     return r
+
+
+def gyrobias_propagate(F, X, dt, u, G1, G2):
+    """Fnew = gyrobias_propagate(F, X, dt, u, G1, G2) (gyrobias_propagate/kernel.cu:8-120): one
+    Euler step of the attitude x gyro-bias model, FP64."""
+    F = np.asfortranarray(F, dtype=np.complex128)
+    out = np.zeros_like(F, order="F")
+    _check(lib().hpup_gyrobias_propagate(_p(F), _dims(F), _p(_interleaved(X, np.complex128)), ctypes.c_double(dt),
+                                         _p(_interleaved(u, np.complex128)),
+                                         _p(np.asfortranarray(G1, dtype=np.float64)),
+                                         _p(np.asfortranarray(G2, dtype=np.float64)), _p(out)))
+    return out
 
 
 def pendulum_plot_getc(f, e, L):

@@ -102,6 +102,16 @@ int hpup_pendulum_propagate32(const float* F, const int64_t* dims_F, const float
                               const float* G, float dtt, float L, const float* u,
                               const float* const* CG, const char* method, float* Fnew);
 
+/* Fnew = gyrobias_propagate(F, X, dt, u, G1, G2)
+ * Replaces propagation/mex/sources/gyrobias_propagate/kernel.cu:8-120 (called by
+ * gyro_bias.m:163): one Euler step of the attitude x gyro-bias density, dF =
+ * -(F (*) X_p / nx) u_p^T + sum_ij G1_ij F u_i^T u_j^T + sum_ij G2_ij c_ij(k) F
+ * (integrate.cu:241-443, gyro_bias.m:235-298) on the bias box L = 2.  F complex
+ * (2B-1,2B-1,B,N,N,N) with dims (6 entries); X complex (N,N,N,3); u complex
+ * (2B-1,2B-1,B,3); G1, G2 real 3x3.  FP64 only, as the reference. */
+int hpup_gyrobias_propagate(const double* F, const int64_t* dims, const double* X, double dt, const double* u,
+                            const double* G1, const double* G2, double* Fnew);
+
 /* Fnew = pendulum_propagate_reduced_den(F, f, X, mR, bt, G, dtt, L, u, d, w, method)
  * Replaces propagation/mex/sources/pendulum_propagate_reduced_den/kernel.cu:9-324
  * (the density form the hybrid driver calls, pendulum_reduced_hybrid.m:228):

@@ -316,3 +316,34 @@ def getc(f, e, L):
                     val += (2 * l + 1) * np.real(np.sum(F[:, :, l] * dd[:, :, l] * ph))
                 c[s, k] += val
     return c
+
+
+def gyrobias_g1_term(B, F, u, G1):
+    """sum_ij G1_ij F u_i^T u_j^T on every l block (gyro_bias.m:273-284, the
+    non-conjugate transposes of the MEX's CUBLAS_OP_T, integrate.cu:363-390)."""
+    lmax = B - 1
+    out = np.zeros_like(F)
+    for l in range(B):
+        ix = slice(lmax - l, lmax + l + 1)
+        Fl = F[ix, ix, l]
+        for i in range(3):
+            for j in range(3):
+                if G1[i, j] == 0:
+                    continue
+                T = np.einsum("mk...,nk->mn...", Fl, u[ix, ix, l, i])
+                T = np.einsum("mk...,nk->mn...", T, u[ix, ix, l, j])
+                out[ix, ix, l] += G1[i, j] * T
+    return out
+
+
+def gyrobias_step(B, N, F, X, dt, u, G1, G2):
+    """One Euler step of gyro_bias.m's integrate() (gyro_bias.m:235-298): the
+    kinematic term and the bias random walk are the refcpu derivative of the 3-D
+    model with no damping, gyroscopic or gravity term on the bias box L = 2
+    (gyro_bias.m:64), plus the gyro random walk above."""
+    M = 2 * B - 1
+    OJO = np.zeros((N, N, N, 3), dtype=np.complex128, order="F")
+    MR = np.zeros((M, M, B, 3), dtype=np.complex128, order="F")
+    dF = derivative(B, 3, N, F, X, OJO, MR, np.zeros(3), G2, 2.0, u, cg_table(B))
+    dF = dF + gyrobias_g1_term(B, np.asarray(F), u, np.asarray(G1))
+    return np.asfortranarray(F + dt * dF)
