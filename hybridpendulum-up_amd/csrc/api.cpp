@@ -204,6 +204,7 @@ template <class T> struct DriftTables {
         for (int i = 0; i < 9; i++) v.G[i] = (T)s.G[i];
         for (int i = 0; i < 3; i++) { v.b[i] = (T)s.b[i]; v.gyro[i] = (T)s.gyro[i]; v.c0[i] = (T)s.c0[i]; }
         v.ramp = s.ramp ? 1 : 0;
+        v.adj = s.adj ? 1 : 0;
         v.has_gyro = s.has_gyro ? 1 : 0;
         v.grav[0] = v.grav[1] = v.grav[2] = nullptr;
     }
@@ -373,6 +374,7 @@ template <class T> struct PlanT : PlanBase {
             mark(2);
             return;
         }
+        HPUP_REQUIRE(!D.adj, HPUP_ERR_UNSUPPORTED, "the adjoint operator needs the fused path (N <= 32)");
         mark(0);
         launch_rotmix<T>(D, Fin, Wb.p, V1.p, V2.p, st);
         mark(1);
@@ -644,6 +646,27 @@ int hpup_plan_create(const hpup_model* m, hpup_plan** out) {
         if (m->dim >= 2) spec.reset(new DriftSpec(build_drift_spec_from_model(*m)));
         auto* h = new hpup_plan();
         h->impl.reset(make_plan(*m, d.data(), w.data(), spec.get()));
+        *out = h;
+    });
+}
+
+int hpup_plan_create_operator(const hpup_model* m, int kind, hpup_plan** out) {
+    return guard([&] {
+        HPUP_REQUIRE(m && out, HPUP_ERR_ARG, "null argument");
+        *out = nullptr;
+        HPUP_REQUIRE(m->method >= 0 && m->method <= 3, HPUP_ERR_METHOD, "unknown integrator");
+        HPUP_REQUIRE(m->dim >= 2, HPUP_ERR_ARG, "operator plans need an omega grid");
+        const int B = m->B, M = 2 * B - 1;
+        std::vector<double> beta, w;
+        host_grid(B, beta, w);
+        std::vector<double> d((size_t)M * M * B * 2 * B);
+        for (int j = 0; j < 2 * B; j++) host_wigner_d(beta[j], B - 1, d.data() + (size_t)j * M * M * B);
+        DriftSpec spec = build_drift_spec_from_model(*m);
+        if (kind == 1) spec = adjoint_spec(spec);
+        else if (kind >= 2) spec = parameter_spec(spec, kind);
+        else HPUP_REQUIRE(kind == 0, HPUP_ERR_ARG, "unknown operator kind");
+        auto* h = new hpup_plan();
+        h->impl.reset(make_plan(*m, d.data(), w.data(), &spec));
         *out = h;
     });
 }

@@ -36,6 +36,8 @@ template <class T> struct DriftDev {
     // every kap_p = c0[p] + i (real circulant): the real-circulant path
     int ramp;
     T c0[3];
+    // the adjoint operator (transposed tables, damping through the V planes)
+    int adj;
 };
 
 // Integrator epilogue of the slab kernel:

@@ -322,7 +322,11 @@ template <bool R> struct FusedWaves<float, GemmF<float, R>> { static constexpr i
 // GRAV: the gravity term comes from the grid (density form, D.grav) instead
 // of the Clebsch-Gordan table; a separate instantiation so that the common
 // case carries none of its addressing.
-template <class T, class G, int DIM, bool GRAV>
+// ADJ (reduced model): the adjoint operator A^H.  The host hands over
+// transposed mixing tables and the conjugate-reversed circulants; the damping
+// term i b_p c_p T_p F becomes T_p^H(-i b_p c_p F), which joins the V planes,
+// so the first GEMM pair is not needed.
+template <class T, class G, int DIM, bool GRAV, bool ADJ = false>
 __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
                                                      const cT<T>* __restrict__ Z3, const cT<T>* F0, cT<T>* Fout,
                                                      cT<T>* ACC, StageCoef<T> sc, int64_t nslab) {
@@ -492,8 +496,17 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         __syncthreads();  // plane A complete; the V planes of the previous slab are consumed
 
         cT<T> P1[4], P2[4];
-        gm.t1(pa_re, pa_im, P1, D.c0[0]);
-        gm.t2(pa_re, pa_im, P2, D.c0[1]);
+        if constexpr (ADJ) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                P1[v] = P2[v] = czero<T>();
+                V1[v] = csub<T>(V1[v], cimul<T>(fo[v], D.b[0] * kc1));
+                V2[v] = csub<T>(V2[v], cimul<T>(fo[v], D.b[1] * kc2v[v]));
+            }
+        } else {
+            gm.t1(pa_re, pa_im, P1, D.c0[0]);
+            gm.t2(pa_re, pa_im, P2, D.c0[1]);
+        }
         if (DIM == 3 && D.has_gyro) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -734,11 +747,11 @@ template <class K> static unsigned persistent_grid(K kernel, int64_t nslab) {
     return (unsigned)g;
 }
 
-template <class T, class G, int DIM, bool GRAV>
+template <class T, class G, int DIM, bool GRAV, bool ADJ = false>
 static void launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3,
                            const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab,
                            hipStream_t st) {
-    auto kern = k_drift_fused<T, G, DIM, GRAV>;
+    auto kern = k_drift_fused<T, G, DIM, GRAV, ADJ>;
     hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, nslab)), dim3(256), 0, st, D, MX, (const cT<T>*)Fin,
                        (const cT<T>*)Z3, (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
 }
@@ -754,6 +767,7 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     if constexpr (sizeof(T) == 4) {
         if (bf16x3) {
             HPUP_REQUIRE(!grav, HPUP_ERR_UNSUPPORTED, "3xBF16 drift has no density-form gravity");
+            HPUP_REQUIRE(!D.adj, HPUP_ERR_UNSUPPORTED, "the adjoint operator has no 3xBF16 form");
             if (D.dim == 3)
                 launch_fused_t<float, GemmBf16x3, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
             else
@@ -763,6 +777,13 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     }
     HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 needs FP32 storage");
     HPUP_REQUIRE(!grav || D.dim == 2, HPUP_ERR_UNSUPPORTED, "density-form gravity is a 2-D path");
+    if (D.adj) {
+        // model plans have coordinate-ramp circulants; only that form is built
+        HPUP_REQUIRE(D.dim == 2 && !grav && D.ramp, HPUP_ERR_UNSUPPORTED,
+                     "the adjoint operator is built for the reduced model on its own omega grid");
+        launch_fused_t<T, GemmF<T, true>, 2, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        return;
+    }
     if (D.ramp) {
         if (D.dim == 3)
             launch_fused_t<T, GemmF<T, true>, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);

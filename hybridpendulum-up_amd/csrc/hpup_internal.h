@@ -126,7 +126,19 @@ struct DriftSpec {
     std::vector<double> u_coef;    // 2 * nRc * TU * 2
     // diagonal u_3 term: e3[r] = -u_3[n,n]^l (complex), 3-D only
     std::vector<double> e3;        // nRc * 2
+    // the adjoint operator A^H (see adjoint_spec)
+    bool adj = false;
 };
+
+// The adjoint A^H of a reduced-model (dim 2) operator, for the gradient
+// harness: mixing tables transposed with conjugate coefficients (the
+// Clebsch-Gordan ones negated as well, since they enter as -i c), circulant
+// kernels conj(kappa(-d)), damping moved after the mixing (adj flag).
+DriftSpec adjoint_spec(const DriftSpec& s);
+// The derivative of the operator with respect to one model parameter, itself
+// an operator of the same form: term = 2 + p gives dA/db_p, term = 2 + dim +
+// i + dim j gives dA/dG_ij (the operator is linear in b and G).
+DriftSpec parameter_spec(const DriftSpec& s, int term);
 
 // Builds the structured operator from the gateway arrays (X, OJO, MR, u, CG
 // as the reference's drivers pass them).  Throws HPUP_ERR_UNSUPPORTED if an

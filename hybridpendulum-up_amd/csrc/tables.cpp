@@ -74,6 +74,98 @@ void host_wigner_d(double beta, int lmax, double* d) {
 // clebsch_gordan.m:1-41: per (l, m) the null vector of the tridiagonal
 // J^2 - l(l+1) block on m1 + m2 = m, last (largest m1) component positive,
 // unit norm; solved by LU with partial pivoting as MATLAB's linsolve does.
+// Adjoint of a reduced-model operator (for the gradient harness; no
+// counterpart in the reference).  With A = sum of (row mixing) o (omega
+// circulant) terms, (K T)^H = T^H K^H and the row mixing acts on the rotation
+// index while T_p acts on omega, so each factor is transposed on its own:
+// mixing r <- src with c becomes src <- r with conj(c) (negated for the
+// Clebsch-Gordan terms, which enter as -i c F), T_p's kernel becomes
+// conj(kappa_p(-d)), and the damping i b_p c_p T_p F becomes
+// T_p^H(-i b_p c_p F), applied by the kernel's adj path.
+DriftSpec adjoint_spec(const DriftSpec& s) {
+    HPUP_REQUIRE(s.dim == 2 && !s.adj, HPUP_ERR_UNSUPPORTED, "adjoint operator: reduced model (dim 2) only");
+    DriftSpec a = s;
+    a.adj = true;
+    const int N = s.N, dim = s.dim;
+    const int64_t nrc = nrc_of(s.B);
+    for (int p = 0; p < dim; p++)
+        for (int j = 0; j < N; j++) {
+            const int jm = (N - j) % N;
+            a.kappa[2 * (N * p + j)] = s.kappa[2 * (N * p + jm)];
+            a.kappa[2 * (N * p + j) + 1] = -s.kappa[2 * (N * p + jm) + 1];
+        }
+    // Clebsch-Gordan terms
+    std::vector<std::vector<std::pair<int32_t, std::vector<cd>>>> cg(nrc);
+    for (int64_t r = 0; r < nrc; r++)
+        for (int t = 0; t < s.TG; t++) {
+            const int32_t src = s.cg_src[r * s.TG + t];
+            if (src < 0) break;
+            std::vector<cd> c(dim);
+            for (int p = 0; p < dim; p++) {
+                const size_t k = ((r * s.TG + t) * dim + p) * 2;
+                c[p] = -std::conj(cd(s.cg_coef[k], s.cg_coef[k + 1]));
+            }
+            cg[src].push_back({(int32_t)r, c});
+        }
+    int TG = 0;
+    for (auto& v : cg) TG = std::max<int>(TG, (int)v.size());
+    a.TG = TG;
+    a.cg_src.assign(nrc * std::max(TG, 1), -1);
+    a.cg_coef.assign(nrc * std::max(TG, 1) * dim * 2, 0.0);
+    for (int64_t r = 0; r < nrc; r++)
+        for (size_t t = 0; t < cg[r].size(); t++) {
+            a.cg_src[r * TG + t] = cg[r][t].first;
+            for (int p = 0; p < dim; p++) {
+                a.cg_coef[((r * TG + t) * dim + p) * 2] = cg[r][t].second[p].real();
+                a.cg_coef[((r * TG + t) * dim + p) * 2 + 1] = cg[r][t].second[p].imag();
+            }
+        }
+    // u_1, u_2 terms
+    std::vector<std::vector<std::pair<int32_t, cd>>> ut(2 * nrc);
+    for (int i = 0; i < 2; i++)
+        for (int64_t r = 0; r < nrc; r++)
+            for (int t = 0; t < s.TU; t++) {
+                const int64_t k = (i * nrc + r) * s.TU + t;
+                const int32_t src = s.u_src[k];
+                if (src < 0) break;
+                ut[i * nrc + src].push_back({(int32_t)r, std::conj(cd(s.u_coef[2 * k], s.u_coef[2 * k + 1]))});
+            }
+    int TU = 0;
+    for (auto& v : ut) TU = std::max<int>(TU, (int)v.size());
+    HPUP_REQUIRE(TU <= 4, HPUP_ERR_UNSUPPORTED, "adjoint u terms not banded");
+    a.TU = TU;
+    a.u_src.assign(2 * nrc * std::max(TU, 1), -1);
+    a.u_coef.assign(2 * nrc * std::max(TU, 1) * 2, 0.0);
+    for (int64_t i = 0; i < 2 * nrc; i++)
+        for (size_t t = 0; t < ut[i].size(); t++) {
+            a.u_src[i * TU + t] = ut[i][t].first;
+            a.u_coef[(i * TU + t) * 2] = ut[i][t].second.real();
+            a.u_coef[(i * TU + t) * 2 + 1] = ut[i][t].second.imag();
+        }
+    return a;
+}
+
+DriftSpec parameter_spec(const DriftSpec& s, int term) {
+    const int dim = s.dim;
+    HPUP_REQUIRE(term >= 2 && term < 2 + dim + dim * dim, HPUP_ERR_ARG, "parameter operator index out of range");
+    DriftSpec a = s;
+    const int64_t nrc = nrc_of(s.B);
+    a.adj = false;
+    a.TG = 0;
+    a.cg_src.assign(nrc, -1);
+    a.cg_coef.assign(nrc * dim * 2, 0.0);
+    a.TU = 0;
+    a.u_src.assign(2 * nrc, -1);
+    a.u_coef.assign(2 * nrc * 2, 0.0);
+    std::fill(a.e3.begin(), a.e3.end(), 0.0);
+    for (int p = 0; p < 3; p++) a.b[p] = a.gyro[p] = 0.0;
+    a.has_gyro = false;
+    for (int i = 0; i < 9; i++) a.G[i] = 0.0;
+    if (term < 2 + dim) a.b[term - 2] = 1.0;
+    else a.G[term - 2 - dim] = 1.0;
+    return a;
+}
+
 void host_clebsch_gordan(int l1, int l2, std::vector<double>& CG) {
     const int dim = (2 * l1 + 1) * (2 * l2 + 1);
     CG.assign((size_t)dim * dim, 0.0);

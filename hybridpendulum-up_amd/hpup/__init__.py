@@ -74,7 +74,7 @@ EXPORTS = [
     "hpup_plan_full_to_compact", "hpup_plan_compact_to_full", "hpup_plan_stat_moments",
     "hpup_plan_profile", "hpup_plan_profile_name", "hpup_plan_profile_bytes",
     "hpup_get_stat", "hpup_psvd", "hpup_pdf_MF_normal", "hpup_pdf_MF_M2S", "hpup_plan_get_stat",
-    "hpup_pendulum_plot_getc", "hpup_gyrobias_propagate",
+    "hpup_pendulum_plot_getc", "hpup_gyrobias_propagate", "hpup_plan_create_operator",
 ]
 
 
@@ -97,6 +97,7 @@ def lib():
         L.hpup_plan_num_coeffs.restype = ctypes.c_int64
         L.hpup_plan_num_grid.restype = ctypes.c_int64
         L.hpup_plan_create.argtypes = [ctypes.POINTER(Model), ctypes.POINTER(ctypes.c_void_p)]
+        L.hpup_plan_create_operator.argtypes = [ctypes.POINTER(Model), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         for name in ("hpup_plan_forward", "hpup_plan_backward", "hpup_plan_derivative",
                      "hpup_plan_full_to_compact", "hpup_plan_compact_to_full"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -396,10 +397,15 @@ class Plan:
     """Device-resident propagator.  Coefficients are torch tensors on cuda:0
     in the compact layout [r][k3][k2][k1] (complex128 for fp64, complex64
     otherwise); the grid is real (2B,2B,2B,N..) in MATLAB order, stored as a
-    flat tensor."""
+    flat tensor.
+
+    operator selects what the plan applies (no counterpart in the reference;
+    used by the gradient harness in hpup.adjoint): 0 the model's operator A,
+    1 its adjoint A^H (dim 2; step then applies the adjoint of the forward
+    step), 2 + p the derivative dA/db_p, 2 + dim + i + dim j dA/dG_ij."""
 
     def __init__(self, B, dim, N=1, L=1.0, b=(0, 0, 0), G=None, Jt=None, dt=0.0, method="euler",
-                 precision="fp64"):
+                 precision="fp64", operator=0):
         import torch
         if not torch.cuda.is_available():
             raise HpupError("Plan needs a GPU")
@@ -424,8 +430,12 @@ class Plan:
         self.nx = self.N ** dim
         self.nrc = B * (2 * B - 1) * (2 * B + 1) // 3
         h = ctypes.c_void_p()
-        _check(lib().hpup_plan_create(ctypes.byref(m), ctypes.byref(h)))
+        if operator:
+            _check(lib().hpup_plan_create_operator(ctypes.byref(m), int(operator), ctypes.byref(h)))
+        else:
+            _check(lib().hpup_plan_create(ctypes.byref(m), ctypes.byref(h)))
         self._h = h
+        self.operator = int(operator)
         self.ncoef = lib().hpup_plan_num_coeffs(h)
         self.ngrid = lib().hpup_plan_num_grid(h)
         self.cdtype = torch.complex128 if self.precision == "fp64" else torch.complex64

@@ -260,6 +260,16 @@ typedef struct {
 int hpup_plan_create(const hpup_model* model, hpup_plan** plan);
 int hpup_plan_destroy(hpup_plan* plan);
 
+/* Operator plans for the gradient harness.  No counterpart in the reference
+ * (it has no adjoint); callers: gradient computations over the plan API.
+ *   kind 0: the model's derivative operator A (as hpup_plan_create);
+ *   kind 1: its adjoint A^H (reduced model, dim 2): hpup_plan_derivative applies
+ *           A^H, hpup_plan_step applies P(dt A^H), the adjoint of the forward
+ *           step P(dt A) for every integrator (P has real coefficients);
+ *   kind 2 + p (p < dim): dA/db_p;  kind 2 + dim + i + dim j: dA/dG_ij
+ *           (A is linear in b and G), for hpup_plan_derivative. */
+int hpup_plan_create_operator(const hpup_model* model, int kind, hpup_plan** plan);
+
 /* Number of compact coefficients: nRc * nx, nRc = B(2B-1)(2B+1)/3. */
 int64_t hpup_plan_num_coeffs(const hpup_plan* plan);
 /* Number of grid points: (2B)^3 * nx. */

@@ -1,0 +1,146 @@
+"""Adjoint and parameter-derivative operators of the reduced model (dim 2),
+and the gradient harness built on them (hpup.adjoint).  The reference has no
+adjoint; these operators are pinned by identities that fix them given the
+forward operator, whose parity with the oracle the other GPU tests hold:
+
+* <A x, y> = <x, A^H y> for random complex x, y (fixes A^H);
+* <P x, y> = <x, P^H y> for the step of every integrator;
+* A(theta + e_t) x - A(theta) x = (dA/dtheta_t) x (A is linear in b and G);
+* the harness's gradients against central differences of J(theta) through
+  separately built forward plans, and the F0 gradient against the exact
+  change of the (linear) objective.
+
+Tolerances: FP64 1e-12 relative on the identities (roundoff of two fused
+evaluations), 2e-7 on the central differences (h = 1e-4: the O(h^2) error of
+a polynomial objective); FP32 identities 2e-5.
+"""
+import numpy as np
+import pytest
+
+import hpup
+from hpup.adjoint import Gradient
+
+pytestmark = pytest.mark.gpu
+
+B, N, L = 5, 12, 8.0
+BT = (0.3, 0.2)
+G2 = np.array([[0.12, 0.03], [0.05, 0.18]])
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _rand(n, dtype, seed):
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    re = torch.randn(n, generator=g, dtype=torch.float64)
+    im = torch.randn(n, generator=g, dtype=torch.float64)
+    return torch.complex(re, im).to(dtype).cuda()
+
+
+def _dot(x, y):
+    import torch
+    return complex(torch.vdot(x, y).item())
+
+
+def _plan(precision="fp64", operator=0, b=BT, G=G2, dt=0.01, method="euler"):
+    return hpup.Plan(B, 2, N, L, b=tuple(b) + (0,), G=G, dt=dt, method=method, precision=precision,
+                     operator=operator)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_operator_adjoint_identity(precision, tol):
+    _print("adjoint identity", precision)
+    A, AH = _plan(precision), _plan(precision, operator=1)
+    x, y = _rand(A.ncoef, A.cdtype, 1), _rand(A.ncoef, A.cdtype, 2)
+    Ax, AHy = A.derivative(x), AH.derivative(y)
+    lhs, rhs = _dot(Ax, y), _dot(x, AHy)
+    scale = float(Ax.abs().pow(2).sum().sqrt() * y.abs().pow(2).sum().sqrt())
+    assert abs(lhs - rhs) <= tol * scale, (lhs, rhs)
+    # A^H is not A: the operator is not self-adjoint
+    assert abs(_dot(A.derivative(y), x) - _dot(y, A.derivative(x))) > 100 * tol * scale
+
+
+@pytest.mark.parametrize("method", ["euler", "midpoint", "rk2", "rk4"])
+def test_step_adjoint_identity(method):
+    _print("step adjoint identity", method)
+    P, PH = _plan(method=method, dt=0.02), _plan(method=method, dt=0.02, operator=1)
+    x, y = _rand(P.ncoef, P.cdtype, 3), _rand(P.ncoef, P.cdtype, 4)
+    Px, PHy = P.step(x.clone(), 2), PH.step(y.clone(), 2)
+    lhs, rhs = _dot(Px, y), _dot(x, PHy)
+    scale = float(Px.abs().pow(2).sum().sqrt() * y.abs().pow(2).sum().sqrt())
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+
+
+@pytest.mark.parametrize("term", range(2, 8))
+def test_parameter_operator_linearity(term):
+    _print("parameter operator", term)
+    b, G = list(BT), G2.copy()
+    if term < 4:
+        b[term - 2] += 1.0
+    else:
+        i, j = (term - 4) % 2, (term - 4) // 2
+        G[i, j] += 1.0
+    x = _rand(_plan().ncoef, _plan().cdtype, 5)
+    diff = _plan(b=b, G=G).derivative(x) - _plan().derivative(x)
+    dA = _plan(operator=term).derivative(x)
+    assert float((diff - dA).abs().max()) <= 1e-12 * float(dA.abs().max())
+    assert float(dA.abs().max()) > 0
+
+
+def _objective(w, F0, b, G, K, method, dt):
+    F = _plan(b=b, G=G, dt=dt, method=method).step(F0.clone(), K)
+    return _dot(w, F).real
+
+
+@pytest.mark.parametrize("method,checkpoint", [("rk4", 3), ("euler", None), ("midpoint", 2)])
+def test_gradient_against_differences(method, checkpoint):
+    _print("gradient", method, checkpoint)
+    K, dt = 7, 0.02
+    gr = Gradient(B, N, L, b=BT, G=G2, dt=dt, method=method)
+    F0 = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 6)
+    w = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 7)
+    FK, gF0, g = gr.run(F0, K, lambda F: w, checkpoint=checkpoint)
+    # the final state is the plain forward propagation
+    ref = _plan(dt=dt, method=method).step(F0.clone(), K)
+    assert float((FK - ref).abs().max()) <= 1e-13 * float(ref.abs().max())
+    # F0: J is linear in F0
+    d = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 8)
+    J0 = _objective(w, F0, BT, G2, K, method, dt)
+    dJ = _objective(w, F0 + d, BT, G2, K, method, dt) - J0
+    assert abs(dJ - _dot(gF0, d).real) <= 1e-11 * abs(dJ)
+    # parameters: central differences
+    h = 1e-4
+    for t in range(6):
+        bp, bm, Gp, Gm = list(BT), list(BT), G2.copy(), G2.copy()
+        if t < 2:
+            bp[t] += h
+            bm[t] -= h
+            ana = g["b"][t]
+        else:
+            i, j = (t - 2) % 2, (t - 2) // 2
+            Gp[i, j] += h
+            Gm[i, j] -= h
+            ana = g["G"][i, j]
+        fd = (_objective(w, F0, bp, Gp, K, method, dt) - _objective(w, F0, bm, Gm, K, method, dt)) / (2 * h)
+        _print("  term", t, ana, fd)
+        assert abs(ana - fd) <= 2e-7 * max(abs(fd), 1e-3 * abs(J0)), (t, ana, fd)
+
+
+def test_gradient_checkpoint_invariance():
+    _print("gradient checkpoints")
+    gr = Gradient(B, N, L, b=BT, G=G2, dt=0.02, method="rk4")
+    F0 = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 9)
+    w = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 10)
+    _, g1, p1 = gr.run(F0, 6, lambda F: w)
+    _, g2, p2 = gr.run(F0, 6, lambda F: w, checkpoint=2)
+    assert float((g1 - g2).abs().max()) == 0.0
+    assert np.array_equal(p1["b"], p2["b"]) and np.array_equal(p1["G"], p2["G"])
+
+
+def test_operator_kind_errors():
+    with pytest.raises(hpup.HpupError):
+        _plan(operator=8)
+    with pytest.raises(hpup.HpupError):
+        hpup.Plan(4, 3, 8, L, precision="fp64", operator=1)  # the adjoint is built for dim 2
