@@ -392,6 +392,16 @@ template <class T> struct PlanT : PlanBase {
     // The stages of one step of a Runge-Kutta method on the state F
     // (pendulum.m:346-385); no fused stage writes its own input, so RK4's
     // middle stages alternate Fs and Fs2 on the fused path.
+    // The drift operator is linear and autonomous, so each integrator's step
+    // is the polynomial P(dt A) of its stability function: 1 + z + z^2/2 for
+    // midpoint and RK2 (pendulum.m:357-371), sum_{j<=4} z^j/j! for RK4
+    // (pendulum.m:373-385).  The stages evaluate it in Horner form,
+    //   y <- F + (dt / j) A y,  j = s, s-1, ..., 1,
+    // the reference's polynomial with one stream (F) besides each stage's input
+    // and output, where the stage sums also read and write an accumulator.
+    // Density-form plans keep the stage sums: their first stage takes gravity
+    // from the caller's grid density, not from the stage input, so the step is
+    // not a polynomial in one operator.
     int rk_stages(cT<T>* F, StageRef* sr) {
         const T dt = (T)model.dt;
         void* Fm = Fs.p;
@@ -399,6 +409,24 @@ template <class T> struct PlanT : PlanBase {
             ensure(Fs2, cbytes());
             Fm = Fs2.p;
         }
+        if (!den) {
+            switch (model.method) {
+            case HPUP_MIDPOINT:
+            case HPUP_RK2:
+                sr[0] = {F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 0, 0, 0, 0, 0}};
+                sr[1] = {Fs.p, F, F, StageCoef<T>{dt, 0, 1, 0, 0, 1, 0, 0, 0}};
+                return 2;
+            case HPUP_RK4:
+                sr[0] = {F, nullptr, Fs.p, StageCoef<T>{dt / 4, 0, 1, 0, 0, 0, 0, 0, 0}};
+                sr[1] = {Fs.p, F, Fm, StageCoef<T>{dt / 3, 0, 1, 0, 0, 1, 0, 0, 0}};
+                sr[2] = {Fm, F, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 0, 1, 0, 0, 0}};
+                sr[3] = {Fs.p, F, F, StageCoef<T>{dt, 0, 1, 0, 0, 1, 0, 0, 0}};
+                return 4;
+            default:
+                throw Error{HPUP_ERR_METHOD, "unknown integrator"};
+            }
+        }
+        ensure(ACC, cbytes());
         switch (model.method) {
         case HPUP_MIDPOINT:
             sr[0] = {F, nullptr, Fs.p, StageCoef<T>{dt / 2, 0, 1, 0, 0, 0, 0, 0, 0}};
@@ -451,7 +479,7 @@ template <class T> struct PlanT : PlanBase {
         HPUP_REQUIRE(fused() || !bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 drift needs an omega grid of at most 32");
         if (!fused()) { ensure(Wb, cbytes()); ensure(V1, cbytes()); ensure(V2, cbytes()); }
         if (model.dim == 3) ensure(Z3, cbytes());
-        if (rk) { ensure(Fs, cbytes()); ensure(ACC, cbytes()); }
+        if (rk) ensure(Fs, cbytes());
     }
     void step(void* Fv, int nsteps, hipStream_t st) override {
         ensure_drift_buffers(true);

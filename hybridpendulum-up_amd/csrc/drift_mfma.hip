@@ -558,19 +558,19 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             // together, under workgroup-uniform conditions only, and every
             // loaded value is consumed.  These streams and the outputs are
             // touched once: non-temporal.
-            const cT<T>* f0p = (sc.read_f0 ? F0 : Fin) + base;
-            if (sc.read_acc) {
+            if (sc.read_f0) {
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    f0v[v] = ld_once<T>(f0p + offw[v]);
-                    acv[v] = ld_once<T>(ACC + base + offw[v]);
-                }
+                for (int v = 0; v < 4; v++) f0v[v] = ld_once<T>(F0 + base + offw[v]);
             } else {
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    f0v[v] = ld_once<T>(f0p + offw[v]);
-                    acv[v] = czero<T>();
-                }
+                for (int v = 0; v < 4; v++) f0v[v] = fo[v];  // the slab itself, already loaded
+            }
+            if (sc.read_acc) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) acv[v] = ld_once<T>(ACC + base + offw[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) acv[v] = czero<T>();
             }
 #pragma unroll
             for (int v = 0; v < 4; v++) {
