@@ -134,9 +134,9 @@ template <class T> struct DriftTables {
                 const int32_t src = s.cg_src[r * s.TG + t];
                 if (src < 0) break;
                 auto& c = slot(r, src);
-                for (int p = 0; p < s.dim; p++) {
-                    c[2 * p] += s.cg_coef[((r * s.TG + t) * s.dim + p) * 2];
-                    c[2 * p + 1] += s.cg_coef[((r * s.TG + t) * s.dim + p) * 2 + 1];
+                for (int p = 0; p < s.dim; p++) {  // stored as -i CG_p
+                    c[2 * p] += s.cg_coef[((r * s.TG + t) * s.dim + p) * 2 + 1];
+                    c[2 * p + 1] -= s.cg_coef[((r * s.TG + t) * s.dim + p) * 2];
                 }
             }
             for (int q = 0; q < 2; q++)
@@ -151,12 +151,19 @@ template <class T> struct DriftTables {
         }
         int TS = 1;
         for (auto& rw : rows) TS = std::max<int>(TS, (int)rw.size());
+        HPUP_REQUIRE(nrc < ((int64_t)1 << 24), HPUP_ERR_UNSUPPORTED, "fused mixing table: more than 2^24 rotation rows");
         std::vector<int32_t> src(nrc * TS, -1);
         std::vector<double> coef(nrc * TS * NC * 2, 0.0);
         for (int64_t r = 0; r < nrc; r++)
             for (size_t t = 0; t < rows[r].size(); t++) {
-                src[r * TS + t] = rows[r][t].first;
-                std::copy(rows[r][t].second.begin(), rows[r][t].second.end(), coef.begin() + (r * TS + t) * NC * 2);
+                const std::vector<double>& c = rows[r][t].second;
+                auto nz = [&](int k) { return c[2 * k] != 0.0 || c[2 * k + 1] != 0.0; };
+                int fl = 0;
+                for (int p = 0; p < s.dim; p++) fl |= nz(p) ? 1 : 0;
+                fl |= nz(s.dim) ? 2 : 0;
+                fl |= nz(s.dim + 1) ? 4 : 0;
+                src[r * TS + t] = rows[r][t].first | (fl << 24);
+                std::copy(c.begin(), c.end(), coef.begin() + (r * TS + t) * NC * 2);
             }
         mix_src = upload(src);
         mix_coef = upload(cast_vec<T>(coef));

@@ -49,7 +49,9 @@ template <class T> struct StageCoef {
 
 // Rotation-mixing sources per output row r for the fused kernel: up to TS
 // distinct source rows; per source dim + 2 complex coefficients
-// [CG_1 .. CG_dim, -u_1, -u_2].
+// [-i CG_1 .. -i CG_dim, -u_1, -u_2].  An entry of src holds the source row
+// in bits 0-23 and which of its terms are nonzero in bits 24 (Clebsch-Gordan),
+// 25 (u_1) and 26 (u_2); -1 pads.
 template <class T> struct MixDev {
     int TS;
     const int32_t* src;   // nrc*TS, -1 padded

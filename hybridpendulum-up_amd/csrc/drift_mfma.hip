@@ -21,6 +21,7 @@
 // k_axis3 computes Z3 = T3 F (circulant along the outermost omega axis) as the
 // GEMM C3 [F_re; F_im] over 64-column chunks of the k1,k2 plane.
 #include <algorithm>
+#include <type_traits>
 
 #include "drift.h"
 #include "kernels_common.h"
@@ -301,8 +302,13 @@ template <class T> struct MixBatch;  // source rows whose loads are issued toget
 template <> struct MixBatch<float> { static constexpr int n = 3; };
 template <> struct MixBatch<double> { static constexpr int n = 2; };
 
-template <class T>
-__device__ __forceinline__ bool cnz(cT<T> c) { return c.x != T(0) || c.y != T(0); }
+// p + ob bytes: a 32-bit unsigned offset from a workgroup-uniform base, so
+// the access takes the scalar-base + 32-bit vector-offset address form
+// instead of a 64-bit address per lane
+template <class P> __device__ __forceinline__ P* at_b(P* p, unsigned ob) {
+    using C = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+    return reinterpret_cast<P*>(reinterpret_cast<C*>(p) + ob);
+}
 
 // Tables read at workgroup-uniform indices (mixing sources and coefficients,
 // per-row and per-k3 constants) are read through the constant address space:
@@ -395,13 +401,14 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         const int lanez = tidz & 63, wz = tidz >> 6;
         const int k1w = 16 * (wz & 1) + (lanez & 15), lqz = lanez >> 4;
         int k2w[4];
-        unsigned offw[4];
+        unsigned offb[4];  // byte offsets of the elements in a slab
         bool ok[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             k2w[v] = 16 * (wz >> 1) + G::orow(lqz, v);
             ok[v] = k1w < N && k2w[v] < N;
-            offw[v] = (unsigned)((k2w[v] < N ? k2w[v] : N - 1) * N + (k1w < N ? k1w : N - 1));
+            offb[v] = (unsigned)((k2w[v] < N ? k2w[v] : N - 1) * N + (k1w < N ? k1w : N - 1)) *
+                      (unsigned)sizeof(cT<T>);
         }
         const int64_t base = ((int64_t)r * nz + k3) * NN;
         const cT<T>* Fb = Fin + base;  // this slab
@@ -413,13 +420,13 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         for (int i = 0; i < 4; i++) {
             const int idx = tidz + 256 * i, a2 = idx >> 5, a1 = idx & 31;
             const bool in = a2 < N && a1 < N;
-            const cT<T> x = Fb[in ? (unsigned)(a2 * N + a1) : 0u];
+            const cT<T> x = *at_b(Fb, (in ? (unsigned)(a2 * N + a1) : 0u) * (unsigned)sizeof(cT<T>));
             own[i] = in ? x : czero<T>();
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            fo[v] = Fb[offw[v]];
-            Zv[v] = DIM == 3 ? ld_once<T>(Z3 + base + offw[v]) : czero<T>();
+            fo[v] = *at_b(Fb, offb[v]);
+            Zv[v] = DIM == 3 ? ld_once<T>(at_b(Z3 + base, offb[v])) : czero<T>();
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {  // slab -> plane A, zero padded to 32 x 32
@@ -429,9 +436,10 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         }
 
         // rotation mixing: sources in batches of MB rows, structurally zero
-        // coefficients skipped (the branches are uniform over the workgroup).
-        // The Clebsch-Gordan terms enter W as -i (sum_p kc_p CG_p) F_src with
-        // the kc of this lane's element, so one accumulator holds all of them.
+        // terms skipped by the table's flags (the branches are uniform over
+        // the workgroup).  The Clebsch-Gordan terms enter W as
+        // (sum_p kc_p (-i CG_p)) F_src with the kc of this lane's element (the
+        // table holds -i CG_p), so one accumulator holds all of them.
         cT<T> V1[4], V2[4], Wv[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -449,34 +457,33 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             for (int u = 0; u < MB; u++) src[u] = t0 + u < TS ? srcT[t0 + u] : -1;
 #pragma unroll
             for (int u = 0; u < MB; u++) {
-                const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : src[u]) * nz + k3) * NN;
+                const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : (src[u] & 0xffffff)) * nz + k3) * NN;
 #pragma unroll
-                for (int v = 0; v < 4; v++) fv[u][v] = Fs[offw[v]];
+                for (int v = 0; v < 4; v++) fv[u][v] = *at_b(Fs, offb[v]);
             }
 #pragma unroll
             for (int u = 0; u < MB; u++) {
                 if (src[u] < 0) continue;
+                const int fl = src[u] >> 24;
                 const int64_t cb = (int64_t)(t0 + u) * NC;
-                const cT<T> g0 = kc_cplx<T>(coefT, cb), g1 = kc_cplx<T>(coefT, cb + 1),
-                            g2 = DIM == 3 ? kc_cplx<T>(coefT, cb + 2) : czero<T>();
-                const bool h0 = cnz<T>(g0) || (DIM == 3 && cnz<T>(g2)), h1 = cnz<T>(g1);
-                if (h0 || h1) {
-                    // kc1 CG_1 + kc3 CG_3 is the same for the lane's four elements
+                if (fl & 1) {
+                    const cT<T> g0 = kc_cplx<T>(coefT, cb), g1 = kc_cplx<T>(coefT, cb + 1),
+                                g2 = DIM == 3 ? kc_cplx<T>(coefT, cb + 2) : czero<T>();
+                    // kc1 (-i CG_1) + kc3 (-i CG_3) is the same for the lane's four elements
                     const cT<T> e = cadd<T>(cscale<T>(g0, kc1), cscale<T>(g2, kc3));
 #pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const cT<T> c = cadd<T>(e, cscale<T>(g1, kc2v[v]));
-                        const cT<T> t = cmul<T>(c, fv[u][v]);
-                        Wv[v] = cmk<T>(Wv[v].x + t.y, Wv[v].y - t.x);  // W -= i c f
-                    }
+                    for (int v = 0; v < 4; v++) cfma<T>(Wv[v], cadd<T>(e, cscale<T>(g1, kc2v[v])), fv[u][v]);
                 }
-                const cT<T> cu1 = kc_cplx<T>(coefT, cb + DIM), cu2 = kc_cplx<T>(coefT, cb + DIM + 1);
-                if (cnz<T>(cu1))
+                if (fl & 2) {
+                    const cT<T> cu1 = kc_cplx<T>(coefT, cb + DIM);
 #pragma unroll
                     for (int v = 0; v < 4; v++) cfma<T>(V1[v], cu1, fv[u][v]);
-                if (cnz<T>(cu2))
+                }
+                if (fl & 4) {
+                    const cT<T> cu2 = kc_cplx<T>(coefT, cb + DIM + 1);
 #pragma unroll
                     for (int v = 0; v < 4; v++) cfma<T>(V2[v], cu2, fv[u][v]);
+                }
             }
         }
         // W += D(k) F - sum_p c_p (grid gravity term)
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             if constexpr (GRAV)
 #pragma unroll
                 for (int p = 0; p < DIM; p++)  // gravity evaluated on the grid (density form)
-                    wv = csub<T>(wv, cimul<T>((D.grav[p] + base)[offw[v]], kcp[p]));
+                    wv = csub<T>(wv, cimul<T>(*at_b(D.grav[p] + base, offb[v]), kcp[p]));
             Wv[v] = wv;
         }
         __syncthreads();  // plane A complete; the V planes of the previous slab are consumed
@@ -551,7 +558,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         if (sc.deriv_only) {
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                if (ok[v]) st_once<T>(Fout + base + offw[v], q[v]);
+                if (ok[v]) st_once<T>(at_b(Fout + base, offb[v]), q[v]);
         } else {
             // RK stages read F0 and ACC here (a stage without F0 takes the
             // slab itself): the loads of the four elements are issued
@@ -560,26 +567,35 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             // touched once: non-temporal.
             if (sc.read_f0) {
 #pragma unroll
-                for (int v = 0; v < 4; v++) f0v[v] = ld_once<T>(F0 + base + offw[v]);
+                for (int v = 0; v < 4; v++) f0v[v] = ld_once<T>(at_b(F0 + base, offb[v]));
             } else {
 #pragma unroll
                 for (int v = 0; v < 4; v++) f0v[v] = fo[v];  // the slab itself, already loaded
             }
             if (sc.read_acc) {
 #pragma unroll
-                for (int v = 0; v < 4; v++) acv[v] = ld_once<T>(ACC + base + offw[v]);
+                for (int v = 0; v < 4; v++) acv[v] = ld_once<T>(at_b(ACC + base, offb[v]));
             } else {
 #pragma unroll
                 for (int v = 0; v < 4; v++) acv[v] = czero<T>();
             }
+            // every loaded value is consumed before the first store: a load
+            // still in flight at the loop's back edge (its use sunk into a
+            // masked store) makes the next slab wait for vmcnt(0), i.e. for
+            // all of this slab's stores as well
+            cT<T> fn[4], an[4];
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const cT<T> upd = cadd<T>(cscale<T>(acv[v], sc.ca), cscale<T>(q[v], sc.cd));
-                const cT<T> fn = cadd<T>(f0v[v], cscale<T>(upd, sc.dt));
-                const cT<T> an = cadd<T>(cscale<T>(acv[v], sc.ea), cscale<T>(q[v], sc.ed));
+                fn[v] = cadd<T>(f0v[v], cscale<T>(upd, sc.dt));
+                an[v] = cadd<T>(cscale<T>(acv[v], sc.ea), cscale<T>(q[v], sc.ed));
+                asm volatile("" : "+v"(fn[v].x), "+v"(fn[v].y), "+v"(an[v].x), "+v"(an[v].y));
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
                 if (ok[v]) {
-                    st_once<T>(Fout + base + offw[v], fn);
-                    if (sc.write_acc) st_once<T>(ACC + base + offw[v], an);
+                    st_once<T>(at_b(Fout + base, offb[v]), fn[v]);
+                    if (sc.write_acc) st_once<T>(at_b(ACC + base, offb[v]), an[v]);
                 }
             }
         }
