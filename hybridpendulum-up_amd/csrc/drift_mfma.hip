@@ -611,6 +611,10 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
 
 // ------------------------------------------------------------ axis 3 ----
 
+// T3 F: one workgroup per rotation row r and chunk of 64 (k1, k2) columns.
+// The chunk's loads are unconditional (out-of-range elements read a clamped
+// address and are zeroed), all eight issued before the first wait, with
+// 32-bit offsets from the row's base.
 template <class T>
 __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* __restrict__ in,
                                                     cT<T>* __restrict__ out) {
@@ -619,19 +623,26 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
     constexpr int CP = 65;
     __shared__ T sre[32 * CP], sim[32 * CP];
     __shared__ CircTab<T> tab;
-    const int N = D.N;
-    const int64_t NN = (int64_t)N * N;
-    const int64_t r = blockIdx.y;
-    const int64_t j0 = (int64_t)blockIdx.x * 64;
+    const int N = D.N, NN = N * N;
+    const int r = blockIdx.y, c = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
-    for (int idx = tid; idx < 32 * 64; idx += 256) {
-        const int a = idx >> 6, jj = idx & 63;
-        const int64_t j = j0 + jj;
-        const cT<T> v = (a < N && j < NN) ? in[(r * N + a) * NN + j] : czero<T>();
-        sre[a * CP + jj] = v.x;
-        sim[a * CP + jj] = v.y;
+    const cT<T>* rin = in + (int64_t)r * N * NN;
+    cT<T>* rout = out + (int64_t)r * N * NN;
+    cT<T> x[8];  // row a = w + 4 i, column 64 c + lane
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int a = w + 4 * i, j = 64 * c + lane;
+        const bool ok = a < N && j < NN;
+        const cT<T> v = *at_b(rin, ok ? (unsigned)(a * NN + j) * (unsigned)sizeof(cT<T>) : 0u);
+        x[i] = ok ? v : czero<T>();
     }
     tab.fill(D.kap + 2 * N, N, tid);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int a = w + 4 * i;
+        sre[a * CP + lane] = x[i].x;
+        sim[a * CP + lane] = x[i].y;
+    }
     __syncthreads();
     // acc[h + 2 * im_out]: output rows k3 = 16 h + ..., real / imaginary part
     acc_t acc[4];
@@ -677,14 +688,14 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
             }
         }
     }
-    const int64_t j = j0 + 16 * w + lr;
+    const int j = 64 * c + 16 * w + lr;
     if (j >= NN) return;
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int k3 = 16 * h + MF::orow(lq, v);
-            if (k3 < N) out[(r * N + k3) * NN + j] = cmk<T>(acc[h][v], acc[h + 2][v]);
+            if (k3 < N) *at_b(rout, (unsigned)(k3 * NN + j) * (unsigned)sizeof(cT<T>)) = cmk<T>(acc[h][v], acc[h + 2][v]);
         }
 }
 
@@ -821,14 +832,17 @@ template <class T>
 void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf16x3, hipStream_t st) {
     HPUP_REQUIRE(D.N <= 32 && D.dim == 3, HPUP_ERR_UNSUPPORTED, "axis-3 MFMA kernel needs dim 3, N <= 32");
     const int64_t NN = (int64_t)D.N * D.N;
-    dim3 grid((unsigned)((NN + 63) / 64), (unsigned)D.nrc);
+    HPUP_REQUIRE(D.nrc <= 65535, HPUP_ERR_UNSUPPORTED, "axis-3 kernel: more than 65535 rotation rows");
+    const int nch = (int)((NN + 63) / 64);
     if constexpr (sizeof(T) == 4) {
         if (bf16x3) {
+            dim3 grid((unsigned)nch, (unsigned)D.nrc);
             hipLaunchKernelGGL(k_axis3_bf16x3, grid, dim3(256), 0, st, D, (const float2*)in, (float2*)out);
             return;
         }
     }
     HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 needs FP32 storage");
+    dim3 grid((unsigned)nch, (unsigned)D.nrc);
     hipLaunchKernelGGL(k_axis3_mfma<T>, grid, dim3(256), 0, st, D, (const cT<T>*)in, (cT<T>*)out);
 }
 
