@@ -31,8 +31,8 @@ MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0 / 3}
 # (scaled by the copy-probe factors) for the kernels of this workload; the
 # summaries are under profiles/.  None where not measured.
 MEASURED_TRAFFIC = {
-    ("k_drift_fused", "fp32", "RK4"): 5.176e9,   # profiles/k_drift_fused_B15_N30_RK4.md
-    ("k_drift_fused", "fp64", "RK4"): 9.447e9,
+    ("k_drift_fused", "fp32", "RK4"): 5.081e9,   # profiles/k_drift_fused_B15_N30_RK4.md
+    ("k_drift_fused", "fp64", "RK4"): 9.456e9,
 }
 
 
@@ -137,7 +137,9 @@ def cpu_baseline(plan64, F0, cfg, stages, budget_s=25.0):
     """The reference's dense derivative (oracle, FP64) on a bounded sample of
     output omega points of this workload, extrapolated linearly in the number
     of output points to one derivative evaluation; a step costs `stages` of
-    them."""
+    them.  The oracle spreads output points over its OpenMP threads, so the
+    samples are T and 2T points (T threads): the slope is the cost of T more
+    points, not the wall time of one."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import numpy as np
     import oracle as O
@@ -151,22 +153,23 @@ def cpu_baseline(plan64, F0, cfg, stages, budget_s=25.0):
     cells = O.cg_table(B)
     bt = np.array(cfg["b"])
     G = np.eye(3) * cfg["G"]
-    threads = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
     nx = N ** 3
     samples = []
-    for S in (2, 6):
+    for S in (threads, 2 * threads):
         t0 = time.perf_counter()
         O.derivative(B, 3, N, Ffull, X, OJO, MR, bt, G, L, u, cells, 0, S)
         samples.append((S, time.perf_counter() - t0))
         log("cpu sample", S, "points", "%.2f s" % samples[-1][1])
         if samples[-1][1] > budget_s:
             break
-    if len(samples) == 2:
+    if len(samples) == 2 and samples[1][1] > samples[0][1]:
         (s1, t1), (s2, t2) = samples
         b = (t2 - t1) / (s2 - s1)
         a = max(t1 - b * s1, 0.0)
     else:
-        s1, t1 = samples[0]
+        s1, t1 = samples[-1]
         a, b = 0.0, t1 / s1
     t_eval = a + b * nx
     t_step = stages * t_eval
