@@ -414,25 +414,19 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         const cT<T>* Fb = Fin + base;  // this slab
         const T kc3 = DIM == 3 ? as_const<T>(D.kc)[k3] : T(0), kd3 = DIM == 3 ? as_const<T>(D.kd)[k3] : T(0);
 
-        // the slab, this lane's elements of F and of T3 F: issued together
-        cT<T> own[4], fo[4], Zv[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int idx = tidz + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-            const bool in = a2 < N && a1 < N;
-            const cT<T> x = *at_b(Fb, (in ? (unsigned)(a2 * N + a1) : 0u) * (unsigned)sizeof(cT<T>));
-            own[i] = in ? x : czero<T>();
-        }
+        // this lane's elements of the slab and of T3 F, issued together.  The
+        // lanes' elements tile the 32 x 32 plane exactly once, so they also
+        // fill plane A (zero in the padding)
+        cT<T> fo[4], Zv[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             fo[v] = *at_b(Fb, offb[v]);
             Zv[v] = DIM == 3 ? ld_once<T>(at_b(Z3 + base, offb[v])) : czero<T>();
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++) {  // slab -> plane A, zero padded to 32 x 32
-            const int idx = tidz + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-            pa_re[a2 * kPad + a1] = own[i].x;
-            pa_im[a2 * kPad + a1] = own[i].y;
+        for (int v = 0; v < 4; v++) {
+            pa_re[k2w[v] * kPad + k1w] = ok[v] ? fo[v].x : T(0);
+            pa_im[k2w[v] * kPad + k1w] = ok[v] ? fo[v].y : T(0);
         }
 
         // rotation mixing: sources in batches of MB rows, structurally zero
