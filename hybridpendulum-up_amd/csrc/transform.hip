@@ -462,6 +462,185 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
     }
 }
 
+// ------------------------------------------- (alpha, gamma) stages, 2B = 30 --
+//
+// At 2B = 30 the two angular DFTs of each (beta_j, omega) tile are length-30
+// DFTs of a whole row or column, done as Good-Thomas prime-factor FFTs
+// (30 = 2 * 3 * 5, coprime factors, no twiddle factors): x[n] is read at
+// n = 15 n1 + 10 n2 + 6 n3 (mod 30) as a 2 x 3 x 5 array, transformed along
+// each axis, and X[k] written back through the same map (15, 10 and 6 are
+// also the CRT idempotents of 2, 3 and 5).  DIR = -1: e^{-2 pi i n k / 30},
+// DIR = +1: e^{+2 pi i n k / 30}.  About 20x fewer operations than the
+// direct DFTs, so the stages become bound by their G and f traffic.
+
+template <class T, int DIR> __device__ __forceinline__ cT<T> irot(cT<T> a) {  // DIR i a
+    return DIR > 0 ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
+}
+
+template <class T, int DIR> __device__ __forceinline__ void dft3(cT<T>& x0, cT<T>& x1, cT<T>& x2) {
+    const T h = T(0.86602540378443864676);  // sin(2 pi / 3)
+    const cT<T> t = cadd<T>(x1, x2), d = csub<T>(x1, x2);
+    const cT<T> m = cmk<T>(x0.x - T(0.5) * t.x, x0.y - T(0.5) * t.y);
+    const cT<T> r = irot<T, DIR>(cscale<T>(d, h));
+    x0 = cadd<T>(x0, t);
+    x1 = cadd<T>(m, r);
+    x2 = csub<T>(m, r);
+}
+
+template <class T, int DIR> __device__ __forceinline__ void dft5(cT<T>* x) {
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);  // cos(2 pi / 5), cos(4 pi / 5)
+    const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);   // sin(2 pi / 5), sin(4 pi / 5)
+    const cT<T> a1 = cadd<T>(x[1], x[4]), b1 = csub<T>(x[1], x[4]);
+    const cT<T> a2 = cadd<T>(x[2], x[3]), b2 = csub<T>(x[2], x[3]);
+    const cT<T> p1 = cmk<T>(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
+    const cT<T> p2 = cmk<T>(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
+    const cT<T> q1 = irot<T, DIR>(cmk<T>(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    const cT<T> q2 = irot<T, DIR>(cmk<T>(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    x[0] = cadd<T>(x[0], cadd<T>(a1, a2));
+    x[1] = cadd<T>(p1, q1);
+    x[4] = csub<T>(p1, q1);
+    x[2] = cadd<T>(p2, q2);
+    x[3] = csub<T>(p2, q2);
+}
+
+template <class T, int DIR> __device__ __forceinline__ void fft30(cT<T>* x) {
+    cT<T> a[2][3][5];
+#pragma unroll
+    for (int n1 = 0; n1 < 2; n1++)
+#pragma unroll
+        for (int n2 = 0; n2 < 3; n2++)
+#pragma unroll
+            for (int n3 = 0; n3 < 5; n3++) a[n1][n2][n3] = x[(15 * n1 + 10 * n2 + 6 * n3) % 30];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; n2++)
+#pragma unroll
+        for (int n3 = 0; n3 < 5; n3++) {
+            const cT<T> u = a[0][n2][n3], v = a[1][n2][n3];
+            a[0][n2][n3] = cadd<T>(u, v);
+            a[1][n2][n3] = csub<T>(u, v);
+        }
+#pragma unroll
+    for (int n1 = 0; n1 < 2; n1++)
+#pragma unroll
+        for (int n3 = 0; n3 < 5; n3++) dft3<T, DIR>(a[n1][0][n3], a[n1][1][n3], a[n1][2][n3]);
+#pragma unroll
+    for (int n1 = 0; n1 < 2; n1++)
+#pragma unroll
+        for (int n2 = 0; n2 < 3; n2++) dft5<T, DIR>(a[n1][n2]);
+#pragma unroll
+    for (int k1 = 0; k1 < 2; k1++)
+#pragma unroll
+        for (int k2 = 0; k2 < 3; k2++)
+#pragma unroll
+            for (int k3 = 0; k3 < 5; k3++) x[(15 * k1 + 10 * k2 + 6 * k3) % 30] = a[k1][k2][k3];
+}
+
+// Inverse (alpha, gamma) stage at B = 15: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}
+// (item (m, xl): its G row, frequency n at index n mod 30), then
+// f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a} (item (g, xl): its H column).
+template <class T, int XC>
+__global__ __launch_bounds__(256) void k_inv_ag_fft30(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
+                                                      int nxc) {
+    extern __shared__ char smem_raw[];
+    constexpr int B = 15, n2 = 30, M = 29;
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl]
+    const int w = xcd_major(blockIdx.x, gridDim.x);
+    const int j = w / nxc;
+    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    const int64_t nx = X.nx;
+    for (int task = threadIdx.x; task < M * XC; task += 256) {
+        const int mi = task / XC, xl = task % XC;
+        const int64_t x = x0 + xl;
+        const bool ld = x < nx;
+        // clamped, branch-free loads; a column beyond nx is zeroed after the load
+        const cT<T>* Gp = G + (int64_t)mi * n2 * nx + (int64_t)j * nx + min<int64_t>(x, nx - 1);
+        cT<T> v[n2];
+#pragma unroll
+        for (int ni = 0; ni < M; ni++) {
+            const cT<T> g = Gp[(int64_t)M * ni * n2 * nx];
+            v[(ni - (B - 1) + n2) % n2] = ld ? g : czero<T>();
+        }
+        v[B] = czero<T>();
+        fft30<T, -1>(v);
+#pragma unroll
+        for (int g = 0; g < n2; g++) sH[(mi * n2 + g) * XC + xl] = v[g];
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < n2 * XC; task += 256) {
+        const int g = task / XC, xl = task % XC;
+        const int64_t x = x0 + xl;
+        cT<T> v[n2];
+#pragma unroll
+        for (int mi = 0; mi < M; mi++) v[(mi - (B - 1) + n2) % n2] = sH[(mi * n2 + g) * XC + xl];
+        v[B] = czero<T>();
+        fft30<T, -1>(v);
+        if (x < nx) {
+            T* fo = f + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x));
+#pragma unroll
+            for (int a = 0; a < n2; a++) fo[a] = v[a].x;
+        }
+    }
+}
+
+// Forward (alpha, gamma) stage at B = 15: H[m][g] = sum_a f[a][g] e^{+i m alpha_a},
+// m >= 0 (item (g, xl): its f row), then G[m][n] = sum_g H[m][g] e^{+i n gamma_g}
+// and G[-m][-n] = conj (item (m, xl): its H row).
+template <class T, int XC>
+__global__ __launch_bounds__(256) void k_fwd_ag_fft30(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
+                                                      int wstride, cT<T>* __restrict__ G, int nxc) {
+    extern __shared__ char smem_raw[];
+    constexpr int B = 15, n2 = 30, lmax = 14, M = 29, rp = n2 + 1;
+    T* sf = reinterpret_cast<T*>(smem_raw);          // [xl][g][a]  XC*n2*rp
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m>=0][g][xl], aliases sf
+    const int w = xcd_major(blockIdx.x, gridDim.x);
+    const int j = w / nxc;
+    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    const int64_t nx = X.nx;
+    const int tid = threadIdx.x;
+    const int64_t n2c = (int64_t)n2 * n2 * n2;
+    for (int i = tid; i < XC * n2 * n2; i += 256) {
+        const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
+        const int64_t x = x0 + xl;
+        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
+        T v = x < nx ? f[iR + n2c * x] : T(0);
+        if (wR) v *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
+        sf[(xl * n2 + g) * rp + a] = v;
+    }
+    __syncthreads();
+    constexpr int TA = n2 * XC;
+    static_assert(TA <= 256, "one alpha row per thread");
+    cT<T> v[n2];
+    const bool actA = tid < TA;
+    const int gA = tid / XC, xlA = tid % XC;
+    if (actA) {
+#pragma unroll
+        for (int a = 0; a < n2; a++) v[a] = cmk<T>(sf[(xlA * n2 + gA) * rp + a], T(0));
+    }
+    __syncthreads();  // sH overwrites sf
+    if (actA) {
+        fft30<T, 1>(v);
+#pragma unroll
+        for (int m = 0; m < B; m++) sH[(m * n2 + gA) * XC + xlA] = v[m];
+    }
+    __syncthreads();
+    for (int task = tid; task < B * XC; task += 256) {
+        const int m = task / XC, xl = task % XC;
+        const int64_t x = x0 + xl;
+#pragma unroll
+        for (int g = 0; g < n2; g++) v[g] = sH[(m * n2 + g) * XC + xl];
+        fft30<T, 1>(v);
+        if (x < nx) {
+            cT<T>* Gx = G + (int64_t)j * nx + x;
+#pragma unroll
+            for (int ni = 0; ni < M; ni++) {
+                const cT<T> z = v[(ni - lmax + n2) % n2];
+                Gx[((int64_t)(m + lmax) + M * ni) * n2 * nx] = z;
+                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * nx] = cconj<T>(z);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------- launchers -----
 
 static int pick_xc(size_t per_x, size_t budget, int64_t nx) {
@@ -478,6 +657,16 @@ template <class T> constexpr int inv_xc() { return sizeof(T) == 8 ? 4 : 8; }
 template <class T>
 void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
     const int n2 = 2 * X.B;
+    if (n2 == 30) {
+        constexpr int XC = kFwdXC;
+        const int nxc = (int)((X.nx + XC - 1) / XC);
+        HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+        const size_t sh = std::max(sizeof(T) * XC * n2 * (n2 + 1), sizeof(cT<T>) * X.B * n2 * XC);
+        allow_lds((const void*)k_fwd_ag_fft30<T, XC>, sh);
+        hipLaunchKernelGGL((k_fwd_ag_fft30<T, XC>), dim3((unsigned)(nxc * n2)), dim3(256), sh, st, X, (const T*)f,
+                           (const T*)wR, wstride, (cT<T>*)G, nxc);
+        return;
+    }
     if (n2 <= kAgMax) {
         constexpr int XC = kFwdXC;
         const int nxc = (int)((X.nx + XC - 1) / XC);
@@ -515,6 +704,16 @@ void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t s
 template <class T>
 void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
     const int n2 = 2 * X.B, M = 2 * X.B - 1;
+    if (n2 == 30) {
+        constexpr int XC = inv_xc<T>();
+        const int nxc = (int)((X.nx + XC - 1) / XC);
+        HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+        const size_t sh = sizeof(cT<T>) * M * n2 * XC;
+        allow_lds((const void*)k_inv_ag_fft30<T, XC>, sh);
+        hipLaunchKernelGGL((k_inv_ag_fft30<T, XC>), dim3((unsigned)(nxc * n2)), dim3(256), sh, st, X,
+                           (const cT<T>*)G, (T*)f, nxc);
+        return;
+    }
     if (n2 <= kAgMax) {
         constexpr int XC = inv_xc<T>();
         const int nxc = (int)((X.nx + XC - 1) / XC);

@@ -15,7 +15,7 @@ def _print(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.parametrize("B,dim,N", [(4, 0, 1), (5, 2, 4), (3, 3, 4), (10, 0, 1)])
+@pytest.mark.parametrize("B,dim,N", [(4, 0, 1), (5, 2, 4), (3, 3, 4), (10, 0, 1), (15, 0, 1), (15, 2, 4)])
 def test_fft_forward_backward(B, dim, N):
     _print("case fft", B, dim, N)
     L = 6.0
@@ -230,3 +230,27 @@ def test_plan_transforms_roundtrip():
     assert rel_l2(p.download_full(F2), Ffull) < 1e-12
     f_ref = O.fft_backward(B, Ffull, d, 3, N)
     assert rel_l2(g.cpu().numpy(), f_ref.ravel(order="F")) < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+def test_plan_transforms_b15(precision, tol):
+    """Plan forward and backward at B = 15 (2B = 30: the prime-factor FFT form of
+    the (alpha, gamma) stages) against the oracle's transforms."""
+    import torch
+    _print("case plan transforms B=15", precision)
+    B, N, L = 15, 4, 6.0
+    f = density(B, 2, N, L, (6, 3, 1), seed=11)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    Ffull = O.fft_forward(B, f, d, w, 2, N)
+    f_ref = O.fft_backward(B, Ffull, d, 2, N)
+    p = hpup.Plan(B, 2, N, L, precision=precision)
+    F = p.upload_full(Ffull)
+    g = p.backward(F)
+    torch.cuda.synchronize()
+    assert rel_l2(g.cpu().numpy().astype(np.float64), f_ref.ravel(order="F")) < tol
+    fdev = torch.from_numpy(np.asfortranarray(f).ravel(order="F").astype(np.float64 if precision == "fp64" else np.float32)).cuda()
+    F2 = p.forward(fdev)
+    torch.cuda.synchronize()
+    mask = np.broadcast_to(valid_mask(B, (N, N)), Ffull.shape)
+    assert rel_l2(p.download_full(F2)[mask], Ffull[mask]) < tol
