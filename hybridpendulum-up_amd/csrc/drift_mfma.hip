@@ -699,19 +699,26 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
     constexpr int CP = 65;
     __shared__ float sre[32 * CP], sim[32 * CP];
     __shared__ CircTab<float> tab;
-    const int N = D.N;
-    const int64_t NN = (int64_t)N * N;
-    const int64_t r = blockIdx.y;
-    const int64_t j0 = (int64_t)blockIdx.x * 64;
+    const int N = D.N, NN = N * N;
+    const int r = blockIdx.y, c = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
-    for (int idx = tid; idx < 32 * 64; idx += 256) {
-        const int a = idx >> 6, jj = idx & 63;
-        const int64_t j = j0 + jj;
-        const float2 v = (a < N && j < NN) ? in[(r * N + a) * NN + j] : czero<float>();
-        sre[a * CP + jj] = v.x;
-        sim[a * CP + jj] = v.y;
+    const float2* rin = in + (int64_t)r * N * NN;
+    float2* rout = out + (int64_t)r * N * NN;
+    float2 x[8];  // unconditional clamped loads, as in k_axis3_mfma
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int a = w + 4 * i, j = 64 * c + lane;
+        const bool ok = a < N && j < NN;
+        const float2 v = *at_b(rin, ok ? (unsigned)(a * NN + j) * (unsigned)sizeof(float2) : 0u);
+        x[i] = ok ? v : czero<float>();
     }
     tab.fill(D.kap + 2 * N, N, tid);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int a = w + 4 * i;
+        sre[a * CP + lane] = x[i].x;
+        sim[a * CP + lane] = x[i].y;
+    }
     __syncthreads();
     f32x4 acc[4];
 #pragma unroll
@@ -740,14 +747,14 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
             acc[it] = mfma_bf16(ah, bh, acc[it]);
         }
     }
-    const int64_t j = j0 + 16 * w + lr;
+    const int j = 64 * c + 16 * w + lr;
     if (j >= NN) return;
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int k3 = 16 * h + 4 * lq + v;
-            if (k3 < N) out[(r * N + k3) * NN + j] = cmk<float>(acc[h][v], acc[h + 2][v]);
+            if (k3 < N) *at_b(rout, (unsigned)(k3 * NN + j) * (unsigned)sizeof(float2)) = cmk<float>(acc[h][v], acc[h + 2][v]);
         }
 }
 
