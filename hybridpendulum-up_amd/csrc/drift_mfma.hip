@@ -278,10 +278,9 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
     return x * q + (x < rem ? x : rem) + i;
 }
 
-// Data the fused kernel touches once (T3 F, F0, ACC, the output).  FP64
-// streams them non-temporally, which keeps the re-read neighbour rows in L2;
-// for FP32 the plain forms measured faster (the non-temporal forms cost
-// registers at the kernel's 4-waves budget).
+// Data the kernels touch once (T3 F, F0, ACC, the output; F and T3 F in the
+// axis-3 kernel).  FP64 streams them non-temporally, which keeps the re-read
+// neighbour rows in L2; for FP32 the plain forms measured faster.
 template <class T> __device__ __forceinline__ cT<T> ld_once(const cT<T>* p) {
     if constexpr (sizeof(T) == 8) return ld_stream<T>(p);
     else return *p;
@@ -543,9 +542,11 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         gm.t1(pb_re, pb_im, Q1, D.c0[0]);
         gm.t2(pc_re, pc_im, Q2, D.c0[1]);
 
-        // epilogue: the RK stage streams F0 and ACC are loaded for all four
-        // elements at once (padded elements read their clamped in-range
-        // address), so their latency is paid once; stores are masked per element
+        // epilogue: the stage's streams F0 (Horner stages 2-4 and the stage
+        // sums) and ACC (stage sums of density-form plans only) are loaded for
+        // all four elements at once (padded elements read their clamped
+        // in-range address), so their latency is paid once; stores are masked
+        // per element
         cT<T> q[4], f0v[4], acv[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) q[v] = cadd<T>(cadd<T>(Q1[v], Q2[v]), S[v]);
@@ -554,11 +555,10 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             for (int v = 0; v < 4; v++)
                 if (ok[v]) st_once<T>(at_b(Fout + base, offb[v]), q[v]);
         } else {
-            // RK stages read F0 and ACC here (a stage without F0 takes the
-            // slab itself): the loads of the four elements are issued
-            // together, under workgroup-uniform conditions only, and every
-            // loaded value is consumed.  These streams and the outputs are
-            // touched once: non-temporal.
+            // a stage without F0 takes the slab itself; the loads of the four
+            // elements are issued together, under workgroup-uniform conditions
+            // only, and every loaded value is consumed.  These streams and the
+            // outputs are touched once (non-temporal for FP64).
             if (sc.read_f0) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) f0v[v] = ld_once<T>(at_b(F0 + base, offb[v]));
