@@ -318,7 +318,7 @@ template <class T> struct PlanT : PlanBase {
     // forward transform of the grid product f(R, x) wR[R * wstride]
     void forward_weighted(const void* f, const T* wR, int wstride, void* F, hipStream_t st) {
         const int M = 2 * model.B - 1;
-        ensure(G, sizeof(cT<T>) * (size_t)M * M * 2 * model.B * nx);
+        ensure(G, sizeof(cT<T>) * (size_t)g_elems(model.B, nx));
         launch_fwd_ag<T>(xf.view, f, wR, wstride, G.p, st);
         launch_beta_fwd<T>(xf.view, G.p, F, st);
         launch_omega_dft<T>(xf.view, F, nullptr, false, st);
@@ -326,7 +326,7 @@ template <class T> struct PlanT : PlanBase {
     }
     void backward(const void* F, void* f, hipStream_t st) override {
         const int M = 2 * model.B - 1;
-        ensure(G, sizeof(cT<T>) * (size_t)M * M * 2 * model.B * nx);
+        ensure(G, sizeof(cT<T>) * (size_t)g_elems(model.B, nx));
         ensure(tmp, cbytes());
         HPUP_HIP(hipMemcpyAsync(tmp.p, F, cbytes(), hipMemcpyDeviceToDevice, st));
         launch_omega_dft<T>(xf.view, tmp.p, nullptr, true, st);

@@ -106,6 +106,12 @@ template <class T> struct XformDev {
     const cT<T>* bsI;   // 96 entries each (launch_slab_dft_mfma)
 };
 
+// Elements of the angular/beta intermediate G: (2B-1)^2 2B rows of the Omega
+// points rounded up to whole blocks of kGX (its layout is Omega-blocked)
+constexpr int kGX = 64;
+inline int64_t g_elems(int B, int64_t nx) {
+    return (int64_t)(2 * B - 1) * (2 * B - 1) * 2 * B * ((nx + kGX - 1) / kGX) * kGX;
+}
 template <class T>
 // wR (nullable): transform the product f(R, x) wR[R * wstride] (grid weight)
 void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st);

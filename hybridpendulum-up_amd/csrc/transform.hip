@@ -4,7 +4,7 @@
 //   1. k_fwd_ag   per (x-chunk, beta_j): real 2-D DFT over (alpha, gamma) with
 //                 the flip(fftshift) sign convention (frequency -m stored at m),
 //                 Hermitian half computed, the other half by conjugation
-//                 -> G[(m,n)][j][x]
+//                 -> G[(m,n)][j][x] (stored Omega-blocked, see g_off)
 //   2. k_beta_fwd per (m,n): F^l_mn(x) = sum_j w_j d^l_mn(beta_j) G[(m,n)][j][x]
 //   3. omega DFT of every compact row (slab kernel on axes 1,2; axis kernel on 3)
 // Inverse: the same stages reversed (1/nx on omega, unnormalized over alpha,
@@ -15,6 +15,18 @@
 #include "kernels_common.h"
 
 namespace hpup {
+
+// Offset of G[(m,n)][j][x] (mn = m + lmax + M (n + lmax)) in the intermediate
+// buffer: Omega points in blocks of kGX, each block holding [(m,n)][j][x % kGX].
+// A (beta_j, Omega chunk) tile of the angular stages then spans one block
+// (tens of KB) instead of one page per (m, n) (the plain [(m,n)][j][x] order
+// put its 841 rows 6.5 MB apart at B = 15, 30^3, and the address translations
+// dominated), while the beta stages still read and write runs of kGX
+// consecutive Omega points.
+__device__ __forceinline__ int64_t g_off(int B, int mn, int j, int64_t x) {
+    const int64_t M = 2 * B - 1, n2 = 2 * B;
+    return ((x / kGX) * (M * M * n2) + (int64_t)mn * n2 + j) * kGX + (x % kGX);
+}
 
 // ---------------------------------------------------------- forward 2-D --
 
@@ -70,8 +82,8 @@ __global__ __launch_bounds__(256) void k_fwd_ag(XformDev<T> X, const T* __restri
             t += step;
             if (t >= n2) t -= n2;
         }
-        G[((int64_t)(m + lmax) + M * (n + lmax)) * n2 * nx + (int64_t)j * nx + x] = acc;
-        if (m > 0) G[((int64_t)(-m + lmax) + M * (-n + lmax)) * n2 * nx + (int64_t)j * nx + x] = cconj<T>(acc);
+        G[g_off(B, (m + lmax) + M * (n + lmax), j, x)] = acc;
+        if (m > 0) G[g_off(B, (-m + lmax) + M * (-n + lmax), j, x)] = cconj<T>(acc);
     }
 }
 
@@ -87,13 +99,13 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
     __syncthreads();
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= X.nx) return;
-    const cT<T>* Gp = G + (int64_t)mn * n2 * X.nx + x;
+    const cT<T>* Gp = G + g_off(B, mn, 0, x);
     for (int l0 = lmin; l0 <= lmax; l0 += 8) {
         cT<T> acc[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) acc[t] = czero<T>();
         for (int j = 0; j < n2; j++) {
-            const cT<T> g = Gp[(int64_t)j * X.nx];
+            const cT<T> g = Gp[(int64_t)j * kGX];
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 if (l0 + t > lmax) break;
@@ -124,7 +136,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
     __syncthreads();
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= X.nx) return;
-    cT<T>* Gp = G + (int64_t)mn * n2 * X.nx + x;
+    cT<T>* Gp = G + g_off(B, mn, 0, x);
     for (int j0 = 0; j0 < n2; j0 += 8) {
         cT<T> acc[8];
 #pragma unroll
@@ -142,7 +154,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             if (j0 + t >= n2) break;
-            Gp[(int64_t)(j0 + t) * X.nx] = acc[t];
+            Gp[(int64_t)(j0 + t) * kGX] = acc[t];
         }
     }
 }
@@ -173,9 +185,9 @@ __global__ __launch_bounds__(256) void k_inv_ag(XformDev<T> X, const cT<T>* __re
 #pragma unroll
         for (int k = 0; k < GC; k++) acc[k] = czero<T>();
         if (x < nx) {
-            const cT<T>* Gp = G + (int64_t)mi * n2 * nx + (int64_t)j * nx + x;
+            const cT<T>* Gp = G + g_off(B, mi, j, x);
             for (int ni = 0; ni < M; ni++) {
-                const cT<T> gv = Gp[(int64_t)M * ni * n2 * nx];
+                const cT<T> gv = Gp[(int64_t)M * ni * n2 * kGX];
                 const int n = ni - lmax;
                 const int dn = ((n % n2) + n2) % n2;  // (n g) mod n2 advances by dn per gamma
                 int t = (int)(((int64_t)dn * g0) % n2);
@@ -386,15 +398,15 @@ __global__ __launch_bounds__(256) void k_fwd_ag_r(XformDev<T> X, const T* __rest
 #pragma unroll
         for (int g = 0; g < kAgMax; g++) h[g] = (act && g < n2) ? sH[(m * n2 + g) * XC + xl] : czero<T>();
         const bool st = act && x < nx;
-        cT<T>* Gx = G + (int64_t)j * nx + min<int64_t>(x, nx - 1);
+        cT<T>* Gx = G + g_off(B, 0, j, min<int64_t>(x, nx - 1));
         for (int ni = part; ni < M; ni += SB) {
             const cTab<T> e = E + 2 * ni * kAgMax;
             cT<T> acc = czero<T>();
 #pragma unroll
             for (int g = 0; g < kAgMax; g++) cfma<T>(acc, h[g], etw<T>(e, g));
             if (st) {
-                Gx[((int64_t)(m + lmax) + M * ni) * n2 * nx] = acc;
-                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * nx] = cconj<T>(acc);
+                Gx[((int64_t)(m + lmax) + M * ni) * n2 * kGX] = acc;
+                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * kGX] = cconj<T>(acc);
             }
         }
     }
@@ -421,11 +433,11 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
         const int64_t x = x0 + xl;
         const bool ld = act && x < nx;
         // clamped, branch-free loads; invalid entries are zeroed after the load
-        const cT<T>* Gp = G + (int64_t)min(mi, M - 1) * n2 * nx + (int64_t)j * nx + min<int64_t>(x, nx - 1);
+        const cT<T>* Gp = G + g_off(B, min(mi, M - 1), j, min<int64_t>(x, nx - 1));
         cT<T> gr[kAgMax];
 #pragma unroll
         for (int ni = 0; ni < kAgMax; ni++) {
-            const cT<T> v = Gp[(int64_t)M * min(ni, M - 1) * n2 * nx];
+            const cT<T> v = Gp[(int64_t)M * min(ni, M - 1) * n2 * kGX];
             gr[ni] = (ld && ni < M) ? v : czero<T>();
         }
         for (int g = part; g < n2; g += S1) {
@@ -538,40 +550,46 @@ template <class T, int DIR> __device__ __forceinline__ void fft30(cT<T>* x) {
 // Inverse (alpha, gamma) stage at B = 15: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}
 // (item (m, xl): its G row, frequency n at index n mod 30), then
 // f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a} (item (g, xl): its H column).
-template <class T, int XC>
-__global__ __launch_bounds__(256) void k_inv_ag_fft30(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
-                                                      int nxc) {
+// Row stride (elements) of the FFT kernels' H tile [m][g][xl]: padded so that
+// the lanes of a wave, which write or read one g of eight (FP32) or four
+// (FP64) rows m at once, spread over all LDS banks (2-way for 8-byte and
+// 4-way for 16-byte elements, the minimum for a wave's 512 or 1024 bytes)
+template <class T> __host__ __device__ constexpr int fft_sh(int XC) { return 30 * XC + (sizeof(T) == 4 ? 8 : 4); }
+
+template <class T, int XC, int NT>
+__global__ __launch_bounds__(NT) void k_inv_ag_fft30(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
+                                                     int nxc) {
     extern __shared__ char smem_raw[];
-    constexpr int B = 15, n2 = 30, M = 29;
-    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl]
+    constexpr int B = 15, n2 = 30, M = 29, SH = fft_sh<T>(XC);
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl], row stride SH
     const int w = xcd_major(blockIdx.x, gridDim.x);
     const int j = w / nxc;
     const int64_t x0 = (int64_t)(w % nxc) * XC;
     const int64_t nx = X.nx;
-    for (int task = threadIdx.x; task < M * XC; task += 256) {
+    for (int task = threadIdx.x; task < M * XC; task += NT) {
         const int mi = task / XC, xl = task % XC;
         const int64_t x = x0 + xl;
         const bool ld = x < nx;
         // clamped, branch-free loads; a column beyond nx is zeroed after the load
-        const cT<T>* Gp = G + (int64_t)mi * n2 * nx + (int64_t)j * nx + min<int64_t>(x, nx - 1);
+        const cT<T>* Gp = G + g_off(B, mi, j, min<int64_t>(x, nx - 1));
         cT<T> v[n2];
 #pragma unroll
         for (int ni = 0; ni < M; ni++) {
-            const cT<T> g = Gp[(int64_t)M * ni * n2 * nx];
+            const cT<T> g = Gp[(int64_t)M * ni * n2 * kGX];
             v[(ni - (B - 1) + n2) % n2] = ld ? g : czero<T>();
         }
         v[B] = czero<T>();
         fft30<T, -1>(v);
 #pragma unroll
-        for (int g = 0; g < n2; g++) sH[(mi * n2 + g) * XC + xl] = v[g];
+        for (int g = 0; g < n2; g++) sH[mi * SH + g * XC + xl] = v[g];
     }
     __syncthreads();
-    for (int task = threadIdx.x; task < n2 * XC; task += 256) {
+    for (int task = threadIdx.x; task < n2 * XC; task += NT) {
         const int g = task / XC, xl = task % XC;
         const int64_t x = x0 + xl;
         cT<T> v[n2];
 #pragma unroll
-        for (int mi = 0; mi < M; mi++) v[(mi - (B - 1) + n2) % n2] = sH[(mi * n2 + g) * XC + xl];
+        for (int mi = 0; mi < M; mi++) v[(mi - (B - 1) + n2) % n2] = sH[mi * SH + g * XC + xl];
         v[B] = czero<T>();
         fft30<T, -1>(v);
         if (x < nx) {
@@ -585,20 +603,20 @@ __global__ __launch_bounds__(256) void k_inv_ag_fft30(XformDev<T> X, const cT<T>
 // Forward (alpha, gamma) stage at B = 15: H[m][g] = sum_a f[a][g] e^{+i m alpha_a},
 // m >= 0 (item (g, xl): its f row), then G[m][n] = sum_g H[m][g] e^{+i n gamma_g}
 // and G[-m][-n] = conj (item (m, xl): its H row).
-template <class T, int XC>
-__global__ __launch_bounds__(256) void k_fwd_ag_fft30(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
-                                                      int wstride, cT<T>* __restrict__ G, int nxc) {
+template <class T, int XC, int NT>
+__global__ __launch_bounds__(NT) void k_fwd_ag_fft30(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
+                                                     int wstride, cT<T>* __restrict__ G, int nxc) {
     extern __shared__ char smem_raw[];
-    constexpr int B = 15, n2 = 30, lmax = 14, M = 29, rp = n2 + 1;
+    constexpr int B = 15, n2 = 30, lmax = 14, M = 29, rp = n2 + 1, SH = fft_sh<T>(XC);
     T* sf = reinterpret_cast<T*>(smem_raw);          // [xl][g][a]  XC*n2*rp
-    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m>=0][g][xl], aliases sf
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m>=0][g][xl], row stride SH, aliases sf
     const int w = xcd_major(blockIdx.x, gridDim.x);
     const int j = w / nxc;
     const int64_t x0 = (int64_t)(w % nxc) * XC;
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
     const int64_t n2c = (int64_t)n2 * n2 * n2;
-    for (int i = tid; i < XC * n2 * n2; i += 256) {
+    for (int i = tid; i < XC * n2 * n2; i += NT) {
         const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
         const int64_t x = x0 + xl;
         const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
@@ -608,7 +626,7 @@ __global__ __launch_bounds__(256) void k_fwd_ag_fft30(XformDev<T> X, const T* __
     }
     __syncthreads();
     constexpr int TA = n2 * XC;
-    static_assert(TA <= 256, "one alpha row per thread");
+    static_assert(TA <= NT, "one alpha row per thread");
     cT<T> v[n2];
     const bool actA = tid < TA;
     const int gA = tid / XC, xlA = tid % XC;
@@ -620,22 +638,22 @@ __global__ __launch_bounds__(256) void k_fwd_ag_fft30(XformDev<T> X, const T* __
     if (actA) {
         fft30<T, 1>(v);
 #pragma unroll
-        for (int m = 0; m < B; m++) sH[(m * n2 + gA) * XC + xlA] = v[m];
+        for (int m = 0; m < B; m++) sH[m * SH + gA * XC + xlA] = v[m];
     }
     __syncthreads();
-    for (int task = tid; task < B * XC; task += 256) {
+    for (int task = tid; task < B * XC; task += NT) {
         const int m = task / XC, xl = task % XC;
         const int64_t x = x0 + xl;
 #pragma unroll
-        for (int g = 0; g < n2; g++) v[g] = sH[(m * n2 + g) * XC + xl];
+        for (int g = 0; g < n2; g++) v[g] = sH[m * SH + g * XC + xl];
         fft30<T, 1>(v);
         if (x < nx) {
-            cT<T>* Gx = G + (int64_t)j * nx + x;
+            cT<T>* Gx = G + g_off(B, 0, j, x);
 #pragma unroll
             for (int ni = 0; ni < M; ni++) {
                 const cT<T> z = v[(ni - lmax + n2) % n2];
-                Gx[((int64_t)(m + lmax) + M * ni) * n2 * nx] = z;
-                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * nx] = cconj<T>(z);
+                Gx[((int64_t)(m + lmax) + M * ni) * n2 * kGX] = z;
+                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * kGX] = cconj<T>(z);
             }
         }
     }
@@ -650,6 +668,19 @@ static int pick_xc(size_t per_x, size_t budget, int64_t nx) {
     return std::max(xc, 1);
 }
 
+// omega points per block and block size of the 2B = 30 FFT kernels (the
+// inverse's XC doubles for FP32)
+#ifndef HPUP_FFT_INV_XC
+#define HPUP_FFT_INV_XC 4
+#endif
+#ifndef HPUP_FFT_FWD_XC
+#define HPUP_FFT_FWD_XC 8
+#endif
+#ifndef HPUP_FFT_NT
+#define HPUP_FFT_NT 256
+#endif
+constexpr int kFftInvXC = HPUP_FFT_INV_XC, kFftFwdXC = HPUP_FFT_FWD_XC, kFftNT = HPUP_FFT_NT;
+
 // omega points per block of the register-row (alpha, gamma) kernels
 constexpr int kFwdXC = 8;
 template <class T> constexpr int inv_xc() { return sizeof(T) == 8 ? 4 : 8; }
@@ -658,12 +689,12 @@ template <class T>
 void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
     const int n2 = 2 * X.B;
     if (n2 == 30) {
-        constexpr int XC = kFwdXC;
+        constexpr int XC = kFftFwdXC, NT = kFftNT;
         const int nxc = (int)((X.nx + XC - 1) / XC);
         HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
-        const size_t sh = std::max(sizeof(T) * XC * n2 * (n2 + 1), sizeof(cT<T>) * X.B * n2 * XC);
-        allow_lds((const void*)k_fwd_ag_fft30<T, XC>, sh);
-        hipLaunchKernelGGL((k_fwd_ag_fft30<T, XC>), dim3((unsigned)(nxc * n2)), dim3(256), sh, st, X, (const T*)f,
+        const size_t sh = std::max(sizeof(T) * XC * n2 * (n2 + 1), sizeof(cT<T>) * X.B * fft_sh<T>(XC));
+        allow_lds((const void*)k_fwd_ag_fft30<T, XC, NT>, sh);
+        hipLaunchKernelGGL((k_fwd_ag_fft30<T, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X, (const T*)f,
                            (const T*)wR, wstride, (cT<T>*)G, nxc);
         return;
     }
@@ -705,12 +736,12 @@ template <class T>
 void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
     const int n2 = 2 * X.B, M = 2 * X.B - 1;
     if (n2 == 30) {
-        constexpr int XC = inv_xc<T>();
+        constexpr int XC = kFftInvXC * (sizeof(T) == 8 ? 1 : 2), NT = kFftNT;
         const int nxc = (int)((X.nx + XC - 1) / XC);
         HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
-        const size_t sh = sizeof(cT<T>) * M * n2 * XC;
-        allow_lds((const void*)k_inv_ag_fft30<T, XC>, sh);
-        hipLaunchKernelGGL((k_inv_ag_fft30<T, XC>), dim3((unsigned)(nxc * n2)), dim3(256), sh, st, X,
+        const size_t sh = sizeof(cT<T>) * M * fft_sh<T>(XC);
+        allow_lds((const void*)k_inv_ag_fft30<T, XC, NT>, sh);
+        hipLaunchKernelGGL((k_inv_ag_fft30<T, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X,
                            (const cT<T>*)G, (T*)f, nxc);
         return;
     }
