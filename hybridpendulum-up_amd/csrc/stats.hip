@@ -23,6 +23,24 @@ constexpr int kStatXU = 4;                        // omega points per load batch
 // grid: (rotation tiles, omega chunks).  Writes per-chunk partials of fR, gR
 // and per-wave partials of fx:
 //   fRp[c][iR], gRp[c][a][iR], fxp[tile*4 + wave][ix]
+static_assert(kStatRT == 4, "ld4 loads four rotation points");
+// four consecutive values as FP64 (zero when !ok; p then points at f[0])
+template <class T> __device__ __forceinline__ void ld4(const T* p, bool ok, double* v) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = ok ? (double)q.x : 0.0;
+        v[1] = ok ? (double)q.y : 0.0;
+        v[2] = ok ? (double)q.z : 0.0;
+        v[3] = ok ? (double)q.w : 0.0;
+    } else {
+        const double2 q0 = reinterpret_cast<const double2*>(p)[0], q1 = reinterpret_cast<const double2*>(p)[1];
+        v[0] = ok ? q0.x : 0.0;
+        v[1] = ok ? q0.y : 0.0;
+        v[2] = ok ? q1.x : 0.0;
+        v[3] = ok ? q1.y : 0.0;
+    }
+}
+
 template <class T, int DIM>
 __global__ __launch_bounds__(kStatThreads) void k_stat_pass(const T* __restrict__ f, int64_t nR, int64_t nx,
                                                             int n2, int64_t xper, const double* __restrict__ w,
@@ -31,12 +49,17 @@ __global__ __launch_bounds__(kStatThreads) void k_stat_pass(const T* __restrict_
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t tile = blockIdx.x, c = blockIdx.y;
     const int64_t x0 = c * xper, x1 = x0 + xper < nx ? x0 + xper : nx;
+    // this thread's kStatRT consecutive rotation points (nR = (2B)^3 is a
+    // multiple of kStatRT, so a group is wholly in range or out): one
+    // 4-element vector load per omega point, a wave's loads contiguous
+    const int64_t R0 = tile * kStatTile + (int64_t)t * kStatRT;
+    const bool inR = R0 < nR;
     int64_t R[kStatRT];
     double wr[kStatRT], af[kStatRT], ag[kStatRT][DIM > 0 ? DIM : 1];
 #pragma unroll
     for (int r = 0; r < kStatRT; r++) {
-        R[r] = tile * kStatTile + r * kStatThreads + t;
-        wr[r] = R[r] < nR ? w[(R[r] / n2) % n2] : 0.0;
+        R[r] = R0 + r;
+        wr[r] = inR ? w[(R[r] / n2) % n2] : 0.0;
         af[r] = 0.0;
 #pragma unroll
         for (int a = 0; a < DIM; a++) ag[r][a] = 0.0;
@@ -45,10 +68,10 @@ __global__ __launch_bounds__(kStatThreads) void k_stat_pass(const T* __restrict_
     for (int64_t xb = x0; xb < x1; xb += kStatXU) {
         double v[kStatXU][kStatRT];
 #pragma unroll
-        for (int u = 0; u < kStatXU; u++)
-#pragma unroll
-            for (int r = 0; r < kStatRT; r++)
-                v[u][r] = (xb + u < x1 && R[r] < nR) ? (double)f[R[r] + nR * (xb + u)] : 0.0;
+        for (int u = 0; u < kStatXU; u++) {
+            const bool ok = inR && xb + u < x1;
+            ld4<T>(f + (ok ? R0 + nR * (xb + u) : 0), ok, v[u]);
+        }
 #pragma unroll
         for (int u = 0; u < kStatXU; u++) {
             const int64_t ix = xb + u;
@@ -66,9 +89,9 @@ __global__ __launch_bounds__(kStatThreads) void k_stat_pass(const T* __restrict_
             if (lane == 0) fxw[ix] = ws;
         }
     }
+    if (!inR) return;
 #pragma unroll
     for (int r = 0; r < kStatRT; r++) {
-        if (R[r] >= nR) continue;
         fRp[c * nR + R[r]] = af[r];
 #pragma unroll
         for (int a = 0; a < DIM; a++) gRp[(c * DIM + a) * nR + R[r]] = ag[r][a];
