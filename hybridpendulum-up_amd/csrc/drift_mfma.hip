@@ -937,15 +937,26 @@ __global__ __launch_bounds__(256) void k_axis_dft_mfma(const cT<T>* in, cT<T>* o
     const int64_t L = blockIdx.y;
     const int64_t j0 = (int64_t)blockIdx.x * 64;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
+    // the chunk's eight elements per thread (row a = w + 4 i, column lane):
+    // unconditional clamped loads, all issued before the first use
+    cT<T> x[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int a = w + 4 * i;
+        const int64_t j = j0 + lane;
+        const bool ok = a < N && j < inner;
+        x[i] = in[ok ? (L * N + a) * inner + j : 0];
+        if (!ok) x[i] = czero<T>();
+    }
     fill_toeplitz<T>(tab, bs, tid);
     if (tid < 32) pk[tid] = bs[64 + tid];
     __syncthreads();
-    for (int idx = tid; idx < 32 * 64; idx += 256) {
-        const int a = idx >> 6, jj = idx & 63;
-        const int64_t j = j0 + jj;
-        const cT<T> v = (a < N && j < inner) ? cmul<T>(in[(L * N + a) * inner + j], pk[a]) : czero<T>();
-        sre[a * CP + jj] = v.x;
-        sim[a * CP + jj] = v.y;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int a = w + 4 * i;
+        const cT<T> v = a < N ? cmul<T>(x[i], pk[a]) : czero<T>();
+        sre[a * CP + lane] = v.x;
+        sim[a * CP + lane] = v.y;
     }
     __syncthreads();
     acc_t acc[4];
