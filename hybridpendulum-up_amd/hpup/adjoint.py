@@ -1,30 +1,34 @@
 """Gradients through the spectral propagator of the reduced pendulum (dim 2).
 
-No counterpart in the reference, which only propagates forward.  One step of
-every integrator of the plan is a polynomial in the operator,
+No counterpart in the reference, which only propagates forward.  The drift
+operator A is linear and autonomous, and the plan evaluates one step of each
+integrator in Horner form,
 
-    F_{k+1} = P(dt A) F_k,   P(z) = sum_j c_j z^j
-    (Euler 1 + z; midpoint and RK2 1 + z + z^2/2; RK4 up to z^4/24),
+    y_0 = F_k,   y_j = F_k + c_j dt A y_{j-1}  (j = 1..s),   F_{k+1} = y_s,
+    c = (1) Euler, (1/2, 1) midpoint and RK2, (1/4, 1/3, 1/2, 1) RK4,
 
-with real c_j, so the adjoint step is P(dt A^H) = P(dt A)^H, which the adjoint
-plan (operator=1) applies with the same fused kernels.  A is linear in the
-damping b and the noise matrix G, so dA/db_p and dA/dG_ij are operators of the
-same form (operator=2+p, 2+dim+i+dim*j), and
+so reverse mode runs through the same stages backwards: with ybar_s the
+incoming adjoint,
 
-    dJ/dtheta = sum_k Re< lam_{k+1}, dP/dtheta F_k >,
-    dP/dtheta = sum_j c_j dt^j sum_{i<j} A^i (dA/dtheta) A^{j-1-i},
-    lam_K = dJ/dF_K,  lam_k = P^H lam_{k+1}
+    ybar_{j-1} = c_j dt A^H ybar_j,   Fbar_k = sum_{j=0..s} ybar_j,
+    dJ/dtheta += c_j dt Re< ybar_j, (dA/dtheta) y_{j-1} >
 
-(<x, y> = sum conj(x) y over the compact coefficients; a gradient g of a
-real J means dJ = Re<g, dF>).  The forward states are kept at checkpoints
-every `checkpoint` steps and recomputed one segment at a time on the way back.
+(<x, y> = sum conj(x) y over the compact coefficients; a gradient g of a real
+J means dJ = Re<g, dF>).  A^H is the adjoint plan (operator=1, the fused
+kernels with transposed tables).  A is linear in the damping b and the noise
+matrix G: dA/db_p = i K_p T_p (the damping term, applied by the operator=2+p
+plan), and dA/dG_ij is the diagonal factor of the diffusion term, so its
+products reduce elementwise.  The forward states are kept at checkpoints
+every `checkpoint` steps and each step's stage values are recomputed from
+them on the way back.
 """
+import math
+
 import numpy as np
 
 from . import HpupError, Plan
 
-POLY = {"euler": [1.0, 1.0], "midpoint": [1.0, 1.0, 0.5], "rk2": [1.0, 1.0, 0.5],
-        "rk4": [1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0]}
+HORNER = {"euler": [1.0], "midpoint": [0.5, 1.0], "rk2": [0.5, 1.0], "rk4": [0.25, 1.0 / 3.0, 0.5, 1.0]}
 
 
 def _dot(x, y):
@@ -32,49 +36,65 @@ def _dot(x, y):
     return float(torch.vdot(x, y).real)
 
 
+def omega_factors(N, L):
+    """kc(k) (first derivative, Nyquist zeroed) and kd(k) (second derivative)
+    of the omega-frequency index k, as the drift tables build them."""
+    Bx = N // 2
+    k = np.arange(N)
+    kk = np.where(k < Bx, k, k - N)
+    kp = np.where(k < Bx, k, np.where(k == Bx, 0, k - N))
+    return 2 * math.pi * kp / L, -(2 * math.pi * kk / L) ** 2
+
+
 class Gradient:
     """Forward propagation plus the adjoint sweep of the reduced model."""
 
     def __init__(self, B, N, L=1.0, b=(0, 0), G=None, dt=0.0, method="euler", precision="fp64"):
+        import torch
         kw = dict(L=L, b=tuple(b) + (0,) * (3 - len(b)), G=G, dt=dt, method=method, precision=precision)
-        self.dim = 2
         self.method = method.lower()
-        if self.method not in POLY:
+        if self.method not in HORNER:
             raise HpupError("unknown integrator %r" % method)
         self.dt = dt
+        self.N = N
         self.fwd = Plan(B, 2, N, **kw)
         self.adj = Plan(B, 2, N, operator=1, **kw)
-        # derivative operators, in the order b_0, b_1, G_00, G_10, G_01, G_11
-        self.dA = [Plan(B, 2, N, operator=2 + t, **kw) for t in range(2 + 4)]
+        # dA/db_0, dA/db_1
+        self.dAb = [Plan(B, 2, N, operator=2 + p, **kw) for p in range(2)]
+        # dA/dG_ij as a factor of the (k2, k1) grid (k1 fastest): D(k) =
+        # G_00 kd(k1) + G_11 kd(k2) - (G_10 + G_01) kc(k1) kc(k2)
+        kc, kd = omega_factors(N, L)
+        one = np.ones(N)
+        f = {(0, 0): np.outer(one, kd), (1, 1): np.outer(kd, one), (1, 0): -np.outer(kc, kc)}
+        f[(0, 1)] = f[(1, 0)]
+        rdt = torch.float64 if precision == "fp64" else torch.float32
+        self.dG = {ij: torch.from_numpy(v).to(rdt).cuda() for ij, v in f.items()}
 
-    def apply_step(self, F, nsteps=1):
-        return self.fwd.step(F, nsteps)
+    def _stages(self, F):
+        """y_0 .. y_{s-1} of one step from F."""
+        ys = [F]
+        for c in HORNER[self.method][:-1]:
+            ys.append(F + (c * self.dt) * self.fwd.derivative(ys[-1]))
+        return ys
 
-    def apply_adjoint_step(self, F, nsteps=1):
-        return self.adj.step(F, nsteps)
-
-    def _param_terms(self, F, lam):
-        """Re< lam, dP/dtheta F > for every parameter theta."""
-        c = POLY[self.method]
-        s = len(c) - 1
-        dt = self.dt
-        # y_q = A^q F and z_i = (A^H)^i lam, q, i < s
-        y = [F]
-        for _ in range(s - 1):
-            y.append(self.fwd.derivative(y[-1]))
-        z = [lam]
-        for _ in range(s - 1):
-            z.append(self.adj.derivative(z[-1]))
-        # w_q = sum_{j > q} c_j dt^j z_{j-1-q}
-        out = np.zeros(len(self.dA))
-        for q in range(s):
-            w = None
-            for j in range(q + 1, s + 1):
-                t = z[j - 1 - q] * (c[j] * dt ** j)
-                w = t if w is None else w + t
-            for t, op in enumerate(self.dA):
-                out[t] += _dot(w, op.derivative(y[q]))
-        return out
+    def _step_back(self, F, lam, acc):
+        """Adjoint of one step taken from F; returns Fbar, adds to acc."""
+        ys = self._stages(F)
+        c = HORNER[self.method]
+        Fbar = None
+        ybar = lam
+        N = self.N
+        for j in range(len(c) - 1, -1, -1):
+            Fbar = ybar.clone() if Fbar is None else Fbar + ybar
+            s = c[j] * self.dt
+            y = ys[j]
+            for p in range(2):
+                acc[p] += s * _dot(ybar, self.dAb[p].derivative(y))
+            prod = (ybar.conj() * y).real.reshape(-1, N, N).sum(0)
+            for (i, jj), fac in self.dG.items():
+                acc[2 + i + 2 * jj] += s * float((prod * fac).sum())
+            ybar = self.adj.derivative(ybar) * s
+        return Fbar + ybar
 
     def run(self, F0, nsteps, dJ, checkpoint=None):
         """Propagates F0 by nsteps steps and back-propagates the gradient.
@@ -94,7 +114,7 @@ class Gradient:
                 checkpoints.append(F.clone())
         FK = F
         lam = dJ(FK).clone()
-        total = np.zeros(len(self.dA))
+        acc = np.zeros(6)
         for seg in range(len(checkpoints) - 1, -1, -1):
             k0 = seg * cp
             k1 = min(K, k0 + cp)
@@ -104,11 +124,41 @@ class Gradient:
                 self.fwd.step(nxt, 1)
                 states.append(nxt)
             for k in range(k1 - 1, k0 - 1, -1):
-                total += self._param_terms(states[k - k0], lam)
-                self.adj.step(lam, 1)
+                lam = self._step_back(states[k - k0], lam, acc)
             del states
         G = np.zeros((2, 2))
         for i in range(2):
             for j in range(2):
-                G[i, j] = total[2 + i + 2 * j]
-        return FK, lam, {"b": total[:2].copy(), "G": G}
+                G[i, j] = acc[2 + i + 2 * j]
+        return FK, lam, {"b": acc[:2].copy(), "G": G}
+
+
+def propagate_function():
+    """A torch.autograd.Function over the propagation:
+    apply(F0, b, G, grad_model, nsteps, checkpoint) -> F_K, differentiable in F0
+    (complex), b (2,) and G (2, 2) (float64 CPU tensors).  grad_model is a
+    Gradient built at the same b and G (it defines the operator); the backward
+    pass calls its adjoint sweep."""
+    import torch
+
+    class Propagate(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, F0, b, G, model, nsteps, checkpoint):
+            ctx.model, ctx.nsteps, ctx.checkpoint = model, nsteps, checkpoint
+            ctx.save_for_backward(F0)
+            F = F0.detach().clone()
+            model.fwd.step(F, nsteps)
+            return F
+
+        @staticmethod
+        def backward(ctx, gout):
+            (F0,) = ctx.saved_tensors
+            # torch's gradient of a real loss w.r.t. a complex tensor is
+            # conj(dJ/dz*) convention-free: dJ = Re<g, dz> with g = gout
+            _, gF0, g = ctx.model.run(F0.detach(), ctx.nsteps, lambda F: gout.contiguous(),
+                                      checkpoint=ctx.checkpoint)
+            gb = torch.tensor(g["b"], dtype=torch.float64)
+            gG = torch.tensor(g["G"], dtype=torch.float64)
+            return gF0, gb, gG, None, None, None
+
+    return Propagate

@@ -6,9 +6,10 @@ forward operator, whose parity with the oracle the other GPU tests hold:
 * <A x, y> = <x, A^H y> for random complex x, y (fixes A^H);
 * <P x, y> = <x, P^H y> for the step of every integrator;
 * A(theta + e_t) x - A(theta) x = (dA/dtheta_t) x (A is linear in b and G);
-* the harness's gradients against central differences of J(theta) through
-  separately built forward plans, and the F0 gradient against the exact
-  change of the (linear) objective.
+* the harness's gradients (reverse mode through the Horner stages) against
+  central differences of J(theta) through separately built forward plans,
+  and the F0 gradient against the exact change of the (linear) objective;
+* the torch.autograd.Function binding against the harness.
 
 Tolerances: FP64 1e-12 relative on the identities (roundoff of two fused
 evaluations), 2e-7 on the central differences (h = 1e-4: the O(h^2) error of
@@ -144,3 +145,23 @@ def test_operator_kind_errors():
         _plan(operator=8)
     with pytest.raises(hpup.HpupError):
         hpup.Plan(4, 3, 8, L, precision="fp64", operator=1)  # the adjoint is built for dim 2
+
+
+def test_autograd_function():
+    """The torch.autograd.Function binding returns the harness's gradients."""
+    import torch
+    from hpup.adjoint import propagate_function
+    _print("autograd function")
+    gr = Gradient(B, N, L, b=BT, G=G2, dt=0.02, method="rk4")
+    F0 = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 11)
+    w = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 12)
+    Fa = F0.clone().requires_grad_(True)
+    b = torch.tensor(BT, dtype=torch.float64, requires_grad=True)
+    G = torch.tensor(G2, dtype=torch.float64, requires_grad=True)
+    FK = propagate_function().apply(Fa, b, G, gr, 5, 2)
+    loss = (w.conj() * FK).real.sum()
+    loss.backward()
+    _, gF0, g = gr.run(F0, 5, lambda F: w, checkpoint=2)
+    assert float((Fa.grad - gF0).abs().max()) <= 1e-12 * float(gF0.abs().max())
+    assert np.allclose(b.grad.numpy(), g["b"], rtol=1e-12, atol=0)
+    assert np.allclose(G.grad.numpy(), g["G"], rtol=1e-12, atol=0)
