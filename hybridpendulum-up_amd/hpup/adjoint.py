@@ -31,11 +31,6 @@ from . import HpupError, Plan
 HORNER = {"euler": [1.0], "midpoint": [0.5, 1.0], "rk2": [0.5, 1.0], "rk4": [0.25, 1.0 / 3.0, 0.5, 1.0]}
 
 
-def _dot(x, y):
-    import torch
-    return float(torch.vdot(x, y).real)
-
-
 def omega_factors(N, L):
     """kc(k) (first derivative, Nyquist zeroed) and kd(k) (second derivative)
     of the omega-frequency index k, as the drift tables build them."""
@@ -78,7 +73,9 @@ class Gradient:
         return ys
 
     def _step_back(self, F, lam, acc):
-        """Adjoint of one step taken from F; returns Fbar, adds to acc."""
+        """Adjoint of one step taken from F; returns Fbar, adds to acc (a
+        device tensor of the six parameter gradients: no host sync here)."""
+        import torch
         ys = self._stages(F)
         c = HORNER[self.method]
         Fbar = None
@@ -88,11 +85,11 @@ class Gradient:
             Fbar = ybar.clone() if Fbar is None else Fbar + ybar
             s = c[j] * self.dt
             y = ys[j]
-            for p in range(2):
-                acc[p] += s * _dot(ybar, self.dAb[p].derivative(y))
+            terms = [torch.vdot(ybar, self.dAb[p].derivative(y)).real for p in range(2)]
             prod = (ybar.conj() * y).real.reshape(-1, N, N).sum(0)
-            for (i, jj), fac in self.dG.items():
-                acc[2 + i + 2 * jj] += s * float((prod * fac).sum())
+            for ij in ((0, 0), (1, 0), (0, 1), (1, 1)):  # flat index 2 + i + 2 j
+                terms.append((prod * self.dG[ij]).sum())
+            acc += s * torch.stack(terms).to(acc.dtype)
             ybar = self.adj.derivative(ybar) * s
         return Fbar + ybar
 
@@ -104,6 +101,7 @@ class Gradient:
         Returns (F_K, grad_F0, grads) with grads = {"b": (2,), "G": (2, 2)}.
         F0 is left unchanged.
         """
+        import torch
         K = int(nsteps)
         cp = max(1, int(checkpoint or K or 1))
         checkpoints = [F0.clone()]
@@ -114,7 +112,7 @@ class Gradient:
                 checkpoints.append(F.clone())
         FK = F
         lam = dJ(FK).clone()
-        acc = np.zeros(6)
+        acc = torch.zeros(6, dtype=torch.float64, device=FK.device)
         for seg in range(len(checkpoints) - 1, -1, -1):
             k0 = seg * cp
             k1 = min(K, k0 + cp)
@@ -126,6 +124,7 @@ class Gradient:
             for k in range(k1 - 1, k0 - 1, -1):
                 lam = self._step_back(states[k - k0], lam, acc)
             del states
+        acc = acc.cpu().numpy()
         G = np.zeros((2, 2))
         for i in range(2):
             for j in range(2):
@@ -153,8 +152,8 @@ def propagate_function():
         @staticmethod
         def backward(ctx, gout):
             (F0,) = ctx.saved_tensors
-            # torch's gradient of a real loss w.r.t. a complex tensor is
-            # conj(dJ/dz*) convention-free: dJ = Re<g, dz> with g = gout
+            # PyTorch's gradient of a real loss with respect to a complex
+            # tensor is dL/dRe + i dL/dIm: the g of dJ = Re<g, dz> used here
             _, gF0, g = ctx.model.run(F0.detach(), ctx.nsteps, lambda F: gout.contiguous(),
                                       checkpoint=ctx.checkpoint)
             gb = torch.tensor(g["b"], dtype=torch.float64)
