@@ -280,6 +280,9 @@ struct PlanBase {
     virtual void backward(const void* F, void* f, hipStream_t st) = 0;
     virtual void step(void* F, int nsteps, hipStream_t st) = 0;
     virtual void derivative(const void* F, void* dF, hipStream_t st) = 0;
+    virtual void adjoint_stage(const void* ybar, const void* y, double scale, void* out, double* gout,
+                               hipStream_t st) = 0;
+    virtual void axpy_stage(const void* in, const void* F0, double scale, void* out, hipStream_t st) = 0;
     virtual void full_to_compact(const void* a, void* b, hipStream_t st) = 0;
     virtual void compact_to_full(const void* a, void* b, hipStream_t st) = 0;
     // device pass of get_stat: rotation marginal, first omega moment per
@@ -299,7 +302,7 @@ template <class T> struct PlanT : PlanBase {
     DriftTables<T> dr;
     bool has_drift = false;
     bool bf16x3 = false;  // circulants as 3xBF16 split products (FP32 storage)
-    DevBuf G, Wb, V1, V2, Z3, Fs, Fs2, ACC, tmp, st_work, st_out;
+    DevBuf G, Wb, V1, V2, Z3, Fs, Fs2, ACC, tmp, st_work, st_out, gpart;
 
     void init_xform(const double* d, const double* wv) {
         const hpup_model& m = model;
@@ -496,6 +499,32 @@ template <class T> struct PlanT : PlanBase {
     void derivative(const void* F, void* dF, hipStream_t st) override {
         ensure_drift_buffers(false);
         stage(F, nullptr, dF, StageCoef<T>{1, 0, 1, 0, 0, 0, 0, 0, 1}, st);
+        HPUP_HIP(hipGetLastError());
+    }
+    // out = F0 + scale A in: one Horner stage (the fused kernel's epilogue)
+    void axpy_stage(const void* in, const void* F0, double scale, void* out, hipStream_t st) override {
+        HPUP_REQUIRE(in && F0 && out, HPUP_ERR_ARG, "null argument");
+        ensure_drift_buffers(false);
+        stage(in, F0, out, StageCoef<T>{(T)scale, 0, 1, 0, 0, 1, 0, 0, 0}, st);
+        HPUP_HIP(hipGetLastError());
+    }
+    // One reverse-mode stage of the gradient sweep on an adjoint plan:
+    // out = scale A^H ybar, gout[0..4] += scale Re<ybar, (dA/dtheta) y> for
+    // theta = b_1, b_2, G_00, G_11 and G_01 (= the G_10 term).
+    void adjoint_stage(const void* ybar, const void* y, double scale, void* out, double* gout,
+                       hipStream_t st) override {
+        HPUP_REQUIRE(has_drift && fused() && dr.view.adj, HPUP_ERR_UNSUPPORTED,
+                     "adjoint_stage needs an adjoint operator plan (operator kind 1, N <= 32)");
+        HPUP_REQUIRE(ybar && y && out && gout, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(ybar != out && y != out, HPUP_ERR_ARG, "adjoint stage cannot run in place");
+        ensure(gpart, sizeof(double) * kGradMaxWG * 5);
+        DriftDev<T> D = dr.view;
+        D.gy = y;
+        D.gpart = gpart.as<double>();
+        D.gout = gout;
+        D.gscale = scale;
+        launch_drift_fused<T>(D, dr.mix, ybar, Z3.p, nullptr, out, ACC.p,
+                              StageCoef<T>{(T)scale, 0, 1, 0, 0, 0, 0, 0, 1}, false, st);
         HPUP_HIP(hipGetLastError());
     }
     // Per-kernel timing of nreps steps of the plan's method: an event before
@@ -724,6 +753,13 @@ int hpup_plan_step(hpup_plan* p, void* F, int nsteps, void* stream) {
 }
 int hpup_plan_derivative(hpup_plan* p, const void* F, void* dF, void* stream) {
     return guard([&] { p->impl->derivative(F, dF, (hipStream_t)stream); });
+}
+int hpup_plan_axpy_stage(hpup_plan* p, const void* in, const void* F0, double scale, void* out, void* stream) {
+    return guard([&] { p->impl->axpy_stage(in, F0, scale, out, (hipStream_t)stream); });
+}
+int hpup_plan_adjoint_stage(hpup_plan* p, const void* ybar, const void* y, double scale, void* ybar_out,
+                            double* grad, void* stream) {
+    return guard([&] { p->impl->adjoint_stage(ybar, y, scale, ybar_out, grad, (hipStream_t)stream); });
 }
 int hpup_plan_full_to_compact(hpup_plan* p, const void* a, void* b, void* stream) {
     return guard([&] { p->impl->full_to_compact(a, b, (hipStream_t)stream); });

@@ -38,7 +38,14 @@ template <class T> struct DriftDev {
     T c0[3];
     // the adjoint operator (transposed tables, damping through the V planes)
     int adj;
+    // gradient stage (adj only, gpart non-null): forward stage value y, the
+    // workgroups' partial sums (kGradMaxWG x 5), and out[5] += gscale * sums
+    const void* gy;
+    double* gpart;
+    double* gout;
+    double gscale;
 };
+constexpr int kGradMaxWG = 4096;
 
 // Integrator epilogue of the slab kernel:
 //   Fout = F0 + dt (ca ACC + cd dF);  ACC <- ea ACC + ed dF

@@ -331,13 +331,22 @@ template <bool R> struct FusedWaves<float, GemmF<float, R>> { static constexpr i
 // transposed mixing tables and the conjugate-reversed circulants; the damping
 // term i b_p c_p T_p F becomes T_p^H(-i b_p c_p F), which joins the V planes,
 // so the first GEMM pair is not needed.
-template <class T, class G, int DIM, bool GRAV, bool ADJ = false>
-__global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
+// GRAD (with ADJ): one reverse-mode stage of the gradient sweep.  Besides
+// A^H ybar it reduces, against the forward stage value y (D.gy), the
+// parameter terms Re<ybar, (dA/dtheta) y>: the damping ones as
+// Re<T_p^H(-i K_p ybar), y>, formed in the GEMM pair the adjoint leaves free
+// (planes A and D hold -i K_1 ybar and -i K_2 ybar), the diffusion ones
+// elementwise; each workgroup writes its five sums to D.gpart.
+template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false>
+__global__ __launch_bounds__(256, (GRAD ? 2 : FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
                                                      const cT<T>* __restrict__ Z3, const cT<T>* F0, cT<T>* Fout,
                                                      cT<T>* ACC, StageCoef<T> sc, int64_t nslab) {
     __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad], pc_re[32 * kPad],
         pc_im[32 * kPad];
     __shared__ CircTab<T> tabs[2];
+    __shared__ T pd_re[GRAD ? 32 * kPad : 1], pd_im[GRAD ? 32 * kPad : 1];
+    __shared__ double gred[GRAD ? 4 * 5 : 1];
+    double ga[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // GRAD: b_1, b_2, G_00, G_11, G_01 (= G_10) terms
     constexpr int NC = DIM + 2;
     constexpr int MB = MixBatch<T>::n;
 
@@ -422,10 +431,24 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             fo[v] = *at_b(Fb, offb[v]);
             Zv[v] = DIM == 3 ? ld_once<T>(at_b(Z3 + base, offb[v])) : czero<T>();
         }
+        cT<T> yv[4];
+        if constexpr (GRAD) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            pa_re[k2w[v] * kPad + k1w] = ok[v] ? fo[v].x : T(0);
-            pa_im[k2w[v] * kPad + k1w] = ok[v] ? fo[v].y : T(0);
+            for (int v = 0; v < 4; v++) yv[v] = ld_once<T>(at_b((const cT<T>*)D.gy + base, offb[v]));
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const cT<T> a = cimul<T>(fo[v], -kc1), d = cimul<T>(fo[v], -kc2v[v]);
+                pa_re[k2w[v] * kPad + k1w] = ok[v] ? a.x : T(0);
+                pa_im[k2w[v] * kPad + k1w] = ok[v] ? a.y : T(0);
+                pd_re[k2w[v] * kPad + k1w] = ok[v] ? d.x : T(0);
+                pd_im[k2w[v] * kPad + k1w] = ok[v] ? d.y : T(0);
+            }
+        } else if constexpr (!ADJ) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                pa_re[k2w[v] * kPad + k1w] = ok[v] ? fo[v].x : T(0);
+                pa_im[k2w[v] * kPad + k1w] = ok[v] ? fo[v].y : T(0);
+            }
         }
 
         // rotation mixing: sources in batches of MB rows, structurally zero
@@ -497,6 +520,22 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
 
         cT<T> P1[4], P2[4];
         if constexpr (ADJ) {
+            if constexpr (GRAD) {
+                cT<T> H1[4], H2[4];  // T_p^H (-i K_p ybar)
+                gm.t1(pa_re, pa_im, H1, D.c0[0]);
+                gm.t2(pd_re, pd_im, H2, D.c0[1]);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    if (!ok[v]) continue;
+                    const double yr = yv[v].x, yi = yv[v].y;
+                    ga[0] += (double)H1[v].x * yr + (double)H1[v].y * yi;
+                    ga[1] += (double)H2[v].x * yr + (double)H2[v].y * yi;
+                    const double q = (double)fo[v].x * yr + (double)fo[v].y * yi;  // Re(conj(ybar) y)
+                    ga[2] += (double)kd1 * q;
+                    ga[3] += (double)kd2v[v] * q;
+                    ga[4] -= (double)kc1 * (double)kc2v[v] * q;
+                }
+            }
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 P1[v] = P2[v] = czero<T>();
@@ -553,7 +592,7 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
         if (sc.deriv_only) {
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                if (ok[v]) st_once<T>(at_b(Fout + base, offb[v]), q[v]);
+                if (ok[v]) st_once<T>(at_b(Fout + base, offb[v]), cscale<T>(q[v], sc.dt));
         } else {
             // a stage without F0 takes the slab itself; the loads of the four
             // elements are issued together, under workgroup-uniform conditions
@@ -601,6 +640,26 @@ __global__ __launch_bounds__(256, (FusedWaves<T, G>::n)) void k_drift_fused(Drif
             k3++;
         }
     }
+    if constexpr (GRAD) {  // the workgroup's five sums, in a fixed order
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            double x = ga[i];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            if (lane == 0) gred[w * 5 + i] = x;
+        }
+        __syncthreads();
+        if (tid < 5) D.gpart[(int64_t)blockIdx.x * 5 + tid] = gred[tid] + gred[5 + tid] + gred[10 + tid] + gred[15 + tid];
+    }
+}
+
+// out[i] += scale * sum over the workgroups' partials (fixed order)
+__global__ void k_grad_reduce(const double* __restrict__ part, int nwg, double scale, double* __restrict__ out) {
+    const int i = threadIdx.x;
+    if (i >= 5) return;
+    double s = 0.0;
+    for (int g = 0; g < nwg; g++) s += part[(int64_t)g * 5 + i];
+    out[i] += scale * s;
 }
 
 // ------------------------------------------------------------ axis 3 ----
@@ -776,13 +835,15 @@ template <class K> static unsigned persistent_grid(K kernel, int64_t nslab) {
     return (unsigned)g;
 }
 
-template <class T, class G, int DIM, bool GRAV, bool ADJ = false>
-static void launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3,
-                           const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab,
-                           hipStream_t st) {
-    auto kern = k_drift_fused<T, G, DIM, GRAV, ADJ>;
-    hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, nslab)), dim3(256), 0, st, D, MX, (const cT<T>*)Fin,
-                       (const cT<T>*)Z3, (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
+template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false>
+static unsigned launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3,
+                               const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab,
+                               hipStream_t st) {
+    auto kern = k_drift_fused<T, G, DIM, GRAV, ADJ, GRAD>;
+    const unsigned grid = persistent_grid(kern, nslab);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, D, MX, (const cT<T>*)Fin, (const cT<T>*)Z3,
+                       (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
+    return grid;
 }
 
 template <class T>
@@ -810,6 +871,15 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
         // model plans have coordinate-ramp circulants; only that form is built
         HPUP_REQUIRE(D.dim == 2 && !grav && D.ramp, HPUP_ERR_UNSUPPORTED,
                      "the adjoint operator is built for the reduced model on its own omega grid");
+        if (D.gpart) {
+            HPUP_REQUIRE(D.gy && sc.deriv_only, HPUP_ERR_ARG, "gradient stage needs the forward stage value");
+            const unsigned nwg = launch_fused_t<T, GemmF<T, true>, 2, false, true, true>(D, MX, Fin, Z3, F0, Fout, ACC,
+                                                                                       sc, nslab, st);
+            HPUP_REQUIRE(nwg <= kGradMaxWG, HPUP_ERR_UNSUPPORTED, "gradient stage: partial buffer too small");
+            hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(64), 0, st, (const double*)D.gpart, (int)nwg,
+                               (double)D.gscale, D.gout);
+            return;
+        }
         launch_fused_t<T, GemmF<T, true>, 2, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
         return;
     }

@@ -75,6 +75,7 @@ EXPORTS = [
     "hpup_plan_profile", "hpup_plan_profile_name", "hpup_plan_profile_bytes",
     "hpup_get_stat", "hpup_psvd", "hpup_pdf_MF_normal", "hpup_pdf_MF_M2S", "hpup_plan_get_stat",
     "hpup_pendulum_plot_getc", "hpup_gyrobias_propagate", "hpup_plan_create_operator",
+    "hpup_plan_adjoint_stage", "hpup_plan_axpy_stage",
 ]
 
 
@@ -102,6 +103,10 @@ def lib():
                      "hpup_plan_full_to_compact", "hpup_plan_compact_to_full"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.hpup_plan_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.hpup_plan_axpy_stage.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                           ctypes.c_void_p, ctypes.c_void_p]
+        L.hpup_plan_adjoint_stage.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.hpup_plan_destroy.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_coeffs.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_grid.argtypes = [ctypes.c_void_p]
@@ -487,6 +492,28 @@ class Plan:
         _check(lib().hpup_plan_derivative(self._h, self._chk(F, self.ncoef, self.cdtype),
                                           self._chk(dF, self.ncoef, self.cdtype), self._stream()))
         return dF
+
+    def axpy_stage(self, x, F0, scale, out=None):
+        """out = F0 + scale A x (one Horner stage of the plan's operator)."""
+        out = self.empty_coeffs() if out is None else out
+        _check(lib().hpup_plan_axpy_stage(self._h, self._chk(x, self.ncoef, self.cdtype),
+                                          self._chk(F0, self.ncoef, self.cdtype), float(scale),
+                                          self._chk(out, self.ncoef, self.cdtype), self._stream()))
+        return out
+
+    def adjoint_stage(self, ybar, y, scale, grad, out=None):
+        """Adjoint plans (operator=1): out = scale A^H ybar and, on the device,
+        grad (float64, 5) += scale Re<ybar, dA/dtheta y> for theta = b_1, b_2,
+        G_00, G_11, G_01 (= G_10 term)."""
+        import torch
+        out = self.empty_coeffs() if out is None else out
+        if not (grad.is_cuda and grad.dtype == torch.float64 and grad.numel() == 5 and grad.is_contiguous()):
+            raise HpupError("grad must be a contiguous CUDA float64 tensor of 5 elements")
+        _check(lib().hpup_plan_adjoint_stage(self._h, self._chk(ybar, self.ncoef, self.cdtype),
+                                             self._chk(y, self.ncoef, self.cdtype), float(scale),
+                                             self._chk(out, self.ncoef, self.cdtype), ctypes.c_void_p(grad.data_ptr()),
+                                             self._stream()))
+        return out
 
     @property
     def mass_index(self):

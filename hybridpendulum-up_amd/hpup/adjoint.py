@@ -16,14 +16,14 @@ incoming adjoint,
 (<x, y> = sum conj(x) y over the compact coefficients; a gradient g of a real
 J means dJ = Re<g, dF>).  A^H is the adjoint plan (operator=1, the fused
 kernels with transposed tables).  A is linear in the damping b and the noise
-matrix G: dA/db_p = i K_p T_p (the damping term, applied by the operator=2+p
-plan), and dA/dG_ij is the diagonal factor of the diffusion term, so its
-products reduce elementwise.  The forward states are kept at checkpoints
+matrix G: dA/db_p = i K_p T_p (the damping term) and dA/dG_ij is the diagonal
+factor of the diffusion term.  Each reverse stage is one fused kernel
+(Plan.adjoint_stage) that applies A^H and reduces all five parameter terms;
+the forward stage values come from the same fused stage kernel
+(Plan.axpy_stage).  The forward states are kept at checkpoints
 every `checkpoint` steps and each step's stage values are recomputed from
 them on the way back.
 """
-import math
-
 import numpy as np
 
 from . import HpupError, Plan
@@ -31,21 +31,10 @@ from . import HpupError, Plan
 HORNER = {"euler": [1.0], "midpoint": [0.5, 1.0], "rk2": [0.5, 1.0], "rk4": [0.25, 1.0 / 3.0, 0.5, 1.0]}
 
 
-def omega_factors(N, L):
-    """kc(k) (first derivative, Nyquist zeroed) and kd(k) (second derivative)
-    of the omega-frequency index k, as the drift tables build them."""
-    Bx = N // 2
-    k = np.arange(N)
-    kk = np.where(k < Bx, k, k - N)
-    kp = np.where(k < Bx, k, np.where(k == Bx, 0, k - N))
-    return 2 * math.pi * kp / L, -(2 * math.pi * kk / L) ** 2
-
-
 class Gradient:
     """Forward propagation plus the adjoint sweep of the reduced model."""
 
     def __init__(self, B, N, L=1.0, b=(0, 0), G=None, dt=0.0, method="euler", precision="fp64"):
-        import torch
         kw = dict(L=L, b=tuple(b) + (0,) * (3 - len(b)), G=G, dt=dt, method=method, precision=precision)
         self.method = method.lower()
         if self.method not in HORNER:
@@ -54,44 +43,26 @@ class Gradient:
         self.N = N
         self.fwd = Plan(B, 2, N, **kw)
         self.adj = Plan(B, 2, N, operator=1, **kw)
-        # dA/db_0, dA/db_1
-        self.dAb = [Plan(B, 2, N, operator=2 + p, **kw) for p in range(2)]
-        # dA/dG_ij as a factor of the (k2, k1) grid (k1 fastest): D(k) =
-        # G_00 kd(k1) + G_11 kd(k2) - (G_10 + G_01) kc(k1) kc(k2)
-        kc, kd = omega_factors(N, L)
-        one = np.ones(N)
-        f = {(0, 0): np.outer(one, kd), (1, 1): np.outer(kd, one), (1, 0): -np.outer(kc, kc)}
-        f[(0, 1)] = f[(1, 0)]
-        rdt = torch.float64 if precision == "fp64" else torch.float32
-        self.dG = {ij: torch.from_numpy(v).to(rdt).cuda() for ij, v in f.items()}
 
     def _stages(self, F):
-        """y_0 .. y_{s-1} of one step from F."""
+        """y_0 .. y_{s-1} of one step from F (the plan's fused stage kernel)."""
         ys = [F]
         for c in HORNER[self.method][:-1]:
-            ys.append(F + (c * self.dt) * self.fwd.derivative(ys[-1]))
+            ys.append(self.fwd.axpy_stage(ys[-1], F, c * self.dt))
         return ys
 
     def _step_back(self, F, lam, acc):
-        """Adjoint of one step taken from F; returns Fbar, adds to acc (a
-        device tensor of the six parameter gradients: no host sync here)."""
-        import torch
+        """Adjoint of one step taken from F; returns Fbar.  acc (device,
+        float64, 5) accumulates the b_1, b_2, G_00, G_11, G_01 terms: each
+        stage is one fused kernel (A^H ybar and the five reductions)."""
         ys = self._stages(F)
         c = HORNER[self.method]
-        Fbar = None
+        Fbar = lam.clone()
         ybar = lam
-        N = self.N
         for j in range(len(c) - 1, -1, -1):
-            Fbar = ybar.clone() if Fbar is None else Fbar + ybar
-            s = c[j] * self.dt
-            y = ys[j]
-            terms = [torch.vdot(ybar, self.dAb[p].derivative(y)).real for p in range(2)]
-            prod = (ybar.conj() * y).real.reshape(-1, N, N).sum(0)
-            for ij in ((0, 0), (1, 0), (0, 1), (1, 1)):  # flat index 2 + i + 2 j
-                terms.append((prod * self.dG[ij]).sum())
-            acc += s * torch.stack(terms).to(acc.dtype)
-            ybar = self.adj.derivative(ybar) * s
-        return Fbar + ybar
+            ybar = self.adj.adjoint_stage(ybar, ys[j], c[j] * self.dt, acc)
+            Fbar.add_(ybar)
+        return Fbar
 
     def run(self, F0, nsteps, dJ, checkpoint=None):
         """Propagates F0 by nsteps steps and back-propagates the gradient.
@@ -112,7 +83,7 @@ class Gradient:
                 checkpoints.append(F.clone())
         FK = F
         lam = dJ(FK).clone()
-        acc = torch.zeros(6, dtype=torch.float64, device=FK.device)
+        acc = torch.zeros(5, dtype=torch.float64, device=FK.device)
         for seg in range(len(checkpoints) - 1, -1, -1):
             k0 = seg * cp
             k1 = min(K, k0 + cp)
@@ -125,10 +96,7 @@ class Gradient:
                 lam = self._step_back(states[k - k0], lam, acc)
             del states
         acc = acc.cpu().numpy()
-        G = np.zeros((2, 2))
-        for i in range(2):
-            for j in range(2):
-                G[i, j] = acc[2 + i + 2 * j]
+        G = np.array([[acc[2], acc[4]], [acc[4], acc[3]]])  # G_10 and G_01 enter A as their sum
         return FK, lam, {"b": acc[:2].copy(), "G": G}
 
 

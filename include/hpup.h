@@ -270,6 +270,24 @@ int hpup_plan_destroy(hpup_plan* plan);
  *           (A is linear in b and G), for hpup_plan_derivative. */
 int hpup_plan_create_operator(const hpup_model* model, int kind, hpup_plan** plan);
 
+/* out = F0 + scale * A in: one Horner stage of the plan's operator (kind 0
+ * or 1), the stage its integrators run (no counterpart in the reference;
+ * callers: gradient sweeps recomputing forward stage values).  out may alias
+ * F0 but not in. */
+int hpup_plan_axpy_stage(hpup_plan* plan, const void* in, const void* F0, double scale, void* out, void* stream);
+
+/* One reverse-mode stage of the gradient sweep, on a kind-1 (adjoint) plan
+ * (no counterpart in the reference; callers: gradient computations).  For a
+ * forward stage y_j = F + scale A y_{j-1} with incoming adjoint ybar (= ybar_j)
+ * and forward stage value y (= y_{j-1}), all device arrays of the plan's
+ * coefficient count:
+ *   ybar_out = scale * A^H ybar,
+ *   grad[0..4] += scale * Re<ybar, (dA/dtheta) y>, theta = b_1, b_2, G_00, G_11
+ *   and G_01 (the same term as G_10); grad is a device array of 5 doubles.
+ * ybar_out may alias neither input. */
+int hpup_plan_adjoint_stage(hpup_plan* plan, const void* ybar, const void* y, double scale, void* ybar_out,
+                            double* grad, void* stream);
+
 /* Number of compact coefficients: nRc * nx, nRc = B(2B-1)(2B+1)/3. */
 int64_t hpup_plan_num_coeffs(const hpup_plan* plan);
 /* Number of grid points: (2B)^3 * nx. */
