@@ -39,8 +39,10 @@ template <class T> struct DriftDev {
     // the adjoint operator (transposed tables, damping through the V planes)
     int adj;
     // gradient stage (adj only, gpart non-null): forward stage value y, the
-    // workgroups' partial sums (kGradMaxWG x 5), and out[5] += gscale * sums
+    // sweep's running sum, the workgroups' partial sums (kGradMaxWG x 5), and
+    // out[5] += gscale * sums
     const void* gy;
+    void* gacc;      // nullable: gacc += the stage output
     double* gpart;
     double* gout;
     double gscale;

@@ -593,6 +593,22 @@ __global__ __launch_bounds__(256, (GRAD ? 2 : FusedWaves<T, G>::n)) void k_drift
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 if (ok[v]) st_once<T>(at_b(Fout + base, offb[v]), cscale<T>(q[v], sc.dt));
+            if constexpr (GRAD) {
+                if (D.gacc) {  // the sweep's running sum: gacc += dt q
+                    cT<T>* A = (cT<T>*)D.gacc + base;
+                    cT<T> av[4];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) av[v] = ld_once<T>(at_b(A, offb[v]));
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        av[v] = cadd<T>(av[v], cscale<T>(q[v], sc.dt));
+                        asm volatile("" : "+v"(av[v].x), "+v"(av[v].y));
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        if (ok[v]) st_once<T>(at_b(A, offb[v]), av[v]);
+                }
+            }
         } else {
             // a stage without F0 takes the slab itself; the loads of the four
             // elements are issued together, under workgroup-uniform conditions

@@ -60,8 +60,7 @@ class Gradient:
         Fbar = lam.clone()
         ybar = lam
         for j in range(len(c) - 1, -1, -1):
-            ybar = self.adj.adjoint_stage(ybar, ys[j], c[j] * self.dt, acc)
-            Fbar.add_(ybar)
+            ybar = self.adj.adjoint_stage(ybar, ys[j], c[j] * self.dt, acc, fbar=Fbar)
         return Fbar
 
     def run(self, F0, nsteps, dJ, checkpoint=None):

@@ -281,12 +281,13 @@ int hpup_plan_axpy_stage(hpup_plan* plan, const void* in, const void* F0, double
  * forward stage y_j = F + scale A y_{j-1} with incoming adjoint ybar (= ybar_j)
  * and forward stage value y (= y_{j-1}), all device arrays of the plan's
  * coefficient count:
- *   ybar_out = scale * A^H ybar,
+ *   ybar_out = scale * A^H ybar,   fbar += ybar_out (fbar nullable: the sweep's
+ *   running sum of stage adjoints),
  *   grad[0..4] += scale * Re<ybar, (dA/dtheta) y>, theta = b_1, b_2, G_00, G_11
  *   and G_01 (the same term as G_10); grad is a device array of 5 doubles.
- * ybar_out may alias neither input. */
+ * No two of ybar_out, fbar and the inputs may alias. */
 int hpup_plan_adjoint_stage(hpup_plan* plan, const void* ybar, const void* y, double scale, void* ybar_out,
-                            double* grad, void* stream);
+                            void* fbar, double* grad, void* stream);
 
 /* Number of compact coefficients: nRc * nx, nRc = B(2B-1)(2B+1)/3. */
 int64_t hpup_plan_num_coeffs(const hpup_plan* plan);
