@@ -338,7 +338,7 @@ template <bool R> struct FusedWaves<float, GemmF<float, R>> { static constexpr i
 // (planes A and D hold -i K_1 ybar and -i K_2 ybar), the diffusion ones
 // elementwise; each workgroup writes its five sums to D.gpart.
 template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false>
-__global__ __launch_bounds__(256, (GRAD ? 2 : FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
+__global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
                                                      const cT<T>* __restrict__ Z3, const cT<T>* F0, cT<T>* Fout,
                                                      cT<T>* ACC, StageCoef<T> sc, int64_t nslab) {
     __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad], pc_re[32 * kPad],
