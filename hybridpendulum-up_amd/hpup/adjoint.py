@@ -128,3 +128,46 @@ def propagate_function():
             return gF0, gb, gG, None, None, None
 
     return Propagate
+
+
+class TransformAdjoint:
+    """Adjoints of a plan's SO(3) x R^n transforms (no counterpart in the
+    reference), for objectives on the grid density.  With A the forward
+    transform (grid f -> coefficients F) and B the inverse (F -> f = Re(...)),
+    stage by stage: the angular DFTs of A and B are each other's adjoints, the
+    beta stages differ by the weights w_j (A) and 2l + 1 (B), and the omega
+    DFTs by 1/nx, so
+
+        B^H g    = ((2l + 1) / nx) A(g / w_j),
+        A^H Fbar = w_j B(nx Fbar / (2l + 1)),
+
+    adjoints for the real inner product on grids and Re<.,.> on coefficients
+    (the same w_j as the plan's quadrature, 1/(4B^3) sin(beta_j) s_j)."""
+
+    def __init__(self, plan):
+        import torch
+        from . import ridx
+        from .density import beta_weights
+        self.plan = plan
+        B, n2, nx = plan.B, 2 * plan.B, plan.nx
+        _, w = beta_weights(B)
+        rdt = plan.rdtype
+        # grid order a + 2B (j + 2B (g + 2B x)): view (nx, g, j, a)
+        self.w = torch.tensor(w, dtype=rdt, device="cuda").view(1, 1, n2, 1)
+        ell = np.zeros(plan.nrc)
+        for l in range(B):
+            for m in range(-l, l + 1):
+                for n in range(-l, l + 1):
+                    ell[ridx(B, l, m, n)] = 2 * l + 1
+        self.lfac = torch.tensor(ell / nx, dtype=rdt, device="cuda").view(-1, 1)  # (2l + 1) / nx per row
+        self.gshape = (nx, n2, n2, n2)
+
+    def backward_adjoint(self, g):
+        """B^H g for a real grid array g (the plan's grid size)."""
+        F = self.plan.forward((g.view(self.gshape) / self.w).reshape(-1).contiguous())
+        return (F.view(self.plan.nrc, -1) * self.lfac).reshape(-1)
+
+    def forward_adjoint(self, Fbar):
+        """A^H Fbar for coefficients Fbar (the plan's coefficient count)."""
+        y = (Fbar.view(self.plan.nrc, -1) / self.lfac).reshape(-1).contiguous()
+        return (self.plan.backward(y).view(self.gshape) * self.w).reshape(-1)

@@ -165,3 +165,57 @@ def test_autograd_function():
     assert float((Fa.grad - gF0).abs().max()) <= 1e-12 * float(gF0.abs().max())
     assert np.allclose(b.grad.numpy(), g["b"], rtol=1e-12, atol=0)
     assert np.allclose(G.grad.numpy(), g["G"], rtol=1e-12, atol=0)
+
+
+def _rand_real(n, dtype, seed):
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    return torch.randn(n, generator=g, dtype=torch.float64).to(dtype).cuda()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_transform_adjoint_identities(dim):
+    """<B F, g> = Re<F, B^H g> and Re<A f, Fbar> = <f, A^H Fbar> (FP64)."""
+    from hpup.adjoint import TransformAdjoint
+    _print("transform adjoints", dim)
+    p = hpup.Plan(4, dim, 4, L, precision="fp64")
+    ta = TransformAdjoint(p)
+    F = _rand(p.ncoef, p.cdtype, 13)
+    g = _rand_real(p.ngrid, p.rdtype, 14)
+    f = _rand_real(p.ngrid, p.rdtype, 15)
+    Fb = _rand(p.ncoef, p.cdtype, 16)
+    Bf = p.backward(F)
+    lhs, rhs = float((Bf * g).sum()), _dot(F, ta.backward_adjoint(g)).real
+    assert abs(lhs - rhs) <= 1e-12 * float(Bf.norm() * g.norm()), (lhs, rhs)
+    AF = p.forward(f)
+    lhs, rhs = _dot(AF, Fb).real, float((f * ta.forward_adjoint(Fb)).sum())
+    assert abs(lhs - rhs) <= 1e-12 * float(AF.norm() * Fb.norm()), (lhs, rhs)
+
+
+def test_grid_objective_gradient():
+    """J = <B(F_K), g> on the grid density after K RK4 steps: the harness fed
+    with B^H g against central differences in b and the exact F0 change."""
+    from hpup.adjoint import TransformAdjoint
+    _print("grid objective gradient")
+    K, dt, h = 4, 0.02, 1e-4
+    gr = Gradient(B, N, L, b=BT, G=G2, dt=dt, method="rk4")
+    ta = TransformAdjoint(gr.fwd)
+    F0 = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 17)
+    g = _rand_real(gr.fwd.ngrid, gr.fwd.rdtype, 18)
+    lam = ta.backward_adjoint(g)
+    _, gF0, grads = gr.run(F0, K, lambda F: lam, checkpoint=2)
+
+    def J(b, F=F0):
+        p = _plan(b=b, dt=dt, method="rk4")
+        return float((p.backward(p.step(F.clone(), K)) * g).sum())
+
+    J0 = J(BT)
+    for t in range(2):
+        bp, bm = list(BT), list(BT)
+        bp[t] += h
+        bm[t] -= h
+        fd = (J(bp) - J(bm)) / (2 * h)
+        assert abs(grads["b"][t] - fd) <= 2e-7 * max(abs(fd), 1e-3 * abs(J0)), (t, grads["b"][t], fd)
+    d = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 19)
+    dJ = J(BT, F0 + d) - J0
+    assert abs(dJ - _dot(gF0, d).real) <= 1e-10 * abs(dJ)
