@@ -171,3 +171,53 @@ class TransformAdjoint:
         """A^H Fbar for coefficients Fbar (the plan's coefficient count)."""
         y = (Fbar.view(self.plan.nrc, -1) / self.lfac).reshape(-1).contiguous()
         return (self.plan.backward(y).view(self.gshape) * self.w).reshape(-1)
+
+
+def transform_functions(plan):
+    """torch.autograd.Function pair over the plan's transforms: Forward(f) -> F
+    (backward through A^H) and Backward(F) -> f (backward through B^H)."""
+    import torch
+    ta = TransformAdjoint(plan)
+
+    class Forward(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, f):
+            return plan.forward(f.detach().contiguous())
+
+        @staticmethod
+        def backward(ctx, gF):
+            return ta.forward_adjoint(gF.contiguous())
+
+    class Backward(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, F):
+            return plan.backward(F.detach().contiguous())
+
+        @staticmethod
+        def backward(ctx, gf):
+            return ta.backward_adjoint(gf.contiguous())
+
+    return Forward, Backward
+
+
+def mfg_density(B, N, L, s, U, mu, sig):
+    """Differentiable (torch, float64, on the GPU) matrix-Fisher x Gaussian grid
+    density of the reduced model: f(R, x) proportional to
+    exp(tr((U diag(s))^T R)) exp(-sum_p ((x_p - mu_p) / sig_p)^2 / 2), with unit
+    mass under the grid quadrature (weights w_j, cells (L/N)^2).  s (3,), mu (2,)
+    and sig (2,) may require gradients; U is a fixed 3 x 3 array."""
+    import torch
+    from .density import beta_weights, rotation_grid
+    R = rotation_grid(B)
+    Ut = torch.as_tensor(np.asarray(U, dtype=np.float64), device="cuda")
+    expo = ((Ut * s.view(1, 3)).view(3, 3, 1) * R).sum(dim=(0, 1))
+    fR = torch.exp(expo - expo.max().detach())
+    _, w = beta_weights(B)
+    wj = torch.as_tensor(w, device="cuda").view(1, 2 * B, 1).expand(2 * B, 2 * B, 2 * B).reshape(-1)
+    ax = -L / 2 + L / N * torch.arange(N, device="cuda", dtype=torch.float64)
+    x1 = ax.view(1, N).expand(N, N).reshape(-1)  # first omega index fastest
+    x2 = ax.view(N, 1).expand(N, N).reshape(-1)
+    q = ((x1 - mu[0]) / sig[0]) ** 2 + ((x2 - mu[1]) / sig[1]) ** 2
+    fx = torch.exp(-0.5 * q)
+    mass = (fR * wj).sum() * fx.sum() * (L / N) ** 2
+    return torch.outer(fx, fR).reshape(-1) / mass
