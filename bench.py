@@ -98,6 +98,37 @@ def timed_steps(plan, F, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
+def gradient_row(B=15, N=30, L=10.0, K=10, precision="fp64"):
+    """Cost of the gradient sweep (hpup.adjoint, reduced model, RK4) against a
+    forward step on a B = 15, 30^2 omega grid: per step of the sweep beyond the
+    forward pass, with a checkpoint every step and every 5 steps."""
+    import numpy as np
+    import torch
+    from hpup.adjoint import Gradient
+    gr = Gradient(B, N, L, b=(0.2, 0.2), G=np.eye(2) * 0.1, dt=0.004, method="rk4", precision=precision)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    F0 = (torch.randn(gr.fwd.ncoef, generator=g, dtype=torch.complex128) * 1e-3).to(gr.fwd.cdtype).cuda()
+    w = torch.randn(gr.fwd.ncoef, generator=g, dtype=torch.complex128).to(gr.fwd.cdtype).cuda()
+    F = F0.clone()
+    gr.fwd.step(F, 3)
+    gr.run(F0, 2, lambda F: w, checkpoint=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F = F0.clone()
+    gr.fwd.step(F, K)
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / K
+    out = {"config": "reduced model, B=%d, %d^2 omega grid, RK4, %s, %d steps" % (B, N, precision, K),
+           "coefficients": gr.fwd.ncoef, "forward_step_ms": tf * 1e3}
+    for cp in (1, 5):
+        t0 = time.perf_counter()
+        gr.run(F0, K, lambda F: w, checkpoint=cp)
+        torch.cuda.synchronize()
+        tg = (time.perf_counter() - t0) / K
+        out["sweep_over_forward_checkpoint_%d" % cp] = (tg - tf) / tf
+    return out
+
+
 def row_timings(plan, F, reps=3):
     """The other rows of the path on this workload, timed with HIP events on
     torch's current stream (the plan's stream): inverse transform (coefficients
@@ -255,6 +286,13 @@ def main():
                 log("rows %s: backward %.2f ms, forward %.2f ms, get_stat %.2f ms"
                     % (tag, rows[tag]["transform_backward"]["ms"], rows[tag]["transform_forward"]["ms"],
                        rows[tag]["get_stat_ms"]))
+        try:
+            rows["gradient"] = gradient_row()
+            log("rows gradient: sweep / forward step %.2f (checkpoint 1), %.2f (checkpoint 5)"
+                % (rows["gradient"]["sweep_over_forward_checkpoint_1"],
+                   rows["gradient"]["sweep_over_forward_checkpoint_5"]))
+        except Exception as e:  # an optional row must not hide the headline result
+            log("gradient row failed:", repr(e))
 
     cpu = None
     if not args.no_cpu:
