@@ -1,7 +1,10 @@
 """Pins the FP64 oracle (oracle/refcpu.c) against facts the mathematics fixes,
 computed by independent routes: sympy's Wigner-d and Clebsch-Gordan, numpy's
 FFT, quadrature exactness, the SO(3) product and derivative rules checked on
-the grid by finite differences, and invariants of the reset map.  The
+the grid by finite differences, the Taylor polynomial of the linear drift
+for the integrators, scipy's Gaussian density and bilinear reproduction for
+the reset's transition kernel and interpolation rule, and invariants of the
+reset map.  The
 reference ships no golden vectors for this path, so these are the anchors."""
 import math
 
@@ -9,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import density, omega_grid, rel_l2
+from helpers import density, omega_grid, operators, rel_l2
 
 
 def _bandlimited(B, deg, rng, nx_shape=()):
@@ -360,6 +363,32 @@ def test_derivative_conserves_mass_and_matches_gravity_product():
     assert rel_l2(dG, pred) < 1e-12
 
 
+# ---------------------------------------------------------- integrators ----
+
+@pytest.mark.parametrize("dim,method,order", [(2, "euler", 1), (2, "midpoint", 2), (2, "RK2", 2),
+                                              (2, "RK4", 4), (3, "RK4", 4)])
+def test_integrators_equal_the_taylor_polynomial_of_the_drift(dim, method, order):
+    """The drift F -> dF is linear and autonomous, so one step of an s-stage
+    explicit Runge-Kutta method of order s <= 4 is exactly
+    sum_{k<=s} (dt A)^k F / k!: the order conditions fix every coefficient of
+    the stability polynomial, and a wrong Butcher weight or node leaves a
+    dt^k term with the wrong coefficient.  A^k F comes from the oracle's
+    derivative, which is pinned on its own."""
+    B, N, L = 3, (6 if dim == 2 else 4), 6.0
+    F = _F_of(B, dim, N, L, 7)
+    X, OJO, MR, u, cells = operators(B, dim, N, L, np.diag([1.0, 0.8, 0.3]) if dim == 3 else None)
+    b = np.array([0.3, 0.2, 0.1])[:dim]
+    G = np.diag([0.1, 0.2, 0.15])[:dim, :dim] + 0.02 * (1 - np.eye(dim))
+    dt = 0.05
+    pred, term, last = F.copy(), F, None
+    for k in range(1, order + 1):
+        term = O.derivative(B, dim, N, term, X, OJO, MR, b, G, L, u, cells) * (dt / k)
+        pred, last = pred + term, term
+    Fn = O.integrate(B, dim, N, F, X, OJO, MR, b, G, dt, L, u, cells, method)
+    assert rel_l2(Fn, pred) < 1e-13
+    assert np.linalg.norm(last) > 1e-6 * np.linalg.norm(F)  # the highest term is resolved
+
+
 # ---------------------------------------------------------------- reset ----
 
 def _reset_setup():
@@ -418,3 +447,71 @@ def test_noisy_reset_conserves_mass_per_rotation():
     mask = np.ones(8 * B ** 3, bool)
     mask[indR] = False
     assert np.all(s[mask] == 0) and np.abs(df).max() > 0
+
+
+def test_index_rule_is_bilinear_interpolation():
+    """getIndRule's rule (getIndRule.cu:50-90) reproduces every bilinear
+    function a + b x1 + c x2 + d x1 x2 at the query point, its four indices are
+    the corners (x1, x2), (x1 + dx, x2), (x1, x2 + dx), (x1 + dx, x2 + dx) of the
+    grid cell holding the query, its weights sum to one, and a NaN query gives
+    the NaN rule."""
+    _, N, L, _, x = _reset_setup()
+    rng = np.random.default_rng(21)
+    xl = np.asfortranarray(x).reshape(2, -1, order="F")
+    dx = L / N
+    numR = 4
+    Om = np.asfortranarray(rng.uniform(xl.min(), xl.max() - 1e-9, size=(2, numR, N, N)))
+    Om[:, 1, 2, 3] = np.nan
+    ind, coeff = O.getIndRule(N, x, Om)
+    Omf = Om.reshape(2, numR, -1, order="F")
+    indf = ind.reshape(4, numR, -1, order="F")
+    cf = coeff.reshape(4, numR, -1, order="F")
+    g = lambda p: 0.3 + 1.1 * p[0] - 0.7 * p[1] + 0.45 * p[0] * p[1]
+    for k in range(numR):
+        for ix in range(N * N):
+            q = Omf[:, k, ix]
+            if np.isnan(q[0]):
+                assert np.all(np.isnan(cf[:, k, ix]))
+                continue
+            pts = xl[:, indf[:, k, ix]]
+            assert np.allclose(pts[:, 1] - pts[:, 0], [dx, 0], atol=1e-12)
+            assert np.allclose(pts[:, 2] - pts[:, 0], [0, dx], atol=1e-12)
+            assert np.allclose(pts[:, 3] - pts[:, 0], [dx, dx], atol=1e-12)
+            assert np.all(pts[:, 0] <= q + 1e-12) and np.all(q <= pts[:, 3] + 1e-12)
+            assert abs(cf[:, k, ix].sum() - 1) < 1e-13
+            assert abs(np.dot(cf[:, k, ix], g(pts)) - g(q)) < 1e-12
+
+
+def test_transition_kernel_keeps_the_densest_sources_and_conserves_outflow():
+    """For each destination getFcL (getFcL.cu:11-166) keeps the nD sources of
+    largest Gaussian density N(x_dest; Omega_src, Gd), evaluated here with
+    scipy.stats, in descending order; each kept weight is that density over the
+    sum of the source's kept densities, times lambda(source), so the kept
+    weights of every source sum to lambda(source)."""
+    from scipy.stats import multivariate_normal
+    _, N, L, _, x = _reset_setup()
+    rng = np.random.default_rng(22)
+    xl = np.asfortranarray(x).reshape(2, -1, order="F")
+    nx, numR, nD = N * N, 3, 8
+    Om = np.asfortranarray(xl[:, None, :] * 0.9 + rng.uniform(-0.1, 0.1, size=(2, numR, nx))).reshape(
+        2, numR, N, N, order="F")
+    lam = np.asfortranarray(rng.uniform(1.0, 3.0, size=(numR, nx)))
+    Gd = np.array([[0.09, 0.02], [0.02, 0.05]])
+    fcL, idx = O.getFcL(N, x, Om, lam, Gd, nD)
+    fcL = fcL.reshape(nD, numR, nx, order="F")
+    idx = idx.reshape(nD, numR, nx, order="F").astype(np.int64)
+    Omf = Om.reshape(2, numR, nx, order="F")
+    for k in range(numR):
+        dens = np.array([[multivariate_normal.pdf(xl[:, d], mean=Omf[:, k, s], cov=Gd) for s in range(nx)]
+                         for d in range(nx)])  # (destination, source)
+        ratio = {}
+        for d in range(nx):
+            top = np.argsort(-dens[d], kind="stable")[:nD]
+            assert np.array_equal(idx[:, k, d], top)
+            for t, s in enumerate(top):
+                ratio.setdefault(s, []).append(fcL[t, k, d] / (lam[k, s] * dens[d, s]))
+        for s, r in ratio.items():
+            r = np.array(r)
+            assert np.abs(r / r[0] - 1).max() < 1e-12  # one normalizer per source
+            kept = fcL[:, k, :][idx[:, k, :] == s].sum()
+            assert abs(kept - lam[k, s]) < 1e-12 * lam[k, s]
