@@ -389,6 +389,70 @@ def test_integrators_equal_the_taylor_polynomial_of_the_drift(dim, method, order
     assert np.linalg.norm(last) > 1e-6 * np.linalg.norm(F)  # the highest term is resolved
 
 
+def _den_case():
+    B, N, L = 4, 6, 6.0
+    F = _F_of(B, 2, N, L, 9)
+    X, _, MR, u, cells = operators(B, 2, N, L)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    b, G = np.array([0.2, 0.3]), np.array([[0.1, 0.02], [0.02, 0.2]])
+    A = lambda Y: O.derivative(B, 2, N, Y, X, None, MR, b, G, L, u, cells)
+    D = lambda Y, f: O.derivative_den(B, N, Y, f, X, O.grid_mR(B), b, G, L, u, d, w)
+    return B, N, L, F, X, u, d, w, b, G, A, D
+
+
+def test_density_form_derivative_matches_the_spectral_derivative():
+    """pendulum_reduced_den.m takes the gravity term from the grid density:
+    with f the inverse transform of F it equals the spectral (Clebsch-Gordan)
+    derivative exactly, because the grid product mR f has degree <= B and the
+    forward transform projects it exactly; for any other grid f the derivative
+    is that plus the gravity term of f - f_F alone, which for a grid function of
+    degree <= B - 2 is the pinned spectral gravity term of its transform."""
+    B, N, L, F, X, u, d, w, b, G, A, D = _den_case()
+    fF = O.fft_backward(B, F, d, 2, N)
+    aF = A(F)
+    assert rel_l2(D(F, fF), aF) < 1e-12
+    assert rel_l2(D(F, None), aF) < 1e-12
+    rng = np.random.default_rng(23)
+    g = O.fft_backward(B, O.fft_forward(B, O.fft_backward(B, _bandlimited(B, B - 2, rng, (N, N)), d, 2, N),
+                                        d, w, 2, N), d, 2, N)
+    grav = O.derivative(B, 2, N, O.fft_forward(B, g, d, w, 2, N), np.zeros_like(X), None, O.build_MR(B, 2),
+                        np.zeros(2), np.zeros((2, 2)), L, u, O.cg_table(B))
+    assert np.linalg.norm(grav) > 1e-3 * np.linalg.norm(aF)
+    assert rel_l2(D(F, fF + g), aF + grav) < 1e-12
+
+
+@pytest.mark.parametrize("method", ["euler", "midpoint", "RK2", "RK4"])
+def test_density_form_integrators_use_the_grid_density_in_the_first_stage_only(method):
+    """pendulum_reduced_den.m's integrators evaluate the first stage with the
+    caller's grid density and every later stage from the stage coefficients.
+    With k1 that first derivative, a = A F and A the (linear) spectral drift,
+    composing the stages gives, for any f,
+      euler     F + dt k1
+      midpoint  F + dt a + dt^2/2 A k1
+      RK2       F + dt/2 (k1 + a) + dt^2/2 A k1
+      RK4       F + dt/6 (k1 + 5a) + dt^2/6 A (k1 + 2a) + dt^3/12 A^2 (k1 + a) + dt^4/24 A^3 k1
+    which reduces to the Taylor polynomial of A when f is the inverse transform of F."""
+    B, N, L, F, X, u, d, w, b, G, A, D = _den_case()
+    dt = 0.05
+    fF = O.fft_backward(B, F, d, 2, N)
+    rng = np.random.default_rng(24)
+    for f in (fF, fF + 0.3 * np.abs(fF).max() * rng.standard_normal(fF.shape)):
+        k1, a = D(F, f), A(F)
+        if method == "euler":
+            pred = F + dt * k1
+        elif method == "midpoint":
+            pred = F + dt * a + dt ** 2 / 2 * A(k1)
+        elif method == "RK2":
+            pred = F + dt / 2 * (k1 + a) + dt ** 2 / 2 * A(k1)
+        else:
+            Ak1, Aa = A(k1), A(a)
+            pred = (F + dt / 6 * (k1 + 5 * a) + dt ** 2 / 6 * (Ak1 + 2 * Aa)
+                    + dt ** 3 / 12 * A(Ak1 + Aa) + dt ** 4 / 24 * A(A(Ak1)))
+        out = O.integrate_den(B, N, F, f, X, O.grid_mR(B), b, G, dt, L, u, d, w, method)
+        assert rel_l2(out, pred) < 1e-13
+
+
 # ---------------------------------------------------------------- reset ----
 
 def _reset_setup():
