@@ -281,7 +281,7 @@ struct PlanBase {
     virtual void step(void* F, int nsteps, hipStream_t st) = 0;
     virtual void derivative(const void* F, void* dF, hipStream_t st) = 0;
     virtual void adjoint_stage(const void* ybar, const void* y, double scale, void* out, void* fbar,
-                               double* gout, hipStream_t st) = 0;
+                               int fbar_init, double* gout, hipStream_t st) = 0;
     virtual void axpy_stage(const void* in, const void* F0, double scale, void* out, hipStream_t st) = 0;
     virtual void full_to_compact(const void* a, void* b, hipStream_t st) = 0;
     virtual void compact_to_full(const void* a, void* b, hipStream_t st) = 0;
@@ -509,11 +509,11 @@ template <class T> struct PlanT : PlanBase {
         HPUP_HIP(hipGetLastError());
     }
     // One reverse-mode stage of the gradient sweep on an adjoint plan:
-    // out = scale A^H ybar, fbar += out (when fbar is given), gout[0..4] +=
-    // scale Re<ybar, (dA/dtheta) y> for theta = b_1, b_2, G_00, G_11 and G_01
-    // (= the G_10 term).
-    void adjoint_stage(const void* ybar, const void* y, double scale, void* out, void* fbar, double* gout,
-                       hipStream_t st) override {
+    // out = scale A^H ybar, fbar += out (when fbar is given; fbar = ybar + out
+    // with fbar_init), gout[0..4] += scale Re<ybar, (dA/dtheta) y> for
+    // theta = b_1, b_2, G_00, G_11 and G_01 (= the G_10 term).
+    void adjoint_stage(const void* ybar, const void* y, double scale, void* out, void* fbar, int fbar_init,
+                       double* gout, hipStream_t st) override {
         HPUP_REQUIRE(has_drift && fused() && dr.view.adj, HPUP_ERR_UNSUPPORTED,
                      "adjoint_stage needs an adjoint operator plan (operator kind 1, N <= 32)");
         HPUP_REQUIRE(ybar && y && out && gout, HPUP_ERR_ARG, "null argument");
@@ -523,6 +523,7 @@ template <class T> struct PlanT : PlanBase {
         DriftDev<T> D = dr.view;
         D.gy = y;
         D.gacc = fbar;
+        D.gacc_init = fbar_init ? 1 : 0;
         D.gpart = gpart.as<double>();
         D.gout = gout;
         D.gscale = scale;
@@ -761,8 +762,8 @@ int hpup_plan_axpy_stage(hpup_plan* p, const void* in, const void* F0, double sc
     return guard([&] { p->impl->axpy_stage(in, F0, scale, out, (hipStream_t)stream); });
 }
 int hpup_plan_adjoint_stage(hpup_plan* p, const void* ybar, const void* y, double scale, void* ybar_out,
-                            void* fbar, double* grad, void* stream) {
-    return guard([&] { p->impl->adjoint_stage(ybar, y, scale, ybar_out, fbar, grad, (hipStream_t)stream); });
+                            void* fbar, int fbar_init, double* grad, void* stream) {
+    return guard([&] { p->impl->adjoint_stage(ybar, y, scale, ybar_out, fbar, fbar_init, grad, (hipStream_t)stream); });
 }
 int hpup_plan_full_to_compact(hpup_plan* p, const void* a, void* b, void* stream) {
     return guard([&] { p->impl->full_to_compact(a, b, (hipStream_t)stream); });

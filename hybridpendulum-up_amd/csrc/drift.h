@@ -43,6 +43,7 @@ template <class T> struct DriftDev {
     // out[5] += gscale * sums
     const void* gy;
     void* gacc;      // nullable: gacc += the stage output
+    int gacc_init;   // gacc = the stage input + the stage output instead (the sweep's first stage)
     double* gpart;
     double* gout;
     double gscale;

@@ -594,11 +594,16 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
             for (int v = 0; v < 4; v++)
                 if (ok[v]) st_once<T>(at_b(Fout + base, offb[v]), cscale<T>(q[v], sc.dt));
             if constexpr (GRAD) {
-                if (D.gacc) {  // the sweep's running sum: gacc += dt q
+                if (D.gacc) {  // the sweep's running sum: gacc += dt q (or input + dt q)
                     cT<T>* A = (cT<T>*)D.gacc + base;
                     cT<T> av[4];
+                    if (D.gacc_init) {
 #pragma unroll
-                    for (int v = 0; v < 4; v++) av[v] = ld_once<T>(at_b(A, offb[v]));
+                        for (int v = 0; v < 4; v++) av[v] = fo[v];  // the stage input, already loaded
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; v++) av[v] = ld_once<T>(at_b(A, offb[v]));
+                    }
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         av[v] = cadd<T>(av[v], cscale<T>(q[v], sc.dt));
@@ -669,13 +674,29 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
     }
 }
 
-// out[i] += scale * sum over the workgroups' partials (fixed order)
-__global__ void k_grad_reduce(const double* __restrict__ part, int nwg, double scale, double* __restrict__ out) {
-    const int i = threadIdx.x;
-    if (i >= 5) return;
-    double s = 0.0;
-    for (int g = 0; g < nwg; g++) s += part[(int64_t)g * 5 + i];
-    out[i] += scale * s;
+// out[i] += scale * sum over the workgroups' partials, one workgroup of
+// kGradRedThreads: thread t sums partials t, t + kGradRedThreads, ... in that
+// order, then a fixed pairwise tree combines the threads, so the result does
+// not depend on timing (the partials are independent loads, all in flight)
+constexpr int kGradRedThreads = 256;
+__global__ __launch_bounds__(kGradRedThreads) void k_grad_reduce(const double* __restrict__ part, int nwg, double scale,
+                                                                 double* __restrict__ out) {
+    __shared__ double red[5][kGradRedThreads];
+    const int t = threadIdx.x;
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int g = t; g < nwg; g += kGradRedThreads)
+#pragma unroll
+        for (int i = 0; i < 5; i++) s[i] += part[(int64_t)g * 5 + i];
+#pragma unroll
+    for (int i = 0; i < 5; i++) red[i][t] = s[i];
+    __syncthreads();
+    for (int h = kGradRedThreads / 2; h > 0; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int i = 0; i < 5; i++) red[i][t] += red[i][t + h];
+        __syncthreads();
+    }
+    if (t < 5) out[t] += scale * red[t][0];
 }
 
 // ------------------------------------------------------------ axis 3 ----
@@ -857,6 +878,8 @@ static unsigned launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const 
                                hipStream_t st) {
     auto kern = k_drift_fused<T, G, DIM, GRAV, ADJ, GRAD>;
     const unsigned grid = persistent_grid(kern, nslab);
+    // the gradient variant writes five partials per workgroup
+    HPUP_REQUIRE(!GRAD || grid <= (unsigned)kGradMaxWG, HPUP_ERR_UNSUPPORTED, "gradient stage: partial buffer too small");
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, D, MX, (const cT<T>*)Fin, (const cT<T>*)Z3,
                        (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
     return grid;
@@ -891,8 +914,7 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
             HPUP_REQUIRE(D.gy && sc.deriv_only, HPUP_ERR_ARG, "gradient stage needs the forward stage value");
             const unsigned nwg = launch_fused_t<T, GemmF<T, true>, 2, false, true, true>(D, MX, Fin, Z3, F0, Fout, ACC,
                                                                                        sc, nslab, st);
-            HPUP_REQUIRE(nwg <= kGradMaxWG, HPUP_ERR_UNSUPPORTED, "gradient stage: partial buffer too small");
-            hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(64), 0, st, (const double*)D.gpart, (int)nwg,
+            hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(kGradRedThreads), 0, st, (const double*)D.gpart, (int)nwg,
                                (double)D.gscale, D.gout);
             return;
         }

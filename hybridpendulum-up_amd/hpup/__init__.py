@@ -106,7 +106,8 @@ def lib():
         L.hpup_plan_axpy_stage.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                            ctypes.c_void_p, ctypes.c_void_p]
         L.hpup_plan_adjoint_stage.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
-                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_void_p]
         L.hpup_plan_destroy.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_coeffs.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_grid.argtypes = [ctypes.c_void_p]
@@ -501,10 +502,11 @@ class Plan:
                                           self._chk(out, self.ncoef, self.cdtype), self._stream()))
         return out
 
-    def adjoint_stage(self, ybar, y, scale, grad, out=None, fbar=None):
+    def adjoint_stage(self, ybar, y, scale, grad, out=None, fbar=None, fbar_init=False):
         """Adjoint plans (operator=1): out = scale A^H ybar, fbar += out (if
-        given) and, on the device, grad (float64, 5) += scale Re<ybar, dA/dtheta y>
-        for theta = b_1, b_2, G_00, G_11, G_01 (= G_10 term)."""
+        given; fbar = ybar + out with fbar_init) and, on the device, grad
+        (float64, 5) += scale Re<ybar, dA/dtheta y> for theta = b_1, b_2, G_00,
+        G_11, G_01 (= G_10 term)."""
         import torch
         out = self.empty_coeffs() if out is None else out
         if not (grad.is_cuda and grad.dtype == torch.float64 and grad.numel() == 5 and grad.is_contiguous()):
@@ -513,7 +515,8 @@ class Plan:
                                              self._chk(y, self.ncoef, self.cdtype), float(scale),
                                              self._chk(out, self.ncoef, self.cdtype),
                                              self._chk(fbar, self.ncoef, self.cdtype) if fbar is not None else None,
-                                             ctypes.c_void_p(grad.data_ptr()), self._stream()))
+                                             1 if fbar_init else 0, ctypes.c_void_p(grad.data_ptr()),
+                                             self._stream()))
         return out
 
     @property

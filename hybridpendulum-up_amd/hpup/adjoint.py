@@ -55,12 +55,13 @@ class Gradient:
         """Adjoint of one step taken from F; returns Fbar.  acc (device,
         float64, 5) accumulates the b_1, b_2, G_00, G_11, G_01 terms: each
         stage is one fused kernel (A^H ybar and the five reductions)."""
+        import torch
         ys = self._stages(F)
         c = HORNER[self.method]
-        Fbar = lam.clone()
+        Fbar = torch.empty_like(lam)  # the last stage starts it at lam
         ybar = lam
         for j in range(len(c) - 1, -1, -1):
-            ybar = self.adj.adjoint_stage(ybar, ys[j], c[j] * self.dt, acc, fbar=Fbar)
+            ybar = self.adj.adjoint_stage(ybar, ys[j], c[j] * self.dt, acc, fbar=Fbar, fbar_init=j == len(c) - 1)
         return Fbar
 
     def run(self, F0, nsteps, dJ, checkpoint=None):

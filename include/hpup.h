@@ -282,12 +282,14 @@ int hpup_plan_axpy_stage(hpup_plan* plan, const void* in, const void* F0, double
  * and forward stage value y (= y_{j-1}), all device arrays of the plan's
  * coefficient count:
  *   ybar_out = scale * A^H ybar,   fbar += ybar_out (fbar nullable: the sweep's
- *   running sum of stage adjoints),
+ *   running sum of stage adjoints), or fbar = ybar + ybar_out when fbar_init
+ *   is nonzero (a step's first reverse stage, whose running sum starts at its
+ *   incoming adjoint),
  *   grad[0..4] += scale * Re<ybar, (dA/dtheta) y>, theta = b_1, b_2, G_00, G_11
  *   and G_01 (the same term as G_10); grad is a device array of 5 doubles.
  * No two of ybar_out, fbar and the inputs may alias. */
 int hpup_plan_adjoint_stage(hpup_plan* plan, const void* ybar, const void* y, double scale, void* ybar_out,
-                            void* fbar, double* grad, void* stream);
+                            void* fbar, int fbar_init, double* grad, void* stream);
 
 /* Number of compact coefficients: nRc * nx, nRc = B(2B-1)(2B+1)/3. */
 int64_t hpup_plan_num_coeffs(const hpup_plan* plan);
