@@ -98,10 +98,10 @@ def timed_steps(plan, F, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def gradient_row(B=15, N=30, L=10.0, K=10, precision="fp64"):
+def gradient_row(B=15, N=30, L=10.0, K=20, precision="fp64", checkpoints=(1, 5, 20)):
     """Cost of the gradient sweep (hpup.adjoint, reduced model, RK4) against a
     forward step on a B = 15, 30^2 omega grid: per step of the sweep beyond the
-    forward pass, with a checkpoint every step and every 5 steps."""
+    forward pass, for checkpoints every 1, 5 and 20 steps."""
     import numpy as np
     import torch
     from hpup.adjoint import Gradient
@@ -120,7 +120,7 @@ def gradient_row(B=15, N=30, L=10.0, K=10, precision="fp64"):
     tf = (time.perf_counter() - t0) / K
     out = {"config": "reduced model, B=%d, %d^2 omega grid, RK4, %s, %d steps" % (B, N, precision, K),
            "coefficients": gr.fwd.ncoef, "forward_step_ms": tf * 1e3}
-    for cp in (1, 5):
+    for cp in checkpoints:
         t0 = time.perf_counter()
         gr.run(F0, K, lambda F: w, checkpoint=cp)
         torch.cuda.synchronize()
@@ -288,9 +288,10 @@ def main():
                        rows[tag]["get_stat_ms"]))
         try:
             rows["gradient"] = gradient_row()
-            log("rows gradient: sweep / forward step %.2f (checkpoint 1), %.2f (checkpoint 5)"
-                % (rows["gradient"]["sweep_over_forward_checkpoint_1"],
-                   rows["gradient"]["sweep_over_forward_checkpoint_5"]))
+            rows["gradient_fp32"] = gradient_row(precision="fp32")
+            for key in ("gradient", "gradient_fp32"):
+                log("rows %s: sweep / forward step %s (checkpoint 1, 5, 20)" % (key, ", ".join(
+                    "%.2f" % rows[key]["sweep_over_forward_checkpoint_%d" % cp] for cp in (1, 5, 20))))
         except Exception as e:  # an optional row must not hide the headline result
             log("gradient row failed:", repr(e))
 
