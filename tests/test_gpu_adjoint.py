@@ -140,6 +140,32 @@ def test_gradient_checkpoint_invariance():
     assert np.array_equal(p1["b"], p2["b"]) and np.array_equal(p1["G"], p2["G"])
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_adjoint_stage_running_sum_modes(precision, tol):
+    """hpup_plan_adjoint_stage: out = scale A^H ybar (the adjoint plan's
+    derivative times scale); fbar += out, or fbar = ybar + out with fbar_init;
+    the five parameter sums do not depend on the fbar mode (bit-identical)."""
+    import torch
+    _print("adjoint stage fbar modes", precision)
+    AH = _plan(precision, operator=1)
+    ybar, y = _rand(AH.ncoef, AH.cdtype, 21), _rand(AH.ncoef, AH.cdtype, 22)
+    f0 = _rand(AH.ncoef, AH.cdtype, 23)
+    scale = 0.37
+    ref = AH.derivative(ybar) * scale
+    g1 = torch.zeros(5, dtype=torch.float64, device="cuda")
+    g2 = torch.zeros(5, dtype=torch.float64, device="cuda")
+    fb = f0.clone()
+    out = AH.adjoint_stage(ybar, y, scale, g1, fbar=fb)
+    rel = lambda a, b: float((a - b).abs().pow(2).sum().sqrt() / b.abs().pow(2).sum().sqrt())
+    assert rel(out, ref) < tol
+    assert rel(fb, f0 + out) < tol
+    fi = torch.full_like(f0, float("nan"))  # every element must be written
+    out2 = AH.adjoint_stage(ybar, y, scale, g2, fbar=fi, fbar_init=True)
+    assert torch.equal(out2, out)
+    assert rel(fi, ybar + out) < tol
+    assert torch.equal(g1, g2) and float(g1.abs().sum()) > 0
+
+
 def test_operator_kind_errors():
     with pytest.raises(hpup.HpupError):
         _plan(operator=8)
