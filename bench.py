@@ -129,6 +129,29 @@ def gradient_row(B=15, N=30, L=10.0, K=20, precision="fp64", checkpoints=(1, 5, 
     return out
 
 
+def output_errors(p64, F64, pm, Fm):
+    """Each output of the path after the timed steps, from the reduced-precision
+    plan against the FP64 plan on the same initial coefficients: relative L2
+    (Frobenius) errors of the grid density, the omega marginal fx, the
+    attitude mean E[R], the angular-velocity mean and covariance, and the
+    fitted matrix-Fisher S and Gaussian Sigma of get_stat."""
+    import numpy as np
+    import torch
+    f64 = p64.backward(F64)
+    fm = pm.backward(Fm)
+    out = {"grid_density": float(torch.linalg.vector_norm(fm.to(torch.float64) - f64) / torch.linalg.vector_norm(f64))}
+    s64, sm = p64.get_stat(f64, return_fx=True), pm.get_stat(fm, return_fx=True)
+    del f64, fm
+
+    def rel(a, b):
+        a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    for key, name in (("fx", "omega_marginal"), ("ER", "attitude_mean_ER"), ("Ex", "omega_mean_Ex"),
+                      ("Varx", "omega_covariance_Varx"), ("S", "matrix_fisher_S"), ("Sigma", "gaussian_Sigma")):
+        out[name] = rel(sm[key], s64[key])
+    return out
+
+
 def row_timings(plan, F, reps=3):
     """The other rows of the path on this workload, timed with HIP events on
     torch's current stream (the plan's stream): inverse transform (coefficients
@@ -226,8 +249,8 @@ def main():
                          "three times: the known-size reference for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU sharding is "
-              "described in DESIGN.md, not implemented)" % args.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (sharding the slabs "
+              "across GPUs is not implemented)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
     cfg = dict(CFG)
@@ -253,6 +276,14 @@ def main():
         msm = timed_steps(pm, Fm, K, W)
         log("%s %.3f ms/step" % (args.precision, msm))
         err = float(torch.linalg.vector_norm((Fm.to(torch.complex128) - F64)) / torch.linalg.vector_norm(F64))
+
+    out_err = None
+    if args.precision != "fp64":
+        try:
+            out_err = output_errors(p64, F64, pm, Fm)
+            log("outputs vs fp64:", ", ".join("%s %.1e" % kv for kv in out_err.items()))
+        except Exception as e:  # an optional report must not hide the headline result
+            log("output errors failed:", repr(e))
 
     # per-kernel timing with HIP events on the stream the kernels run on
     Fp = Fm.clone()
@@ -324,6 +355,7 @@ def main():
         "fp64_steps_per_s": 1000.0 / ms64,
         "speedup_vs_fp64": ms64 / msm,
         "rel_l2_vs_fp64": err,
+        "output_errors_vs_fp64": out_err,
         "fp64_mass_drift": mass_drift,
         "roofline": roof,
         "kernels_ms": {n: t for n, t, _ in prof},

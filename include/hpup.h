@@ -10,8 +10,8 @@
  *     plain pointers and explicit dims, writing the array the gateway would
  *     return in plhs[0] into a caller-allocated buffer.  Host memory in, host
  *     memory out, synchronous, no device state kept between calls -- exactly
- *     the reference's calling convention, so a mexFunction shim is a few lines
- *     (see INTEGRATION.md).  Precision is selected by the name (`...32` takes
+ *     the reference's calling convention, so a mexFunction shim is a few
+ *     lines.  Precision is selected by the name (`...32` takes
  *     float arrays) as in the reference.
  *
  *  2. Device-resident plan API.  Tables are built once, coefficients stay in
