@@ -7,8 +7,9 @@ forward operator, whose parity with the oracle the other GPU tests hold:
 * <P x, y> = <x, P^H y> for the step of every integrator;
 * A(theta + e_t) x - A(theta) x = (dA/dtheta_t) x (A is linear in b and G);
 * the harness's gradients (reverse mode through the Horner stages) against
-  central differences of J(theta) through separately built forward plans,
-  and the F0 gradient against the exact change of the (linear) objective;
+  central differences of J(theta) through separately built forward plans
+  and through the oracle's CPU propagation, and the F0 gradient against the
+  exact change of the (linear) objective;
 * the torch.autograd.Function binding against the harness.
 
 Tolerances: FP64 1e-12 relative on the identities (roundoff of two fused
@@ -125,6 +126,57 @@ def test_gradient_against_differences(method, checkpoint):
             Gm[i, j] -= h
             ana = g["G"][i, j]
         fd = (_objective(w, F0, bp, Gp, K, method, dt) - _objective(w, F0, bm, Gm, K, method, dt)) / (2 * h)
+        _print("  term", t, ana, fd)
+        assert abs(ana - fd) <= 2e-7 * max(abs(fd), 1e-3 * abs(J0)), (t, ana, fd)
+
+
+def test_gradient_against_oracle_differences():
+    """The GPU sweep's gradients of J = Re<w, F_K> in b and G against central
+    differences of the oracle's CPU propagation (the reference's dense
+    derivative and RK4, FP64) from a matrix-Fisher x Gaussian initial density,
+    and the F0 gradient against the oracle's exact change of J along a
+    direction (J is linear in F0)."""
+    import oracle as O
+    from helpers import density, operators, rel_l2
+    _print("gradient vs oracle differences")
+    Bo, No, Lo, K, dt = 3, 6, 8.0, 4, 0.02
+    gr = Gradient(Bo, No, Lo, b=BT, G=G2, dt=dt, method="rk4")
+    f = density(Bo, 2, No, Lo, (6, 3, 1), seed=12, Sigma=np.eye(2) * 0.8)
+    d = O.wigner_table(Bo)
+    _, _, _, wq = O.grid(Bo)
+    Ffull = O.fft_forward(Bo, f, d, wq, 2, No)
+    F0 = gr.fwd.upload_full(Ffull)
+    wv = _rand(gr.fwd.ncoef, gr.fwd.cdtype, 31)
+    _, gF0, g = gr.run(F0, K, lambda F: wv, checkpoint=2)
+    wfull = gr.fwd.download_full(wv)  # zero outside the compact entries
+    X, _, MR, u, cells = operators(Bo, 2, No, Lo)
+    # the model plan's operator is the oracle's reduced-model derivative
+    ref = O.derivative(Bo, 2, No, Ffull, X, None, MR, np.asarray(BT, dtype=np.float64), G2, Lo, u, cells)
+    assert rel_l2(gr.fwd.download_full(gr.fwd.derivative(F0)), ref) < 1e-10
+
+    def J(b, G, F=Ffull):
+        for _ in range(K):
+            F = O.integrate(Bo, 2, No, F, X, None, MR, np.asarray(b, dtype=np.float64), G, dt, Lo, u, cells, "RK4")
+        return float(np.real(np.vdot(wfull.ravel(order="F"), np.asarray(F).ravel(order="F"))))
+
+    J0 = J(BT, G2)
+    dirn = gr.fwd.download_full(_rand(gr.fwd.ncoef, gr.fwd.cdtype, 32))
+    dJ = J(BT, G2, Ffull + dirn) - J0
+    gfull = gr.fwd.download_full(gF0)
+    assert abs(dJ - float(np.real(np.vdot(gfull.ravel(order="F"), dirn.ravel(order="F"))))) <= 1e-10 * abs(dJ)
+    h = 1e-4
+    for t in range(6):
+        bp, bm, Gp, Gm = list(BT), list(BT), G2.copy(), G2.copy()
+        if t < 2:
+            bp[t] += h
+            bm[t] -= h
+            ana = g["b"][t]
+        else:
+            i, j = (t - 2) % 2, (t - 2) // 2
+            Gp[i, j] += h
+            Gm[i, j] -= h
+            ana = g["G"][i, j]
+        fd = (J(bp, Gp) - J(bm, Gm)) / (2 * h)
         _print("  term", t, ana, fd)
         assert abs(ana - fd) <= 2e-7 * max(abs(fd), 1e-3 * abs(J0)), (t, ana, fd)
 
