@@ -303,8 +303,8 @@ def test_measurement_likelihood_gradient():
     """log-likelihood of synthetic (attitude, angular velocity) grid measurements
     after K RK4 steps, differentiated end to end with torch autograd through
     the density, both transforms and the propagation: gradients in the initial
-    matrix-Fisher singular values, the Gaussian mean and width, and the
-    damping, against central differences of the FP64 pipeline."""
+    matrix-Fisher singular values, the Gaussian mean and width, the damping
+    and the noise diagonal, against central differences of the FP64 pipeline."""
     import torch
     from hpup.adjoint import mfg_density, propagate_function, transform_functions
     _print("measurement likelihood gradient")
@@ -324,16 +324,18 @@ def test_measurement_likelihood_gradient():
 
     t = {k: torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for k, v in theta0.items()}
     bt = torch.tensor(BT, dtype=torch.float64, requires_grad=True)
+    Gt = torch.tensor(G2, dtype=torch.float64, requires_grad=True)
     f0 = mfg_density(B, N, L, t["s"], U, t["mu"], t["sig"])
-    FK = Prop.apply(Fwd.apply(f0), bt, torch.tensor(G2, dtype=torch.float64), gr, K, 2)
+    FK = Prop.apply(Fwd.apply(f0), bt, Gt, gr, K, 2)
     fK = Bwd.apply(FK)
     idx = torch.topk(fK.detach(), 12).indices  # synthetic measurements where the density is large
     loglik = torch.log(fK[idx]).sum()
     loglik.backward()
 
-    def value(th, b=BT):
+    def value(th, b=BT, G=G2):
         tt = {k: torch.tensor(v, dtype=torch.float64, device="cuda") for k, v in th.items()}
-        model = gr if tuple(b) == tuple(BT) else Gradient(B, N, L, b=b, G=G2, dt=dt, method="rk4")
+        same = tuple(b) == tuple(BT) and np.array_equal(G, G2)
+        model = gr if same else Gradient(B, N, L, b=b, G=G, dt=dt, method="rk4")
         with torch.no_grad():
             return float(torch.log(pipeline(tt, b, model)[idx]).sum())
 
@@ -353,3 +355,10 @@ def test_measurement_likelihood_gradient():
         bm[i] -= h
         fd = (value(theta0, bp) - value(theta0, bm)) / (2 * h)
         assert abs(float(bt.grad[i]) - fd) <= 1e-6 * max(abs(fd), 1.0), (i, float(bt.grad[i]), fd)
+    for i in range(2):  # the noise diagonal
+        Gp, Gm = G2.copy(), G2.copy()
+        Gp[i, i] += h
+        Gm[i, i] -= h
+        fd = (value(theta0, BT, Gp) - value(theta0, BT, Gm)) / (2 * h)
+        _print("   G", i, float(Gt.grad[i, i]), fd)
+        assert abs(float(Gt.grad[i, i]) - fd) <= 1e-6 * max(abs(fd), 1.0), (i, float(Gt.grad[i, i]), fd)
