@@ -152,6 +152,47 @@ def output_errors(p64, F64, pm, Fm):
     return out
 
 
+def reset_row(B=15, N=32, L=3.4, nD=10, reps=3):
+    """The hybrid reset gateways with the reduced hybrid driver's wall, contact
+    and noise parameters (d = 0.12, h = 0.2, r = 0.025, theta_t = 5 deg,
+    lambda_max = 100 t_scale, epsilon = 0.8, Gd = 0.05^2 I, nD = 10) on a
+    B = 15, 32^2 grid over [-1.7, 1.7]^2: the best of `reps` wall times per
+    call.  The gateways take and return host arrays, as the MEX functions they
+    replace do, so these times include the host copies; the kernel times come
+    from a kernel trace of the same run."""
+    import numpy as np
+    import hpup
+    from hpup.density import rotation_grid
+    n2 = 2 * B
+    R = rotation_grid(B, device="cpu").numpy().reshape(3, 3, n2, n2, n2, order="F")
+    ax = -L / 2 + L / N * np.arange(N)
+    x = np.stack(np.meshgrid(ax, ax, indexing="ij"))
+    par = dict(d=0.12, h=0.2, r=0.025, theta_t=5 * np.pi / 180, lambda_max=100.0 * 0.1177)
+    t = {}
+
+    def timed(key, fn):
+        best, out = float("inf"), None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            best = min(best, time.perf_counter() - t0)
+        t[key] = best * 1e3
+        return out
+    lam, indR, PC = timed("getLambda_ms", lambda: hpup.getLambda_mex(R, x, **par))
+    Om = timed("getOmega_ms", lambda: hpup.getOmega_mex(R, x, indR, 0.8, PC, "new"))
+    fcL, idx = timed("getFcL_ms", lambda: hpup.getFcL_mex(x, Om, lam, np.eye(2) * 0.05 ** 2, nD))
+    f = np.asfortranarray(np.random.default_rng(3).random((n2, n2, n2, N, N)))
+    timed("discrete_propagate_noise_ms", lambda: hpup.pendulum_reduced_discrete_propagate(True, f, lam, indR, fcL, idx))
+    lc, indc, lists, PCc = hpup.getLambda_mex(R, x, compact=True, **par)
+    Omo = hpup.getOmega_mex(R, x, indc, 0.8, PCc, "old")
+    ind, co = timed("getIndRule_ms", lambda: hpup.getIndRule_mex(x, Omo))
+    timed("discrete_propagate_nonoise_ms",
+          lambda: hpup.pendulum_reduced_discrete_propagate(False, f, lc, indc, lists, ind, co))
+    return dict({"config": "reduced hybrid reset, B=%d, %d^2 omega grid, nD=%d (gateway wall times, host copies "
+                           "included)" % (B, N, nD), "numR": int(len(indR)), "grid_bytes": int(f.nbytes),
+                 "fcL_bytes": int(fcL.nbytes + idx.nbytes)}, **t)
+
+
 def row_timings(plan, F, reps=3):
     """The other rows of the path on this workload, timed with HIP events on
     torch's current stream (the plan's stream): inverse transform (coefficients
@@ -325,6 +366,12 @@ def main():
                     "%.2f" % rows[key]["sweep_over_forward_checkpoint_%d" % cp] for cp in (1, 5, 20))))
         except Exception as e:  # an optional row must not hide the headline result
             log("gradient row failed:", repr(e))
+        try:
+            rows["reset"] = reset_row()
+            log("rows reset:", ", ".join("%s %s" % (k, ("%.2f" % v) if isinstance(v, float) else v)
+                                         for k, v in rows["reset"].items() if k != "config"))
+        except Exception as e:
+            log("reset row failed:", repr(e))
 
     cpu = None
     if not args.no_cpu:
