@@ -193,11 +193,27 @@ __global__ void k_disc_noise(const double* __restrict__ f, const double* __restr
     const int k = blockIdx.y;
     if (ix >= nx) return;
     const int64_t iR = indR[k];
+    const int64_t o0 = (int64_t)nD * (k + (int64_t)numR * ix);
+    // the first kMaxND sources: all indices and weights loaded, then all
+    // gathers issued, then summed in source order (one round trip each
+    // instead of one dependent pair per source); sources beyond kMaxND follow
+    // in the same order
+    int sid[kMaxND];
+    double wt[kMaxND], fv[kMaxND];
+#pragma unroll
+    for (int t = 0; t < kMaxND; t++)
+        if (t < nD) {
+            sid[t] = idx[o0 + t];
+            wt[t] = fcL[o0 + t];
+        }
+#pragma unroll
+    for (int t = 0; t < kMaxND; t++)
+        if (t < nD) fv[t] = f[iR + nR * sid[t]];
     double s = 0.0;
-    for (int t = 0; t < nD; t++) {
-        const int64_t o = t + (int64_t)nD * (k + (int64_t)numR * ix);
-        s += f[iR + nR * idx[o]] * fcL[o];
-    }
+#pragma unroll
+    for (int t = 0; t < kMaxND; t++)
+        if (t < nD) s += fv[t] * wt[t];
+    for (int t = kMaxND; t < nD; t++) s += f[iR + nR * idx[o0 + t]] * fcL[o0 + t];
     const int64_t i = iR + nR * ix;
     df[i] = s - f[i] * lam[k + (int64_t)numR * ix];
 }
