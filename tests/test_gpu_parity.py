@@ -16,7 +16,7 @@ def _print(*a):
 
 
 @pytest.mark.parametrize("B,dim,N", [(4, 0, 1), (5, 2, 4), (3, 3, 4), (10, 0, 1), (15, 0, 1), (15, 2, 4), (17, 0, 1),
-                                     (17, 2, 2)])
+                                     (17, 2, 2), (30, 0, 1), (30, 2, 2)])
 def test_fft_forward_backward(B, dim, N):
     _print("case fft", B, dim, N)
     L = 6.0
