@@ -193,6 +193,40 @@ def reset_row(B=15, N=32, L=3.4, nD=10, reps=3):
                  "fcL_bytes": int(fcL.nbytes + idx.nbytes)}, **t)
 
 
+def hybrid_size_row(B=30, N=60, L=3.4, steps=3, precision="fp64"):
+    """The reduced hybrid driver's own grid (bandlimit 30, 60^2 omega grid over
+    [-1.7, 1.7]^2): one RK4 step of the continuous part (N > 32 runs the
+    LDS-tiled FMA drift kernels) and the inverse and forward transforms
+    (2B = 60 > 32: the LDS transform kernels), device-resident, HIP events
+    on the plan's stream."""
+    import numpy as np
+    import torch
+    import hpup
+    from hpup import density as D
+    p = hpup.Plan(B, 2, N, L, b=(0.2, 0.2, 0.0), G=np.eye(2) * 0.1, dt=0.0025, method="RK4", precision=precision)
+    rng = np.random.default_rng(30)
+    fR, fx = D.matrix_fisher_gaussian(B, 2, N, L, (8, 4, 2), D.random_rotation(rng), Sigma=np.eye(2) * 0.3)
+    f = D.grid_density(fR, fx, p.rdtype)
+    del fR, fx
+    F = p.forward(f)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    p.step(F, 1)
+    torch.cuda.synchronize()
+    ev[0].record()
+    p.step(F, steps)
+    ev[1].record()
+    ev[2].record()
+    g = p.backward(F, f)
+    ev[3].record()
+    ev[4].record()
+    p.forward(g, F)
+    ev[5].record()
+    torch.cuda.synchronize()
+    return {"config": "reduced hybrid driver grid, B=%d, %d^2 omega grid, RK4 dt=0.0025, %s" % (B, N, precision),
+            "coefficients": p.ncoef, "rk4_step_ms": ev[0].elapsed_time(ev[1]) / steps,
+            "transform_backward_ms": ev[2].elapsed_time(ev[3]), "transform_forward_ms": ev[4].elapsed_time(ev[5])}
+
+
 def row_timings(plan, F, reps=3):
     """The other rows of the path on this workload, timed with HIP events on
     torch's current stream (the plan's stream): inverse transform (coefficients
@@ -366,6 +400,13 @@ def main():
                     "%.2f" % rows[key]["sweep_over_forward_checkpoint_%d" % cp] for cp in (1, 5, 20))))
         except Exception as e:  # an optional row must not hide the headline result
             log("gradient row failed:", repr(e))
+        try:
+            rows["hybrid_size"] = hybrid_size_row()
+            log("rows hybrid_size: RK4 step %.2f ms, backward %.2f ms, forward %.2f ms" % (
+                rows["hybrid_size"]["rk4_step_ms"], rows["hybrid_size"]["transform_backward_ms"],
+                rows["hybrid_size"]["transform_forward_ms"]))
+        except Exception as e:
+            log("hybrid-size row failed:", repr(e))
         try:
             rows["reset"] = reset_row()
             log("rows reset:", ", ".join("%s %s" % (k, ("%.2f" % v) if isinstance(v, float) else v)

@@ -157,6 +157,29 @@ def test_fallback_path_large_omega_grid():
     assert rel_l2(out, ref) < 1e-10
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
+def test_plan_steps_large_omega_grid(precision, tol):
+    """A device-resident plan with N > 32 (the LDS-tiled FMA kernels) through
+    two Horner-form RK4 steps, against the oracle's RK4."""
+    import torch
+    _print("case plan N=40 fallback", precision)
+    B, N, L, dt = 3, 40, 8.0, 0.01
+    f = density(B, 2, N, L, (8, 4, 2), seed=9, Sigma=np.eye(2) * 1.2)
+    d = O.wigner_table(B)
+    _, _, _, w = O.grid(B)
+    Ffull = O.fft_forward(B, f, d, w, 2, N)
+    X, _, MR, u, cells = operators(B, 2, N, L)
+    bt, G = np.array([0.2, 0.3]), np.diag([0.1, 0.15])
+    ref = Ffull
+    for _ in range(2):
+        ref = O.integrate(B, 2, N, ref, X, None, MR, bt, G, dt, L, u, cells, "RK4")
+    p = hpup.Plan(B, 2, N, L, b=(0.2, 0.3, 0.0), G=G, dt=dt, method="RK4", precision=precision)
+    F = p.upload_full(Ffull)
+    p.step(F, 2)
+    torch.cuda.synchronize()
+    assert rel_l2(p.download_full(F), ref) < tol
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5), ("bf16x3", 1e-4)])
 def test_plan_derivative_and_mass(precision, tol):
     """Device-resident plan: derivative against the oracle (plan built from the
