@@ -184,8 +184,8 @@ __global__ void k_indrule(const double* __restrict__ x, const double* __restrict
 // holds one region point k: its sources all lie in the grid row iR = indR[k]
 // (nx lines, reused from L2 by every destination).  With k across the lanes
 // instead (contiguous fcL, fcL_indx and df accesses) each lane gathers from
-// its own rotation row, and the kernel measured 1.86 ms against 1.27 ms
-// (B = 15, 32^2 grid, 6300 region points, nD = 10).
+// its own rotation row, and the kernel (one source at a time) measured
+// 1.86 ms against 1.27 ms (B = 15, 32^2 grid, 6300 region points, nD = 10).
 __global__ void k_disc_noise(const double* __restrict__ f, const double* __restrict__ lam, const int32_t* __restrict__ indR,
                              int numR, int64_t nR, int64_t nx, const double* __restrict__ fcL,
                              const int16_t* __restrict__ idx, int nD, double* __restrict__ df) {
