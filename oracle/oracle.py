@@ -31,6 +31,8 @@ def lib():
         L.ref_integrate.restype = ctypes.c_int
         L.ref_getLambda.restype = ctypes.c_int
         L.ref_getLambda_compact.restype = ctypes.c_int
+        L.ref_derivative_structured.restype = ctypes.c_int
+        L.ref_integrate_structured.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -163,6 +165,37 @@ def integrate(B, dim, N, F, X, OJO, MR, b, G, dt, L, u, cells, method):
                              ctypes.c_double(L), _p(np.asfortranarray(u)), ptrs, method.encode(), _p(Fn))
     if rc != 0:
         raise ValueError("unknown method %r" % method)
+    return Fn
+
+
+def derivative_structured(B, dim, N, F, X, Jt, MR, b, G, L, u, cells):
+    """derivative() on the operator's nonzero support (refcpu_structured.c);
+    Jt (3-D) gives the gyroscopic term, None omits it."""
+    F = np.asfortranarray(F, dtype=np.complex128)
+    dF = np.zeros_like(F, order="F")
+    J = _p(np.asfortranarray(Jt, dtype=np.float64)) if Jt is not None else _D()
+    rc = lib().ref_derivative_structured(B, dim, N, _p(F), _p(np.asfortranarray(X)), J, _p(np.asfortranarray(MR)),
+                                         _p(np.ascontiguousarray(b, dtype=np.float64)),
+                                         _p(np.asfortranarray(G, dtype=np.float64)), ctypes.c_double(L),
+                                         _p(np.asfortranarray(u)), _cg_ptrs(cells, B), _p(dF))
+    if rc != 0:
+        raise ValueError("operator lacks the structure the structured form needs")
+    return dF
+
+
+def integrate_structured(B, dim, N, F, X, Jt, MR, b, G, dt, L, u, cells, method):
+    F = np.asfortranarray(F, dtype=np.complex128)
+    Fn = np.zeros_like(F, order="F")
+    J = _p(np.asfortranarray(Jt, dtype=np.float64)) if Jt is not None else _D()
+    rc = lib().ref_integrate_structured(B, dim, N, _p(F), _p(np.asfortranarray(X)), J, _p(np.asfortranarray(MR)),
+                                        _p(np.ascontiguousarray(b, dtype=np.float64)),
+                                        _p(np.asfortranarray(G, dtype=np.float64)), ctypes.c_double(dt),
+                                        ctypes.c_double(L), _p(np.asfortranarray(u)), _cg_ptrs(cells, B),
+                                        method.encode(), _p(Fn))
+    if rc == -2:
+        raise ValueError("unknown method %r" % method)
+    if rc != 0:
+        raise ValueError("operator lacks the structure the structured form needs")
     return Fn
 
 

@@ -45,6 +45,19 @@ int  ref_integrate(int B, int dim, int N, const double* F, const double* X,
                    const double* G, double dt, double L, const double* u,
                    const double* const* CG, const char* method, double* Fnew);
 
+/* the same derivative and integrators on the operator's exact nonzero support
+ * (refcpu_structured.c); Jt replaces OJO (the gyroscopic quadratic form is
+ * built from it), NULL for no gyroscopic term.  -1: X or MR lacks the
+ * structure, -2: unknown method. */
+int  ref_derivative_structured(int B, int dim, int N, const double* F, const double* X,
+                               const double* Jt, const double* MR, const double* b,
+                               const double* G, double L, const double* u,
+                               const double* const* CG, double* dF);
+int  ref_integrate_structured(int B, int dim, int N, const double* F, const double* X,
+                              const double* Jt, const double* MR, const double* b,
+                              const double* G, double dt, double L, const double* u,
+                              const double* const* CG, const char* method, double* Fnew);
+
 /* density-form reduced derivative and integrator (pendulum_reduced_den.m) */
 int  ref_derivative_den(int B, int N, const double* F, const double* f, const double* X,
                         const double* mR, const double* b, const double* G, double L,
