@@ -101,7 +101,7 @@ constexpr int kMaxND = 16;
 // from every source; the nD largest kept in stable descending order.
 __global__ void k_fcl_topk(const double* __restrict__ x, const double* __restrict__ Om, int numR, int64_t nx, double ig0,
                            double ig12, double ig3, double cn, int nD, double* __restrict__ fcL,
-                           int16_t* __restrict__ idx, double* __restrict__ norm) {
+                           int16_t* __restrict__ idx) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (d >= nx) return;
@@ -129,8 +129,36 @@ __global__ void k_fcl_topk(const double* __restrict__ x, const double* __restric
         const int64_t o = t + (int64_t)nD * (k + (int64_t)numR * d);
         fcL[o] = v[t];
         idx[o] = (int16_t)id[t];
-        atomicAdd(&norm[(int64_t)k * nx + id[t]], v[t]);
     }
+}
+
+// getFcL.cu:94-101: each source's normalizer, the sum of its kept densities
+// over the destinations in ascending order (and list position t within a
+// destination), the order of the reference's host loop, so the result does
+// not depend on scheduling.  One thread per source; a block stages the lists
+// of kFclChunk destinations of its region point in LDS and every thread scans
+// them for its own source.
+constexpr int kFclChunk = 64;
+__global__ void k_fcl_norm(const double* __restrict__ fcL, const int16_t* __restrict__ idx, int numR, int64_t nx,
+                           int nD, double* __restrict__ norm) {
+    __shared__ double sv[kFclChunk * kMaxND];
+    __shared__ int16_t si[kFclChunk * kMaxND];
+    const int k = blockIdx.y;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    for (int64_t d0 = 0; d0 < nx; d0 += kFclChunk) {
+        const int nd = (int)(nx - d0 < kFclChunk ? nx - d0 : kFclChunk);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nd * nD; e += blockDim.x) {
+            const int64_t o = (e % nD) + (int64_t)nD * (k + (int64_t)numR * (d0 + e / nD));
+            sv[e] = fcL[o];
+            si[e] = idx[o];
+        }
+        __syncthreads();
+        for (int e = 0; e < nd * nD; e++)
+            if (si[e] == s) acc += sv[e];
+    }
+    if (s < nx) norm[(int64_t)k * nx + s] = acc;
 }
 
 // getFcL.cu:94-101,160-166: per-source normalization, then lambda(source) dx^2.
@@ -398,9 +426,10 @@ int hpup_getFcL(const double* x, int N, const double* Omega, const double* lambd
         DevBuf xd = up(x, 2 * nx), Om = up(Omega, 2 * numR * nx), lam = up(lambda, (size_t)numR * nx),
                fc(sizeof(double) * nD * numR * nx), id(sizeof(int16_t) * nD * numR * nx),
                nrm(sizeof(double) * numR * nx);
-        HPUP_HIP(hipMemset(nrm.p, 0, nrm.bytes));
         hipLaunchKernelGGL(k_fcl_topk, grid_rx(nx, numR), dim3(128), 0, 0, xd.as<double>(), Om.as<double>(), numR, nx,
-                           ig0, ig12, ig3, cn, nD, fc.as<double>(), id.as<int16_t>(), nrm.as<double>());
+                           ig0, ig12, ig3, cn, nD, fc.as<double>(), id.as<int16_t>());
+        hipLaunchKernelGGL(k_fcl_norm, grid_rx(nx, numR), dim3(128), 0, 0, fc.as<double>(), id.as<int16_t>(), numR, nx,
+                           nD, nrm.as<double>());
         hipLaunchKernelGGL(k_fcl_final, grid_rx(nx, numR), dim3(128), 0, 0, numR, nx, nD, nrm.as<double>(),
                            lam.as<double>(), dx2, fc.as<double>(), id.as<int16_t>());
         HPUP_HIP(hipGetLastError());

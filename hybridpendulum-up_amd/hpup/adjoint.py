@@ -41,6 +41,8 @@ class Gradient:
             raise HpupError("unknown integrator %r" % method)
         self.dt = dt
         self.N = N
+        self.b = np.array(tuple(b)[:2], dtype=np.float64)
+        self.G = np.zeros((2, 2)) if G is None else np.asarray(G, dtype=np.float64).reshape(2, 2).copy()
         self.fwd = Plan(B, 2, N, **kw)
         self.adj = Plan(B, 2, N, operator=1, **kw)
 
@@ -104,13 +106,20 @@ def propagate_function():
     """A torch.autograd.Function over the propagation:
     apply(F0, b, G, grad_model, nsteps, checkpoint) -> F_K, differentiable in F0
     (complex), b (2,) and G (2, 2) (float64 CPU tensors).  grad_model is a
-    Gradient built at the same b and G (it defines the operator); the backward
-    pass calls its adjoint sweep."""
+    Gradient built at the same b and G (it defines the operator; other values
+    raise HpupError); the backward pass calls its adjoint sweep."""
     import torch
 
     class Propagate(torch.autograd.Function):
         @staticmethod
         def forward(ctx, F0, b, G, model, nsteps, checkpoint):
+            # the operator is the one model's plans were built with: b and G
+            # are the differentiation points, so they must be model's own
+            bv = np.asarray(b.detach().cpu(), dtype=np.float64).reshape(-1)
+            Gv = np.asarray(G.detach().cpu(), dtype=np.float64).reshape(2, 2)
+            if not (np.array_equal(bv, model.b) and np.array_equal(Gv, model.G)):
+                raise HpupError("b and G must equal the Gradient model's parameters (b=%s, G=%s); build a "
+                                "Gradient at the new values" % (model.b.tolist(), model.G.tolist()))
             ctx.model, ctx.nsteps, ctx.checkpoint = model, nsteps, checkpoint
             ctx.save_for_backward(F0)
             F = F0.detach().clone()

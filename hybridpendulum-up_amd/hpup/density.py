@@ -56,9 +56,13 @@ def rotation_grid(B, device="cuda", dtype=None):
 
 def matrix_fisher_gaussian(B, dim, N, L, s, U, Sigma=None, mu=None, weights=None, device="cuda"):
     """Returns (fR, fx): rotation factor ((2B)^3,) and omega factor (N^dim,) of a
-    separable density, scaled so the product has unit grid mass.  `weights`
-    mixes several components is not supported here; see mixture()."""
+    separable density, scaled so the product has unit grid mass.  A mixture
+    of several components is not separable: build it with mixture_density
+    (passing `weights` here is an error)."""
     import torch
+    if weights is not None:
+        raise ValueError("matrix_fisher_gaussian builds one separable component; use mixture_density for a "
+                         "weighted mixture")
     R = rotation_grid(B, device=device)
     US = torch.as_tensor(np.asarray(U) @ np.diag(s), device=device, dtype=torch.float64)
     expo = (US.view(3, 3, 1) * R).sum(dim=(0, 1))
@@ -88,4 +92,26 @@ def grid_density(fR, fx, dtype=None):
     """Flat grid f[R + nR*x] = fR[R] fx[x] (MATLAB order, rotation index fastest)."""
     import torch
     f = torch.outer(fx, fR).reshape(-1)
+    return f if dtype is None else f.to(dtype)
+
+
+def mixture_density(B, dim, N, L, components, weights, dtype=None, device="cuda"):
+    """Grid density f = sum_k weights[k] f_k of matrix-Fisher x Gaussian
+    components, each f_k of unit grid mass (so f has mass sum(weights)).
+    components: sequence of dicts with keys s (3 proper singular values), U
+    (3x3 rotation, F_k = U diag(s)) and optionally Sigma and mu of the
+    Gaussian in omega.  Returned flat in MATLAB order (rotation index
+    fastest), like grid_density."""
+    import torch
+    if len(components) != len(weights) or len(components) == 0:
+        raise ValueError("one weight per component")
+    f = None
+    for comp, wk in zip(components, weights):
+        fR, fx = matrix_fisher_gaussian(B, dim, N, L, comp["s"], comp["U"], Sigma=comp.get("Sigma"),
+                                        mu=comp.get("mu"), device=device)
+        if f is None:
+            f = float(wk) * torch.outer(fx, fR).reshape(-1)
+        else:
+            f.add_(torch.outer(fx, fR).reshape(-1), alpha=float(wk))
+        del fR, fx
     return f if dtype is None else f.to(dtype)

@@ -12,7 +12,8 @@ names (what their save(...) without '-v7.3' writes).  The drivers write f with
 level-5 MAT file, which MATLAB's load reads under the same variable name, when
 it fits that format's 2^31-byte limit per variable, and as f<k>.npy (numpy
 format, memory-mappable) otherwise.  Readers here take either form.
-A run resumes from the last f<k> on disk (resume_index, load_f).
+Every file is written under a temporary name and renamed into place, so a
+run resumes from the last complete f<k> on disk (resume_index, load_f).
 """
 import os
 import re
@@ -29,17 +30,32 @@ def _sio():
     return scipy.io
 
 
+def _atomic(name, write):
+    """Writes through a temporary name in the same directory and renames it
+    into place, so a run killed mid-write leaves no truncated file under the
+    final name (resume_index only ever sees complete files)."""
+    d, base = os.path.split(name)
+    tmp = os.path.join(d, ".%s.tmp%d" % (base, os.getpid()))
+    try:
+        with open(tmp, "wb") as fh:
+            write(fh)
+            fh.flush()
+            os.fsync(fh.fileno())
+        os.replace(tmp, name)
+    finally:
+        if os.path.exists(tmp):
+            os.remove(tmp)
+    return name
+
+
 def save_f(path, k, f):
     """Writes the grid density of step k as f<k> (MAT level 5, or .npy when
     f exceeds the level-5 limit); returns the file name."""
     f = np.asarray(f)
     if f.nbytes < MAT5_LIMIT:
-        name = os.path.join(path, "f%d.mat" % k)
-        _sio().savemat(name, {"f": f}, do_compression=False)
-    else:
-        name = os.path.join(path, "f%d.npy" % k)
-        np.save(name, f)
-    return name
+        return _atomic(os.path.join(path, "f%d.mat" % k), lambda fh: _sio().savemat(fh, {"f": f},
+                                                                                  do_compression=False))
+    return _atomic(os.path.join(path, "f%d.npy" % k), lambda fh: np.save(fh, f))
 
 
 def load_f(path, k, mmap=True):
@@ -57,11 +73,12 @@ def resume_index(path):
 
 
 def save_c(path, k, c):
-    _sio().savemat(os.path.join(path, "c%d.mat" % k), {"c": np.asarray(c, dtype=np.float64)})
+    _atomic(os.path.join(path, "c%d.mat" % k), lambda fh: _sio().savemat(fh, {"c": np.asarray(c, dtype=np.float64)}))
 
 
 def save_fx(path, k, fx):
-    _sio().savemat(os.path.join(path, "fx%d.mat" % k), {"fx": np.asarray(fx, dtype=np.float64)})
+    _atomic(os.path.join(path, "fx%d.mat" % k),
+            lambda fh: _sio().savemat(fh, {"fx": np.asarray(fx, dtype=np.float64)}))
 
 
 class History:
@@ -84,8 +101,8 @@ class History:
 
     def save(self, path):
         sio = _sio()
-        sio.savemat(os.path.join(path, "stat.mat"), {"stat": self.struct(STAT_FIELDS)})
-        sio.savemat(os.path.join(path, "MFG.mat"), {"MFG": self.struct(MFG_FIELDS)})
+        _atomic(os.path.join(path, "stat.mat"), lambda fh: sio.savemat(fh, {"stat": self.struct(STAT_FIELDS)}))
+        _atomic(os.path.join(path, "MFG.mat"), lambda fh: sio.savemat(fh, {"MFG": self.struct(MFG_FIELDS)}))
 
 
 def load_struct(path, name):

@@ -1,6 +1,7 @@
 """Hybrid reset gateways on the GPU against the oracle (reduced model, 2-D
 omega), with the hybrid driver's parameters (pendulum_reduced_hybrid.m:28-49).
-Indices must agree exactly; values to FP64 rounding."""
+Indices must agree exactly (a top-nD slot may hold another source only where
+two Gaussian densities tie to rounding); values to FP64 rounding."""
 import numpy as np
 import pytest
 
@@ -68,12 +69,28 @@ def test_getFcL_and_noisy_propagate(setup):
     nD = 10
     fcL, idx = hpup.getFcL_mex(x, Om, lam, Gd, nD)
     fcL_o, idx_o = O.getFcL(N, x, Om, lam, Gd, nD)
-    # identical top-nD selection except where two densities tie to rounding
-    diff = idx != idx_o
-    if diff.any():
-        assert np.abs(fcL[diff] - fcL_o[diff]).max() <= 1e-12 * np.abs(fcL_o).max()
-    assert diff.mean() < 0.01
-    assert rel_l2(np.sort(fcL, axis=0), np.sort(fcL_o, axis=0)) < 1e-12
+    # the normalizers are summed in a fixed order: a second call is bit-identical
+    fcL2, idx2 = hpup.getFcL_mex(x, Om, lam, Gd, nD)
+    assert np.array_equal(idx2, idx) and np.array_equal(fcL2.view(np.int64), fcL.view(np.int64))
+    # identical top-nD selection; a slot may hold another source only where the
+    # two candidates' Gaussian densities (numpy) tie to rounding, i.e. where
+    # the last ulp of exp decides a stable sort
+    nx, numR = N * N, len(indR)
+    xl = np.asfortranarray(x).reshape(2, nx, order="F")
+    Omf = Om.reshape(2, numR, nx, order="F")
+    Gi = np.linalg.inv(Gd)
+
+    def dens(k, dest, src):
+        dv = xl[:, dest] - Omf[:, k, src]
+        return np.exp(-0.5 * dv @ Gi @ dv)
+    diff = np.argwhere(idx != idx_o)
+    for t, k, d1, d2 in diff:
+        dest = d1 + N * d2
+        a, b = dens(k, dest, int(idx[t, k, d1, d2])), dens(k, dest, int(idx_o[t, k, d1, d2]))
+        assert abs(a - b) <= 1e-13 * max(a, b), (t, k, dest, a, b)
+    print("  top-nD slots holding a tied source: %d of %d" % (len(diff), idx.size), flush=True)
+    same = idx == idx_o
+    assert np.abs(fcL[same] - fcL_o[same]).max() <= 1e-12 * np.abs(fcL_o).max()
     f = density(B, 2, N, L, (8, 4, 2), seed=4, Sigma=np.eye(2) * 0.5)
     df = hpup.pendulum_reduced_discrete_propagate(True, f, lam, indR, fcL_o, idx_o)
     df_o = O.discrete_noise(B, N, f, lam, indR, fcL_o, idx_o)

@@ -60,3 +60,26 @@ def test_c_fx_files(tmp_path):
     S.save_fx(tmp_path, 3, fx)
     assert np.array_equal(scipy.io.loadmat(tmp_path / "c3.mat")["c"], c)
     assert np.array_equal(scipy.io.loadmat(tmp_path / "fx3.mat")["fx"], fx)
+
+
+def test_interrupted_write_leaves_no_partial_file(tmp_path, monkeypatch):
+    """A write that dies midway (here: the MAT writer raises after writing part
+    of the file) leaves neither a truncated f<k> under the final name nor its
+    temporary, so resume_index still points at the last complete step."""
+    rng = np.random.default_rng(3)
+    f1 = np.asfortranarray(rng.random((4, 4, 4, 3, 3)))
+    S.save_f(tmp_path, 1, f1)
+    real = scipy.io.savemat
+
+    def dying(fh, *a, **k):
+        fh.write(b"MATLAB 5.0 MAT-file, truncated")
+        raise OSError("disk full")
+    monkeypatch.setattr(scipy.io, "savemat", dying)
+    try:
+        S.save_f(tmp_path, 2, f1 * 2)
+    except OSError:
+        pass
+    monkeypatch.setattr(scipy.io, "savemat", real)
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["f1.mat"]
+    assert S.resume_index(tmp_path) == 1
+    assert np.array_equal(S.load_f(tmp_path, 1), f1)
