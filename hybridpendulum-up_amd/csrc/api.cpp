@@ -665,19 +665,6 @@ static PlanBase* make_plan(const hpup_model& m, const double* d, const double* w
     return p.release();
 }
 
-template <class F> int guard(F&& fn) {
-    try {
-        fn();
-        return HPUP_OK;
-    } catch (const Error& e) {
-        set_error(e.msg);
-        return e.code;
-    } catch (const std::exception& e) {
-        set_error(e.what());
-        return HPUP_ERR_ARG;
-    }
-}
-
 }  // namespace hpup
 
 using namespace hpup;

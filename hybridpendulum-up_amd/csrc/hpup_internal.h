@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,21 @@ struct Error {
     do {                                               \
         if (!(cond)) throw ::hpup::Error{(code), (msg)}; \
     } while (0)
+
+// Runs fn, turning an Error (or another exception) into its status code and
+// the thread's last-error message: the body of every extern "C" entry point.
+template <class F> int guard(F&& fn) {
+    try {
+        fn();
+        return HPUP_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return HPUP_ERR_ARG;
+    }
+}
 
 // ------------------------------------------------------------ index maths --
 

@@ -287,19 +287,6 @@ using namespace hpup;
 
 namespace {
 
-template <class F> int guard2(F&& fn) {
-    try {
-        fn();
-        return HPUP_OK;
-    } catch (const Error& e) {
-        set_error(e.msg);
-        return e.code;
-    } catch (const std::exception& e) {
-        set_error(e.what());
-        return HPUP_ERR_ARG;
-    }
-}
-
 template <class T> DevBuf up(const T* p, size_t n) {
     DevBuf b(sizeof(T) * std::max<size_t>(n, 1));
     if (n) HPUP_HIP(hipMemcpy(b.p, p, sizeof(T) * n, hipMemcpyHostToDevice));
@@ -328,7 +315,7 @@ extern "C" {
 
 int hpup_getLambda(const double* R, int B, const double* x, int N, double d, double h, double r, double theta_t,
                    double lambda_max, double* lambda, int32_t* indR, int* numR, double* PC) {
-    return guard2([&] {
+    return guard([&] {
         HPUP_REQUIRE(R && x && lambda && indR && numR && PC && B > 0 && N > 1, HPUP_ERR_ARG, "bad argument");
         const int64_t nR = 8LL * B * B * B, nx = (int64_t)N * N;
         const double theta0 = std::asin(d / std::sqrt(h * h + r * r)) - std::asin(r / std::sqrt(h * h + r * r));
@@ -356,7 +343,7 @@ int hpup_getLambda(const double* R, int B, const double* x, int N, double d, dou
 int hpup_getLambda_compact(const double* R, int B, const double* x, int N, double d, double h, double r,
                            double theta_t, double lambda_max, double* lambda, int32_t* indR, int* numR, int32_t* indx,
                            int32_t* numx, double* PC) {
-    return guard2([&] {
+    return guard([&] {
         HPUP_REQUIRE(R && x && lambda && indR && numR && indx && numx && PC && B > 0 && N > 1, HPUP_ERR_ARG,
                      "bad argument");
         const int64_t nR = 8LL * B * B * B, nx = (int64_t)N * N;
@@ -392,7 +379,7 @@ int hpup_getLambda_compact(const double* R, int B, const double* x, int N, doubl
 
 int hpup_getOmega(const double* R, int B, const double* x, int N, const int32_t* indR, int numR, double epsilon,
                   const double* PC, const char* direction, double* Omega) {
-    return guard2([&] {
+    return guard([&] {
         HPUP_REQUIRE(R && x && indR && PC && direction && Omega && numR >= 0, HPUP_ERR_ARG, "bad argument");
         int old_dir;
         if (!strcasecmp(direction, "new")) old_dir = 0;
@@ -413,7 +400,7 @@ int hpup_getOmega(const double* R, int B, const double* x, int N, const int32_t*
 
 int hpup_getFcL(const double* x, int N, const double* Omega, const double* lambda, int numR, const double* Gd, int nD,
                 double* fcL, int16_t* fcL_indx) {
-    return guard2([&] {
+    return guard([&] {
         HPUP_REQUIRE(x && Omega && lambda && Gd && fcL && fcL_indx, HPUP_ERR_ARG, "bad argument");
         HPUP_REQUIRE(nD >= 1 && nD <= kMaxND, HPUP_ERR_UNSUPPORTED, "nD must be in [1, 16]");
         const int64_t nx = (int64_t)N * N;
@@ -439,7 +426,7 @@ int hpup_getFcL(const double* x, int N, const double* Omega, const double* lambd
 }
 
 int hpup_getIndRule(const double* x, int N, const double* Omega, int numR, int32_t* ind_interp, double* coeff_interp) {
-    return guard2([&] {
+    return guard([&] {
         HPUP_REQUIRE(x && Omega && ind_interp && coeff_interp, HPUP_ERR_ARG, "bad argument");
         if (numR == 0) return;
         const int64_t nx = (int64_t)N * N;
@@ -456,7 +443,7 @@ int hpup_getIndRule(const double* x, int N, const double* Omega, int numR, int32
 
 int hpup_discrete_propagate_noise(const double* f, int B, int N, const double* lambda, const int32_t* indR, int numR,
                                   const double* fcL, const int16_t* fcL_indx, int nD, double* df) {
-    return guard2([&] {
+    return guard([&] {
         HPUP_REQUIRE(f && lambda && indR && fcL && fcL_indx && df && nD >= 1, HPUP_ERR_ARG, "bad argument");
         const int64_t nR = 8LL * B * B * B, nx = (int64_t)N * N;
         check_range(indR, numR, nR, "lambda_indR");
@@ -478,7 +465,7 @@ int hpup_discrete_propagate_noise(const double* f, int B, int N, const double* l
 int hpup_discrete_propagate_nonoise(const double* f, int B, int N, const double* lambda, const int32_t* indR, int numR,
                                     const int32_t* const* lambda_indx, const int32_t* lambda_numx,
                                     const int32_t* ind_interp, const double* coeff_interp, double* df) {
-    return guard2([&] {
+    return guard([&] {
         HPUP_REQUIRE(f && lambda && indR && df && (numR == 0 || (lambda_indx && lambda_numx && ind_interp && coeff_interp)),
                      HPUP_ERR_ARG, "bad argument");
         const int64_t nR = 8LL * B * B * B, nx = (int64_t)N * N;
