@@ -199,6 +199,35 @@ def integrate_structured(B, dim, N, F, X, Jt, MR, b, G, dt, L, u, cells, method)
     return Fn
 
 
+def reset3_tables(B, R, dwall, h, r, theta_t, lambda_max, tilt):
+    """Per-rotation rate lambda (0 outside the collision region), contact
+    point PC (3, nR) and tangent t (3, nR) of the 3-D reset."""
+    nR = 8 * B ** 3
+    lam = np.zeros(nR)
+    PC = _f64((3, nR))
+    t = _f64((3, nR))
+    lib().ref_reset3_tables(B, _p(np.asfortranarray(R)), *(ctypes.c_double(v) for v in
+                                                            (dwall, h, r, theta_t, lambda_max, tilt)),
+                            _p(lam), _p(PC), _p(t))
+    return lam, PC, t
+
+
+def reset3_pullback(Rr, t, epsilon, Om):
+    out = np.zeros(3)
+    lib().ref_reset3_pullback(_p(np.asfortranarray(Rr, dtype=np.float64)), _p(np.ascontiguousarray(t, dtype=np.float64)),
+                              ctypes.c_double(epsilon), _p(np.ascontiguousarray(Om, dtype=np.float64)), _p(out))
+    return out
+
+
+def reset3_df(B, N, L, R, f, lam, PC, t, epsilon, tilt, jacobian=True):
+    f = np.asfortranarray(f, dtype=np.float64)
+    df = np.zeros_like(f, order="F")
+    lib().ref_reset3_df(B, N, ctypes.c_double(L), _p(np.asfortranarray(R)), _p(f), _p(np.ascontiguousarray(lam)),
+                        _p(np.asfortranarray(PC)), _p(np.asfortranarray(t)), ctypes.c_double(epsilon),
+                        ctypes.c_double(tilt), 1 if jacobian else 0, _p(df))
+    return df
+
+
 def getLambda(B, N, R, x, dwall, h, r, theta_t, lambda_max):
     nR, nx = 8 * B ** 3, N * N
     lam = np.zeros(nR * nx)

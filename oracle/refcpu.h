@@ -89,6 +89,16 @@ void ref_discrete_nonoise(int B, int N, const double* f, const double* lambda,
                           const int* lambda_numx, const int* ind_interp,
                           const double* coeff_interp, double* df);
 
+/* hybrid reset on a 3-D omega grid (refcpu_reset3.c): per-rotation rate,
+ * contact point and tangent; the pulled-back omega; df on the grid */
+void ref_reset3_tables(int B, const double* R, double dwall, double h, double r, double theta_t,
+                       double lambda_max, double tilt, double* lam, double* PC, double* t);
+void ref_reset3_pullback(const double* Rr, const double* t, double epsilon, const double* Om,
+                         double* Om_old);
+void ref_reset3_df(int B, int N, double L, const double* R, const double* f, const double* lam,
+                   const double* PC, const double* t, double epsilon, double tilt, int jacobian,
+                   double* df);
+
 /* statistics on the grid */
 void ref_stat_moments(int B, int dim, int N, double L, const double* f,
                       const double* R, const double* x, const double* w,

@@ -1,0 +1,181 @@
+"""Pins of the oracle's 3-D hybrid reset (oracle/refcpu_reset3.c): the
+reference's reduced reset (getLambda.cu, getOmega.cu, getIndRule.cu,
+propagate.cu) with a full body angular velocity and a tilted wall.
+
+* with no tilt its rate and contact point are the pinned 2-D getLambda's, and
+  on Omega3 = 0 its pulled-back angular velocity is the pinned 2-D getOmega
+  'old' map;
+* with a tilt, the contact point is the point of the end disc reaching
+  furthest along the wall normal (numpy), the map obeys the law of
+  restitution vC(omega+) = -eps vC(omega-) with vC from numpy's cross product,
+  changes omega only along the tangent t, and the forward map (numpy) undoes
+  it;
+* the jump term is exact on densities trilinear in Omega (trilinear
+  interpolation reproduces them), and with the Jacobian 1/eps it conserves
+  each rotation's probability up to interpolation error.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import density, omega_grid
+
+PAR = dict(dwall=0.12, h=0.2, r=0.025, theta_t=5 * np.pi / 180, lambda_max=100.0 * 0.1173)
+EPS = 0.8
+
+
+def _ew(tilt):
+    return np.array([math.cos(tilt), 0.0, math.sin(tilt)])
+
+
+def test_no_tilt_matches_the_reduced_law():
+    B, N, L = 3, 8, 3.4
+    R = O.rotations(B)
+    x2 = omega_grid(2, N, L)
+    lam, PC, t = O.reset3_tables(B, R, tilt=0.0, **PAR)
+    lc, indR, lists, PC2 = O.getLambda_compact(B, N, R, x2, PAR["dwall"], PAR["h"], PAR["r"], PAR["theta_t"],
+                                               PAR["lambda_max"])
+    assert np.array_equal(np.nonzero(lam)[0], indR)
+    assert np.abs(lam[indR] - lc).max() < 1e-13
+    assert np.abs(PC - PC2).max() < 1e-15
+    Om_old = O.getOmega(B, N, R, x2, indR, EPS, PC2, "old")
+    Rf = R.reshape(3, 3, -1, order="F")
+    xl = np.asfortranarray(x2).reshape(2, -1, order="F")
+    Omf = Om_old.reshape(2, len(indR), -1, order="F")
+    n = 0
+    for k in range(0, len(indR), 3):
+        iR = indR[k]
+        for ix in range(0, N * N, 5):
+            if np.isnan(Omf[0, k, ix]):
+                continue
+            o3 = O.reset3_pullback(Rf[:, :, iR], t[:, iR], EPS, np.array([xl[0, ix], xl[1, ix], 0.0]))
+            assert np.abs(o3[:2] - Omf[:, k, ix]).max() < 1e-13
+            n += 1
+    assert n > 20
+
+
+@pytest.mark.parametrize("tilt", [0.0, 0.17, -0.3])
+def test_contact_point_map_restitution_and_involution(tilt):
+    B = 3
+    R = O.rotations(B)
+    lam, PC, t = O.reset3_tables(B, R, tilt=tilt, **PAR)
+    ew = _ew(tilt)
+    Rf = R.reshape(3, 3, -1, order="F")
+    rng = np.random.default_rng(7)
+    ang = np.linspace(0, 2 * np.pi, 20001)
+    h, r = PAR["h"], PAR["r"]
+    theta0 = math.asin(PAR["dwall"] / math.hypot(h, r)) - math.asin(r / math.hypot(h, r))
+    region = np.nonzero(lam)[0]
+    assert len(region) > 0
+    for iR in region[:: max(1, len(region) // 25)]:
+        Rr = Rf[:, :, iR]
+        r3 = Rr[:, 2]
+        th = math.asin(float(r3 @ ew))
+        assert th >= theta0 - PAR["theta_t"]
+        v = PC[:, iR] - h * r3
+        assert abs(v @ r3) < 1e-14 and abs(np.linalg.norm(v) - r) < 1e-14
+        a = np.cross(r3, [0.0, 1.0, 0.0])
+        a /= np.linalg.norm(a)
+        b = np.cross(r3, a)
+        circle = h * r3[:, None] + r * (np.cos(ang)[None] * a[:, None] + np.sin(ang)[None] * b[:, None])
+        assert abs((ew @ circle).max() - ew @ PC[:, iR]) < 1e-9
+        tt = np.cross(r3, ew)
+        tt /= np.linalg.norm(tt)
+        assert np.abs(tt - t[:, iR]).max() < 1e-14
+        for _ in range(4):
+            Om = rng.uniform(-3, 3, 3)
+            Oo = O.reset3_pullback(Rr, t[:, iR], EPS, Om)
+            w, wo = Rr @ Om, Rr @ Oo
+            # the forward map (collision response) undoes the pull-back
+            wp = wo - (1 + EPS) * (wo @ tt) * tt
+            assert np.abs(wp - w).max() < 1e-13
+            # only the tangential component changes, and it reverses with -eps
+            d = w - wo
+            assert np.linalg.norm(d - (d @ tt) * tt) < 1e-13
+            vc = lambda q: np.cross(q, PC[:, iR]) @ ew
+            assert abs(vc(w) + EPS * vc(wo)) < 1e-13 * max(1.0, abs(vc(wo)))
+
+
+def _trilinear(B, N, L, seed):
+    rng = np.random.default_rng(seed)
+    nR = 8 * B ** 3
+    c = rng.uniform(0.5, 1.5, (8, nR))
+    x = omega_grid(3, N, L).reshape(3, -1, order="F")
+
+    def val(q):  # q (3, n) -> (nR, n)
+        m = [np.ones(q.shape[1]), q[0], q[1], q[2], q[0] * q[1], q[0] * q[2], q[1] * q[2], q[0] * q[1] * q[2]]
+        return sum(c[i][:, None] * m[i][None, :] for i in range(8))
+    f = val(x)
+    return f, val, x
+
+
+@pytest.mark.parametrize("jacobian", [True, False])
+def test_jump_term_exact_for_trilinear_densities(jacobian):
+    B, N, L, tilt = 3, 8, 6.0, 0.2
+    R = O.rotations(B)
+    lam, PC, t = O.reset3_tables(B, R, tilt=tilt, **PAR)
+    f, val, x = _trilinear(B, N, L, 3)
+    nR = 8 * B ** 3
+    fg = np.asfortranarray(f.reshape((2 * B,) * 3 + (N,) * 3, order="F"))
+    df = O.reset3_df(B, N, L, R, fg, lam, PC, t, EPS, tilt, jacobian).reshape(nR, -1, order="F")
+    Rf = R.reshape(3, 3, -1, order="F")
+    ew = _ew(tilt)
+    h = L / N
+    lo, hi = -L / 2 + 1e-10, -L / 2 + h * (N - 1) - 1e-10
+    J = 1 / EPS if jacobian else 1.0
+    inflows = 0
+    for iR in range(nR):
+        if lam[iR] == 0:
+            assert np.all(df[iR] == 0)
+            continue
+        Rr = Rf[:, :, iR]
+        w = Rr @ x
+        tt = t[:, iR]
+        wo = w - (1 + EPS) / EPS * (tt @ w)[None, :] * tt[:, None]
+        Oo = Rr.T @ wo
+        vc = lambda q: np.cross(q.T, PC[:, iR]) @ ew
+        inside = np.all((Oo >= lo) & (Oo <= hi), axis=0) & (vc(wo) >= 0)
+        pred = -lam[iR] * f[iR] * (vc(w) >= 0)
+        pred[inside] += J * lam[iR] * val(Oo[:, inside])[iR]
+        inflows += int(inside.sum())
+        # points whose contact velocity is zero to rounding (omega on the
+        # plane of PC and e_w) sit on the switching surface: not compared
+        ok = (np.abs(vc(w)) > 1e-12) & (np.abs(vc(wo)) > 1e-12)
+        assert np.abs(df[iR] - pred)[ok].max() < 1e-12 * max(1.0, np.abs(pred).max())
+    assert inflows > 100
+
+
+def test_jump_term_conserves_probability_with_the_jacobian():
+    """With J = 1/eps the inflow is the change of variables of the outflow:
+    per rotation the inflow sum matches the outflow sum (omega- with
+    vC >= 0) up to interpolation error, for a density well inside the box;
+    without the Jacobian it falls short by the factor eps."""
+    B, N, L, tilt = 3, 24, 12.0, 0.2
+    R = O.rotations(B)
+    lam, PC, t = O.reset3_tables(B, R, tilt=tilt, **PAR)
+    f = density(B, 3, N, L, (4, 2, 1), seed=5, Sigma=np.eye(3) * 1.2)
+    nR = 8 * B ** 3
+    x = omega_grid(3, N, L).reshape(3, -1, order="F")
+    fr = np.asarray(f).reshape(nR, -1, order="F")
+    Rf = R.reshape(3, 3, -1, order="F")
+    ew = _ew(tilt)
+    res = {}
+    for jac in (True, False):
+        df = O.reset3_df(B, N, L, R, f, lam, PC, t, EPS, tilt, jac).reshape(nR, -1, order="F")
+        tin = tout = 0.0
+        ratios = []
+        for iR in np.nonzero(lam)[0]:
+            vc = np.cross((Rf[:, :, iR] @ x).T, PC[:, iR]) @ ew
+            outflow = lam[iR] * fr[iR][vc >= 0].sum()
+            tin += df[iR].sum() + outflow
+            tout += outflow
+            if outflow > 1e-3 * lam[iR] * fr[iR].sum():
+                ratios.append((df[iR].sum() + outflow) / outflow)  # inflow / outflow
+        res[jac] = (tin / tout, np.array(ratios))
+    # the grid cells the switching surface vC = 0 cuts make single rotations
+    # deviate by a few percent; over the region the balance is tight
+    assert len(res[True][1]) > 20
+    assert abs(res[True][0] - 1) < 5e-3 and np.abs(res[True][1] - 1).max() < 6e-2
+    assert abs(res[False][0] - EPS) < 5e-3 and np.abs(res[False][1] - EPS).max() < 6e-2
