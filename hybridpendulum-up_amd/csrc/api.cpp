@@ -13,6 +13,7 @@
 #include "getc.h"
 #include "hpup_internal.h"
 #include "matfisher.h"
+#include "reset_plan.h"
 #include "stats.h"
 
 namespace hpup {
@@ -242,34 +243,6 @@ static void grid_marginals(int B, int dim, int64_t nx, const void* f, const doub
     HPUP_HIP(hipStreamSynchronize(st));
 }
 
-// R = Rz(alpha) Ry(beta) Rz(gamma) on the grid alpha_i = pi i/B,
-// beta_j = pi (2j+1)/(4B), gamma_k = pi k/B (fftSO3R.m:38-49), column-major.
-struct EulerGrid {
-    int n2;
-    std::vector<double> ca, sa, cb, sb;
-    explicit EulerGrid(int B) : n2(2 * B), ca(n2), sa(n2), cb(n2), sb(n2) {
-        for (int i = 0; i < n2; i++) {
-            ca[i] = std::cos(kPi * i / B);
-            sa[i] = std::sin(kPi * i / B);
-            cb[i] = std::cos(kPi * (2 * i + 1) / (4.0 * B));
-            sb[i] = std::sin(kPi * (2 * i + 1) / (4.0 * B));
-        }
-    }
-    void operator()(int64_t iR, double* R) const {
-        const int64_t i = iR % n2, j = (iR / n2) % n2, k = iR / ((int64_t)n2 * n2);
-        const double a = ca[i], s = sa[i], b = cb[j], t = sb[j], c = ca[k], u = sa[k];
-        R[0] = a * b * c - s * u;
-        R[1] = s * b * c + a * u;
-        R[2] = -t * c;
-        R[3] = -a * b * u - s * c;
-        R[4] = -s * b * u + a * c;
-        R[5] = t * u;
-        R[6] = a * t;
-        R[7] = s * t;
-        R[8] = b;
-    }
-};
-
 struct PlanBase {
     hpup_model model{};
     int64_t nx = 1, nrc = 0, ncoef = 0, ngrid = 0;
@@ -295,6 +268,15 @@ struct PlanBase {
     virtual int profile(void* F, int nreps, double* ms, int maxk, hipStream_t st) = 0;
     virtual const char* profile_name(int i) const = 0;
     virtual double profile_bytes(int i) const = 0;
+    // hybrid collision reset on the grid (set_reset builds its tables)
+    std::unique_ptr<ResetGeom> reset;
+    void set_reset(const hpup_reset_params& prm) {
+        auto g = std::make_unique<ResetGeom>();
+        g->build(model.B, model.dim, model.N, model.L, prm);
+        reset = std::move(g);
+    }
+    virtual void reset_grid(const void* fin, void* out, double scale, int mode, hipStream_t st) = 0;
+    virtual void hybrid_step(void* F, void* f, int nsteps, hipStream_t st) = 0;
 };
 
 template <class T> struct PlanT : PlanBase {
@@ -302,7 +284,7 @@ template <class T> struct PlanT : PlanBase {
     DriftTables<T> dr;
     bool has_drift = false;
     bool bf16x3 = false;  // circulants as 3xBF16 split products (FP32 storage)
-    DevBuf G, Wb, V1, V2, Z3, Fs, Fs2, ACC, tmp, st_work, st_out, gpart;
+    DevBuf G, Wb, V1, V2, Z3, Fs, Fs2, ACC, tmp, st_work, st_out, gpart, dfc;
 
     void init_xform(const double* d, const double* wv) {
         const hpup_model& m = model;
@@ -606,6 +588,34 @@ template <class T> struct PlanT : PlanBase {
         launch_compact_to_full<T>(model.B, nx, a, b, st);
         HPUP_HIP(hipGetLastError());
     }
+    // mode 0: out = fin + scale df(fin) (out may be fin); mode 1: out = scale df(fin)
+    void reset_grid(const void* fin, void* out, double scale, int mode, hipStream_t st) override {
+        HPUP_REQUIRE(reset, HPUP_ERR_ARG, "the plan has no reset (hpup_plan_set_reset)");
+        HPUP_REQUIRE(fin && out, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(mode == 0 || mode == 1, HPUP_ERR_ARG, "reset mode must be 0 or 1");
+        HPUP_REQUIRE(mode == 0 || fin != out, HPUP_ERR_ARG, "reset mode 1 cannot run in place");
+        ensure(dfc, sizeof(T) * (size_t)std::max<int64_t>(1, (int64_t)reset->numR * nx));
+        reset->df<T>(fin, dfc.p, st);
+        if (mode == 0) {
+            if (out != fin) HPUP_HIP(hipMemcpyAsync(out, fin, sizeof(T) * (size_t)ngrid, hipMemcpyDeviceToDevice, st));
+            reset->apply<T>(dfc.p, out, out, scale, st);
+        } else {
+            reset->apply<T>(dfc.p, nullptr, out, scale, st);
+        }
+    }
+    // pendulum_reduced_hybrid.m:224-234: continuous step on the coefficients,
+    // inverse transform, f += dt df on the grid, forward transform; f holds
+    // the last step's grid density after the reset
+    void hybrid_step(void* F, void* f, int nsteps, hipStream_t st) override {
+        HPUP_REQUIRE(reset, HPUP_ERR_ARG, "the plan has no reset (hpup_plan_set_reset)");
+        HPUP_REQUIRE(F && f && nsteps >= 0, HPUP_ERR_ARG, "bad argument");
+        for (int it = 0; it < nsteps; it++) {
+            step(F, 1, st);
+            backward(F, f, st);
+            reset_grid(f, f, model.dt, 0, st);
+            forward(f, F, st);
+        }
+    }
     void marginals(const void* f, const double* xg, std::vector<double>& hR, std::vector<double>& hG,
                    std::vector<double>& hx, hipStream_t st) override {
         grid_marginals<T>(model.B, model.dim, nx, f, w_dev.as<double>(), xg, st_work, st_out, hR, hG, hx, st);
@@ -779,6 +789,38 @@ int hpup_plan_stat_moments(hpup_plan* p, const void* f, double* ER, double* Ex, 
         if (ER) std::memcpy(ER, s.ER, sizeof(s.ER));
         if (Ex) std::memcpy(Ex, s.Ex, sizeof(double) * dim);
         if (Varx) std::memcpy(Varx, s.Varx, sizeof(double) * dim * dim);
+    });
+}
+
+int hpup_plan_set_reset(hpup_plan* p, const hpup_reset_params* prm) {
+    return guard([&] {
+        HPUP_REQUIRE(p && prm, HPUP_ERR_ARG, "null argument");
+        p->impl->set_reset(*prm);
+    });
+}
+int hpup_plan_reset_region(const hpup_plan* p, int32_t* indR, int max_rows) {
+    try {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        HPUP_REQUIRE(p->impl->reset, HPUP_ERR_ARG, "the plan has no reset (hpup_plan_set_reset)");
+        const auto& reg = p->impl->reset->region;
+        if (indR)
+            for (int k = 0; k < (int)reg.size() && k < max_rows; k++) indR[k] = reg[k];
+        return (int)reg.size();
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+}
+int hpup_plan_reset(hpup_plan* p, const void* f_in, void* f_out, double scale, int mode, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        p->impl->reset_grid(f_in, f_out, scale, mode, (hipStream_t)stream);
+    });
+}
+int hpup_plan_hybrid_step(hpup_plan* p, void* F, void* f, int nsteps, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        p->impl->hybrid_step(F, f, nsteps, (hipStream_t)stream);
     });
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <exception>
 #include <string>
@@ -109,6 +110,38 @@ DevBuf upload(const std::vector<T>& v) {
     if (!v.empty()) HPUP_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return b;
 }
+
+// R = Rz(alpha) Ry(beta) Rz(gamma) on the grid alpha_i = (pi/B) i,
+// beta_j = (pi/(4B)) (2j+1), gamma_k = (pi/B) k (fftSO3R.m:27-49: the angles
+// rounded as the reference forms them, and each entry the same products and
+// sums as the matrix product Ra Rb Rg, whose other terms are exact zeros),
+// column-major.
+struct EulerGrid {
+    int n2;
+    std::vector<double> ca, sa, cb, sb;
+    explicit EulerGrid(int B) : n2(2 * B), ca(n2), sa(n2), cb(n2), sb(n2) {
+        const double pi = 3.14159265358979323846;
+        for (int i = 0; i < n2; i++) {
+            ca[i] = std::cos(pi / B * i);
+            sa[i] = std::sin(pi / B * i);
+            cb[i] = std::cos(pi / (4.0 * B) * (2 * i + 1));
+            sb[i] = std::sin(pi / (4.0 * B) * (2 * i + 1));
+        }
+    }
+    void operator()(int64_t iR, double* R) const {
+        const int64_t i = iR % n2, j = (iR / n2) % n2, k = iR / ((int64_t)n2 * n2);
+        const double a = ca[i], s = sa[i], b = cb[j], t = sb[j], c = ca[k], u = sa[k];
+        R[0] = a * b * c - s * u;
+        R[1] = s * b * c + a * u;
+        R[2] = -t * c;
+        R[3] = -a * b * u - s * c;
+        R[4] = -s * b * u + a * c;
+        R[5] = t * u;
+        R[6] = a * t;
+        R[7] = s * t;
+        R[8] = b;
+    }
+};
 
 // ----------------------------------------------------------- host tables --
 

@@ -1,0 +1,39 @@
+// Device-resident hybrid reset of a plan (see reset_plan.hip).
+#pragma once
+
+#include <vector>
+
+#include "hpup_internal.h"
+
+namespace hpup {
+
+constexpr int kResetMaxN = 64;
+
+// Launch constants of the reset kernels (by value).
+struct ResetConsts {
+    int N;
+    double xg[kResetMaxN];  // omega grid values along an axis
+    double x0, hx;          // 3-D: grid origin and spacing
+    double xlo, xhi;        // box of valid pull-backs
+    double Lg, dx2;         // 2-D: getIndRule's box length and cell area
+    double kfac;            // (1 + eps) / eps
+    double J;               // jump-term Jacobian factor (1/eps or 1)
+    double ew[3];           // wall normal
+};
+
+struct ResetGeom {
+    int dim = 0, N = 0, numR = 0;
+    int64_t nR = 0, nx = 0;
+    hpup_reset_params params{};
+    ResetConsts consts{};
+    DevBuf indR, geo;             // region rows and their geometry (FP64, structure of arrays)
+    std::vector<int32_t> region;  // host copy of the region rows
+    void build(int B, int dim, int N, double L, const hpup_reset_params& p);
+    // dfc (numR, nx) = df on the region's rows
+    template <class T> void df(const void* f, void* dfc, hipStream_t st) const;
+    // out = base + scale dfc on the region's rows (base may be out); base
+    // nullptr: out = scale dfc, zero elsewhere
+    template <class T> void apply(const void* dfc, const void* base, void* out, double scale, hipStream_t st) const;
+};
+
+}  // namespace hpup

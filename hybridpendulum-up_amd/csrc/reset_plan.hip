@@ -1,0 +1,282 @@
+// Device-resident hybrid collision reset on a plan's grid (no host copies):
+// the jump term of the reset law applied to the grid density f(R, Omega),
+//   df(R, Omega) = J lambda(R) f(R, g(R, Omega)) [vC(g) >= 0, g in the box]
+//                  - lambda(R) f(R, Omega) [vC(Omega) >= 0],
+// with the pull-back g(R, Omega) = R^T (omega - ((1 + eps)/eps)(omega . t) t),
+// omega = R Omega, and f(R, g) interpolated on the omega grid.
+//
+// dim 2 is the reference's reduced law (no-noise branch): rate and contact
+// point from getLambda.cu:192-246, the 'old' map and its box and contact
+// tests from getOmega.cu:64-137, the bilinear rule of getIndRule.cu:50-90,
+// and the inflow and outflow of propagate.cu:123-229, with J = 1 (the
+// reference omits the Jacobian).  Omega = (x1, x2) enters as R (x1, x2, 0)
+// and the pull-back keeps its first two body components.
+//
+// dim 3 restates the same law with a full body angular velocity and a wall
+// normal e_w = (cos tau, 0, sin tau) (tau = 0 is the reference's wall):
+// theta = asin(r3 . e_w), PC = (h - r tan theta) r3 + r sec theta e_w,
+// t = r3 x e_w / |r3 x e_w|, vC(omega) = (omega x PC) . e_w, trilinear
+// interpolation, and J = 1/eps (|det g|, which conserves probability) or 1.
+//
+// Layouts: the plan's grid (nR, nx) with the rotation index fastest; the
+// region's rows k (rotations with theta >= theta0 - theta_t, ascending) and a
+// compact df (numR, nx) with k fastest.  The geometry is FP64 in every
+// precision (IEEE operations in the oracle's order: contraction off), so the
+// masks and interpolation weights do not depend on the plan's precision.
+#include <cmath>
+#include <vector>
+
+#include "hpup_internal.h"
+#include "reset_plan.h"
+
+namespace hpup {
+
+namespace {
+
+// per-region geometry, structure of arrays: [c * numR + k]
+enum { G_R = 0, G_T = 9, G_PC = 12, G_LAM = 15, G_N = 16 };
+
+template <class T, int DIM>
+__global__ __launch_bounds__(256) void k_reset_df(const T* __restrict__ f, const double* __restrict__ geo,
+                                                  const int32_t* __restrict__ indR, int numR, int64_t nR,
+                                                  ResetConsts c, T* __restrict__ dfc) {
+#pragma clang fp contract(off)
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t ix = blockIdx.y;
+    if (k >= numR) return;
+    double Rr[9], t[3], P[3];
+#pragma unroll
+    for (int a = 0; a < 9; a++) Rr[a] = geo[(int64_t)(G_R + a) * numR + k];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        t[a] = geo[(int64_t)(G_T + a) * numR + k];
+        P[a] = geo[(int64_t)(G_PC + a) * numR + k];
+    }
+    const double lam = geo[(int64_t)G_LAM * numR + k];
+    const int64_t iR = indR[k];
+    const int N = c.N;
+    const int64_t i0 = ix % N, i1 = (ix / N) % N;
+    double out;
+    if constexpr (DIM == 2) {
+        // getLambda.cu:234-246 (outflow where vC >= 0), getOmega.cu:64-137 ('old')
+        const double x1 = c.xg[i0], x2 = c.xg[i1];
+        const double o0 = Rr[0] * x1 + Rr[3] * x2;
+        double o1 = Rr[1] * x1 + Rr[4] * x2;
+        double o2 = Rr[2] * x1 + Rr[5] * x2;
+        const double vC = o1 * P[2] - o2 * P[1];
+        out = vC < 0.0 ? 0.0 : -(double)f[iR + nR * ix] * lam;
+        const double fac = c.kfac;
+        const double ot = fac * (o1 * t[1] + o2 * t[2]);
+        o1 -= ot * t[1];
+        o2 -= ot * t[2];
+        const double a0 = Rr[0] * o0 + Rr[1] * o1 + Rr[2] * o2;
+        const double a1 = Rr[3] * o0 + Rr[4] * o1 + Rr[5] * o2;
+        const double vCo = o1 * P[2] - o2 * P[1];
+        if (!(vCo < 0.0 || a0 < c.xlo || a0 > c.xhi || a1 < c.xlo || a1 > c.xhi)) {
+            // getIndRule.cu:50-90: cell by truncation, weights from the grid values
+            const int j1 = (int)((a0 / c.Lg + 0.5) * N);
+            const int j2 = (int)((a1 / c.Lg + 0.5) * N);
+            if (!(j1 < 0 || j1 > N - 2 || j2 < 0 || j2 > N - 2)) {
+                const double g10 = c.xg[j1], g11 = c.xg[j1 + 1], g20 = c.xg[j2], g21 = c.xg[j2 + 1];
+                const double a = a0 - g10, b = g11 - a0, cc = a1 - g20, e = g21 - a1;
+                const double w[4] = {b * e / c.dx2, a * e / c.dx2, b * cc / c.dx2, a * cc / c.dx2};
+                const int64_t q0 = j1 + (int64_t)j2 * N;
+                const int64_t q[4] = {q0, q0 + 1, q0 + N, q0 + N + 1};
+                double fv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) fv[u] = (double)f[iR + nR * q[u]];
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; u++) s += fv[u] * w[u];
+                out = s * lam + out;
+            }
+        }
+    } else {
+        const int64_t i2 = ix / ((int64_t)N * N);
+        const double Om[3] = {c.x0 + c.hx * (double)i0, c.x0 + c.hx * (double)i1, c.x0 + c.hx * (double)i2};
+        double w[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) w[a] = Rr[a] * Om[0] + Rr[a + 3] * Om[1] + Rr[a + 6] * Om[2];
+        auto vc = [&](const double* v) {
+            const double x0 = v[1] * P[2] - v[2] * P[1], x1 = v[2] * P[0] - v[0] * P[2],
+                         x2 = v[0] * P[1] - v[1] * P[0];
+            return x0 * c.ew[0] + x1 * c.ew[1] + x2 * c.ew[2];
+        };
+        out = vc(w) < 0.0 ? 0.0 : -lam * (double)f[iR + nR * ix];
+        const double fac = c.kfac * (w[0] * t[0] + w[1] * t[1] + w[2] * t[2]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) w[a] -= fac * t[a];
+        double Oo[3], wo[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) Oo[a] = Rr[3 * a] * w[0] + Rr[3 * a + 1] * w[1] + Rr[3 * a + 2] * w[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++) wo[a] = Rr[a] * Oo[0] + Rr[a + 3] * Oo[1] + Rr[a + 6] * Oo[2];
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (!(Oo[a] >= c.xlo && Oo[a] <= c.xhi)) inside = false;
+        if (inside && !(vc(wo) < 0.0)) {
+            int64_t cl[3];
+            double s[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                int64_t ci = (int64_t)floor((Oo[a] - c.x0) / c.hx);
+                if (ci > N - 2) ci = N - 2;
+                cl[a] = ci;
+                s[a] = (Oo[a] - (c.x0 + c.hx * (double)ci)) / c.hx;
+            }
+            double fv[8], wt[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int b0 = q & 1, b1 = (q >> 1) & 1, b2 = (q >> 2) & 1;
+                wt[q] = (b0 ? s[0] : 1 - s[0]) * (b1 ? s[1] : 1 - s[1]) * (b2 ? s[2] : 1 - s[2]);
+                const int64_t xq = (cl[0] + b0) + N * ((cl[1] + b1) + (int64_t)N * (cl[2] + b2));
+                fv[q] = (double)f[iR + nR * xq];
+            }
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) v += wt[q] * fv[q];
+            out += c.J * lam * v;
+        }
+    }
+    dfc[k + (int64_t)numR * ix] = (T)out;
+}
+
+// out[iR + nR ix] = base[...] + scale dfc[k + numR ix] on the region's rows
+// (base = out for an in-place update; base = nullptr: out = scale dfc)
+template <class T>
+__global__ __launch_bounds__(256) void k_reset_apply(const T* __restrict__ dfc, const int32_t* __restrict__ indR,
+                                                     int numR, int64_t nR, const T* base, T* out, double scale) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t ix = blockIdx.y;
+    if (k >= numR) return;
+    const int64_t i = indR[k] + nR * ix;
+    const double d = scale * (double)dfc[k + (int64_t)numR * ix];
+    out[i] = base ? (T)((double)base[i] + d) : (T)d;
+}
+
+template <class T> __global__ void k_fill_zero(T* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+dim3 grid_k(int numR, int64_t nx) { return dim3((unsigned)((numR + 255) / 256), (unsigned)nx); }
+
+}  // namespace
+
+void ResetGeom::build(int B, int dim_, int N_, double L, const hpup_reset_params& p) {
+    HPUP_REQUIRE(dim_ == 2 || dim_ == 3, HPUP_ERR_ARG, "the reset needs an omega grid (dim 2 or 3)");
+    HPUP_REQUIRE(p.epsilon > 0.0 && p.epsilon <= 1.0, HPUP_ERR_ARG, "restitution coefficient must be in (0, 1]");
+    HPUP_REQUIRE(p.theta_t > 0.0 && p.h > 0.0 && p.r >= 0.0, HPUP_ERR_ARG, "bad contact geometry");
+    HPUP_REQUIRE(dim_ == 3 || p.tilt == 0.0, HPUP_ERR_ARG, "the reduced model's wall has no tilt");
+    HPUP_REQUIRE(N_ >= 2 && N_ <= kResetMaxN, HPUP_ERR_ARG, "omega grid out of range for the reset");
+    dim = dim_;
+    N = N_;
+    params = p;
+    const int n2 = 2 * B;
+    nR = (int64_t)n2 * n2 * n2;
+    nx = ipow64(N, dim);
+    HPUP_REQUIRE(nx <= 65535, HPUP_ERR_ARG, "omega grid too large for the reset launch");
+    // the omega grid x = -L/2 + (L/N) i, as the drivers build it
+    ResetConsts& c = consts;
+    c.N = N;
+    for (int i = 0; i < kResetMaxN; i++) c.xg[i] = i < N ? -L / 2 + L / N * (double)i : 0.0;
+    c.x0 = -L / 2;
+    c.hx = L / N;
+    if (dim == 2) {
+        // getOmega's box (the gateway's xlo/xhi) and getIndRule's L and dx^2
+        const double dx = c.xg[1] - c.xg[0];
+        c.xlo = c.xg[0] + 1e-10;
+        c.xhi = c.xg[N - 1] - 1e-10;
+        c.Lg = c.xg[N - 1] - c.xg[0] + dx;
+        c.dx2 = dx * dx;
+        c.kfac = (1.0 + p.epsilon) / p.epsilon;
+        c.J = 1.0;
+    } else {
+        c.xlo = c.x0 + 1e-10;
+        c.xhi = c.x0 + c.hx * (N - 1) - 1e-10;
+        c.kfac = (1.0 + p.epsilon) / p.epsilon;
+        c.J = p.jacobian ? 1.0 / p.epsilon : 1.0;
+    }
+    c.ew[0] = std::cos(p.tilt);
+    c.ew[1] = 0.0;
+    c.ew[2] = std::sin(p.tilt);
+    // per-rotation rate, contact point and tangent (getLambda.cu:192-232)
+    const double hr = std::sqrt(p.h * p.h + p.r * p.r);
+    const double theta0 = std::asin(p.d / hr) - std::asin(p.r / hr);
+    std::vector<int32_t> ind;
+    std::vector<double> cols[G_N];
+    const EulerGrid eg(B);
+    for (int64_t iR = 0; iR < nR; iR++) {
+        double R[9];
+        eg(iR, R);
+        const double r3[3] = {R[6], R[7], R[8]};
+        const double th = dim == 2 ? std::asin(R[6])
+                                   : std::asin(r3[0] * c.ew[0] + r3[1] * c.ew[1] + r3[2] * c.ew[2]);
+        if (th < theta0 - p.theta_t) continue;  // outside the collision region
+        const double lam = th > theta0 + p.theta_t
+                               ? p.lambda_max
+                               : p.lambda_max / 2 * std::sin(M_PI / (2 * p.theta_t) * (th - theta0)) + p.lambda_max / 2;
+        const double lr3 = p.h - p.r * std::tan(th);
+        double PC[3], t[3];
+        if (dim == 2) {
+            PC[0] = lr3 * R[6] + p.r / std::cos(th);
+            PC[1] = lr3 * R[7];
+            PC[2] = lr3 * R[8];
+            const double nt = std::sqrt(R[7] * R[7] + R[8] * R[8]);
+            t[0] = 0.0;
+            t[1] = R[8] / nt;
+            t[2] = -R[7] / nt;
+        } else {
+            for (int a = 0; a < 3; a++) PC[a] = lr3 * r3[a] + p.r / std::cos(th) * c.ew[a];
+            const double cr[3] = {r3[1] * c.ew[2] - r3[2] * c.ew[1], r3[2] * c.ew[0] - r3[0] * c.ew[2],
+                                  r3[0] * c.ew[1] - r3[1] * c.ew[0]};
+            const double nc = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            for (int a = 0; a < 3; a++) t[a] = nc > 0 ? cr[a] / nc : 0.0;
+        }
+        ind.push_back((int32_t)iR);
+        for (int a = 0; a < 9; a++) cols[G_R + a].push_back(R[a]);
+        for (int a = 0; a < 3; a++) {
+            cols[G_T + a].push_back(t[a]);
+            cols[G_PC + a].push_back(PC[a]);
+        }
+        cols[G_LAM].push_back(lam);
+    }
+    numR = (int)ind.size();
+    std::vector<double> g((size_t)G_N * std::max(numR, 1), 0.0);
+    for (int cidx = 0; cidx < G_N; cidx++)
+        for (int k = 0; k < numR; k++) g[(size_t)cidx * numR + k] = cols[cidx][k];
+    indR = upload(ind.empty() ? std::vector<int32_t>{0} : ind);
+    geo = upload(g);
+    region = ind;
+}
+
+template <class T> void ResetGeom::df(const void* f, void* dfc, hipStream_t st) const {
+    if (numR == 0) return;
+    if (dim == 2)
+        hipLaunchKernelGGL((k_reset_df<T, 2>), grid_k(numR, nx), dim3(256), 0, st, (const T*)f, geo.as<double>(),
+                           indR.as<int32_t>(), numR, nR, consts, (T*)dfc);
+    else
+        hipLaunchKernelGGL((k_reset_df<T, 3>), grid_k(numR, nx), dim3(256), 0, st, (const T*)f, geo.as<double>(),
+                           indR.as<int32_t>(), numR, nR, consts, (T*)dfc);
+    HPUP_HIP(hipGetLastError());
+}
+
+template <class T>
+void ResetGeom::apply(const void* dfc, const void* base, void* out, double scale, hipStream_t st) const {
+    if (!base) {  // out = scale df: zero outside the region (a fill kernel, no memset node)
+        const int64_t n = nR * nx;
+        hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)out, n);
+    }
+    if (numR > 0)
+        hipLaunchKernelGGL(k_reset_apply<T>, grid_k(numR, nx), dim3(256), 0, st, (const T*)dfc, indR.as<int32_t>(),
+                           numR, nR, (const T*)base, (T*)out, scale);
+    HPUP_HIP(hipGetLastError());
+}
+
+template void ResetGeom::df<double>(const void*, void*, hipStream_t) const;
+template void ResetGeom::df<float>(const void*, void*, hipStream_t) const;
+template void ResetGeom::apply<double>(const void*, const void*, void*, double, hipStream_t) const;
+template void ResetGeom::apply<float>(const void*, const void*, void*, double, hipStream_t) const;
+
+}  // namespace hpup

@@ -59,6 +59,12 @@ class Model(ctypes.Structure):
                 ("dt", ctypes.c_double), ("method", ctypes.c_int), ("precision", ctypes.c_int)]
 
 
+class ResetParams(ctypes.Structure):
+    _fields_ = [("d", ctypes.c_double), ("h", ctypes.c_double), ("r", ctypes.c_double),
+                ("theta_t", ctypes.c_double), ("lambda_max", ctypes.c_double), ("epsilon", ctypes.c_double),
+                ("tilt", ctypes.c_double), ("jacobian", ctypes.c_int)]
+
+
 _lib = None
 
 # every symbol of include/hpup.h
@@ -76,6 +82,7 @@ EXPORTS = [
     "hpup_get_stat", "hpup_psvd", "hpup_pdf_MF_normal", "hpup_pdf_MF_M2S", "hpup_plan_get_stat",
     "hpup_pendulum_plot_getc", "hpup_gyrobias_propagate", "hpup_plan_create_operator",
     "hpup_plan_adjoint_stage", "hpup_plan_axpy_stage",
+    "hpup_plan_set_reset", "hpup_plan_reset_region", "hpup_plan_reset", "hpup_plan_hybrid_step",
 ]
 
 
@@ -124,6 +131,12 @@ def lib():
         L.hpup_psvd.argtypes = [_D, _D, _D, _D]
         L.hpup_pdf_MF_normal.argtypes = [_D, ctypes.c_int, _D, _D, _D]
         L.hpup_pdf_MF_M2S.argtypes = [_D, _D, _D, _D, ctypes.POINTER(ctypes.c_int)]
+        L.hpup_plan_set_reset.argtypes = [ctypes.c_void_p, ctypes.POINTER(ResetParams)]
+        L.hpup_plan_reset_region.argtypes = [ctypes.c_void_p, _I32, ctypes.c_int]
+        L.hpup_plan_reset.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_void_p]
+        L.hpup_plan_hybrid_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -574,6 +587,37 @@ class Plan:
             _check(n)
         return [(lib().hpup_plan_profile_name(self._h, i).decode(), float(ms[i]),
                  float(lib().hpup_plan_profile_bytes(self._h, i))) for i in range(n)]
+
+    # hybrid collision reset (device grid)
+    def set_reset(self, d, h, r, theta_t, lambda_max, epsilon, tilt=0.0, jacobian=False):
+        """Builds the reset tables (getLambda_mex's d, h, r, theta_t,
+        lambda_max; getOmega_mex's epsilon; the wall tilt and the 1/epsilon
+        Jacobian of the 3-D law)."""
+        prm = ResetParams(float(d), float(h), float(r), float(theta_t), float(lambda_max), float(epsilon),
+                          float(tilt), 1 if jacobian else 0)
+        _check(lib().hpup_plan_set_reset(self._h, ctypes.byref(prm)))
+        n = lib().hpup_plan_reset_region(self._h, None, 0)
+        if n < 0:
+            _check(n)
+        self.reset_rows = np.zeros(n, dtype=np.int32)
+        if n:
+            lib().hpup_plan_reset_region(self._h, _p(self.reset_rows, _I32), n)
+        return self.reset_rows
+
+    def reset(self, f, out=None, scale=1.0, mode=0):
+        """mode 0: out = f + scale df(f) (out may be f); mode 1: out = scale df(f)."""
+        out = self.empty_grid() if out is None else out
+        _check(lib().hpup_plan_reset(self._h, self._chk(f, self.ngrid, self.rdtype),
+                                     self._chk(out, self.ngrid, self.rdtype), float(scale), int(mode),
+                                     self._stream()))
+        return out
+
+    def hybrid_step(self, F, f, nsteps=1):
+        """nsteps hybrid steps: integrator step, inverse transform, reset with
+        dt on the grid f, forward transform; f ends as the post-reset grid."""
+        _check(lib().hpup_plan_hybrid_step(self._h, self._chk(F, self.ncoef, self.cdtype),
+                                           self._chk(f, self.ngrid, self.rdtype), int(nsteps), self._stream()))
+        return F, f
 
     # host conveniences (MATLAB full layout numpy arrays)
     def upload_full(self, Ffull_np):

@@ -15,7 +15,7 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 BUILD = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(PKG_ROOT, "libhpup.so")
 
-SOURCES = ["tables.cpp", "matfisher.cpp", "api.cpp", "drift.hip", "drift_mfma.hip", "transform.hip", "stats.hip", "reset.hip", "getc.hip"]
+SOURCES = ["tables.cpp", "matfisher.cpp", "api.cpp", "drift.hip", "drift_mfma.hip", "transform.hip", "stats.hip", "reset.hip", "reset_plan.hip", "getc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # packed FP32 instructions are disabled for every HIP source

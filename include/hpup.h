@@ -339,6 +339,51 @@ int hpup_plan_stat_moments(hpup_plan* plan, const void* f_dev, double* ER, doubl
  * Synchronizes `stream`. */
 int hpup_plan_get_stat(hpup_plan* plan, const void* f_dev, hpup_stat* out, double* fx, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Device-resident hybrid reset (pendulum_reduced_hybrid.m:224-234 without the
+ * host round trips of its gateways).
+ * ------------------------------------------------------------------------- */
+
+/* Parameters of the collision reset.  d, h, r, theta_t, lambda_max as
+ * getLambda_mex takes them (getLambda.cu:192-232: wall distance, pendulum
+ * length to the end disc, disc radius, half-width of the rate ramp, maximal
+ * rate); epsilon the restitution coefficient as getOmega_mex takes it
+ * (getOmega.cu:99-137).  tilt: the wall normal is (cos tilt, 0, sin tilt)
+ * (0 is the reference's wall; dim 3 only).  jacobian: 1 multiplies the jump
+ * term by 1/epsilon (the Jacobian of the map, which conserves probability;
+ * dim 3 only), 0 follows the reference's law, which omits it. */
+typedef struct {
+    double d, h, r, theta_t, lambda_max;
+    double epsilon;
+    double tilt;
+    int jacobian;
+} hpup_reset_params;
+
+/* Builds the reset tables of the plan's grid: the collision region (the
+ * rotations with theta >= theta0 - theta_t), and per region rotation its rate,
+ * contact point and tangent.  dim 2 is the reference's no-noise reduced law
+ * (getLambda.cu:94-246 compact form, getOmega.cu:64-137 'old' map,
+ * getIndRule.cu:50-90, propagate.cu:123-229): the chain
+ * getLambda_mex(..., true) -> getOmega_mex(..., 'old') -> getIndRule_mex ->
+ * pendulum_reduced_discrete_propagate(false, ...) in one device pass.  dim 3
+ * extends it to a full body angular velocity (trilinear interpolation). */
+int hpup_plan_set_reset(hpup_plan* plan, const hpup_reset_params* params);
+/* The region's rotation indices (ascending, 0-based as lambda_indR): writes
+ * up to max_rows of them into indR (nullable) and returns their number, or a
+ * negative hpup_status. */
+int hpup_plan_reset_region(const hpup_plan* plan, int32_t* indR, int max_rows);
+/* Reset on a device grid of the plan's layout and precision.
+ *   mode 0: f_out = f_in + scale df(f_in)   (f_out may be f_in)
+ *   mode 1: f_out = scale df(f_in)          (zero outside the region)
+ * df is the reset law's jump term (outflow and inflow on the region's rows). */
+int hpup_plan_reset(hpup_plan* plan, const void* f_in, void* f_out, double scale, int mode, void* stream);
+/* nsteps hybrid steps (pendulum_reduced_hybrid.m:224-234): one step of the
+ * plan's integrator on F_dev, f_dev = inverse transform, f_dev += dt df(f_dev),
+ * F_dev = forward transform of f_dev.  f_dev (grid) receives the last step's
+ * density after the reset, the one the driver's get_stat reads.  The
+ * continuous step is the plan's spectral derivative (pendulum.m's form). */
+int hpup_plan_hybrid_step(hpup_plan* plan, void* F_dev, void* f_dev, int nsteps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

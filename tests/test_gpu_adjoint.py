@@ -315,11 +315,12 @@ def test_measurement_likelihood_gradient():
     Prop = propagate_function()
     theta0 = {"s": [4.0, 2.5, 1.0], "mu": [0.3, -0.2], "sig": [1.1, 0.9]}
 
-    def pipeline(th, b=BT, model=gr):
+    def pipeline(th, model=gr):
+        # the propagation runs at the model's own b and G (its plans' operator)
         f0 = mfg_density(B, N, L, th["s"], U, th["mu"], th["sig"])
         F0 = Fwd.apply(f0)
-        FK = Prop.apply(F0, torch.tensor(b, dtype=torch.float64, requires_grad=True),
-                        torch.tensor(G2, dtype=torch.float64), model, K, 2)
+        FK = Prop.apply(F0, torch.tensor(model.b, dtype=torch.float64, requires_grad=True),
+                        torch.tensor(model.G, dtype=torch.float64), model, K, 2)
         return Bwd.apply(FK)
 
     t = {k: torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for k, v in theta0.items()}
@@ -337,7 +338,7 @@ def test_measurement_likelihood_gradient():
         same = tuple(b) == tuple(BT) and np.array_equal(G, G2)
         model = gr if same else Gradient(B, N, L, b=b, G=G, dt=dt, method="rk4")
         with torch.no_grad():
-            return float(torch.log(pipeline(tt, b, model)[idx]).sum())
+            return float(torch.log(pipeline(tt, model)[idx]).sum())
 
     for key in ("s", "mu", "sig"):
         for i in range(len(theta0[key])):
