@@ -215,6 +215,7 @@ template <class T> struct DriftTables {
         v.adj = s.adj ? 1 : 0;
         v.has_gyro = s.has_gyro ? 1 : 0;
         v.grav[0] = v.grav[1] = v.grav[2] = nullptr;
+        v.s3 = nullptr;
     }
 };
 
@@ -276,6 +277,8 @@ struct PlanBase {
         reset = std::move(g);
     }
     virtual void reset_grid(const void* fin, void* out, double scale, int mode, hipStream_t st) = 0;
+    virtual void reset_adjoint(const void* g, void* out, double scale, hipStream_t st) = 0;
+    virtual void reset_eps_grad(const void* f, const void* g, double scale, double* out, hipStream_t st) = 0;
     virtual void hybrid_step(void* F, void* f, int nsteps, hipStream_t st) = 0;
 };
 
@@ -284,7 +287,7 @@ template <class T> struct PlanT : PlanBase {
     DriftTables<T> dr;
     bool has_drift = false;
     bool bf16x3 = false;  // circulants as 3xBF16 split products (FP32 storage)
-    DevBuf G, Wb, V1, V2, Z3, Fs, Fs2, ACC, tmp, st_work, st_out, gpart, dfc;
+    DevBuf G, Wb, V1, V2, Z3, S3, Fs, Fs2, ACC, tmp, st_work, st_out, gpart, dfc, epart;
 
     void init_xform(const double* d, const double* wv) {
         const hpup_model& m = model;
@@ -360,7 +363,15 @@ template <class T> struct PlanT : PlanBase {
         if (den) den_gravity(Fin, D, st);
         if (fused()) {
             mark(0);
-            if (D.dim == 3) launch_axis3_mfma<T>(D, Fin, Z3.p, bf16x3, st);
+            if (D.dim == 3 && D.adj) {
+                // T3^H F for the gyroscopic terms; T3^H [(e3^* - i b3 kc3) F] for the axis-3 term
+                ensure(S3, cbytes());
+                if (D.has_gyro) launch_axis3_mfma<T>(D, Fin, Z3.p, false, st);
+                launch_axis3_mfma<T>(D, Fin, S3.p, false, st, true);
+                D.s3 = S3.as<cT<T>>();
+            } else if (D.dim == 3) {
+                launch_axis3_mfma<T>(D, Fin, Z3.p, bf16x3, st);
+            }
             mark(1);
             launch_drift_fused<T>(D, dr.mix, Fin, Z3.p, F0, Fout, ACC.p, sc, bf16x3, st);
             mark(2);
@@ -603,6 +614,22 @@ template <class T> struct PlanT : PlanBase {
             reset->apply<T>(dfc.p, nullptr, out, scale, st);
         }
     }
+    // out = g + scale R^H g (R the jump term; out may be g)
+    void reset_adjoint(const void* g, void* out, double scale, hipStream_t st) override {
+        HPUP_REQUIRE(reset, HPUP_ERR_ARG, "the plan has no reset (hpup_plan_set_reset)");
+        HPUP_REQUIRE(g && out, HPUP_ERR_ARG, "null argument");
+        ensure(dfc, sizeof(T) * (size_t)std::max<int64_t>(1, (int64_t)reset->numR * nx));
+        reset->adjoint<T>(g, dfc.p, st);
+        if (out != g) HPUP_HIP(hipMemcpyAsync(out, g, sizeof(T) * (size_t)ngrid, hipMemcpyDeviceToDevice, st));
+        reset->apply<T>(dfc.p, out, out, scale, st);
+    }
+    // out[0] (device) += scale <g, dR/d eps f>
+    void reset_eps_grad(const void* f, const void* g, double scale, double* out, hipStream_t st) override {
+        HPUP_REQUIRE(reset, HPUP_ERR_ARG, "the plan has no reset (hpup_plan_set_reset)");
+        HPUP_REQUIRE(f && g && out, HPUP_ERR_ARG, "null argument");
+        ensure(epart, sizeof(double) * (size_t)reset->eps_grad_partials());
+        reset->eps_grad<T>(f, g, epart.as<double>(), scale, out, st);
+    }
     // pendulum_reduced_hybrid.m:224-234: continuous step on the coefficients,
     // inverse transform, f += dt df on the grid, forward transform; f holds
     // the last step's grid density after the reset
@@ -815,6 +842,18 @@ int hpup_plan_reset(hpup_plan* p, const void* f_in, void* f_out, double scale, i
     return guard([&] {
         HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
         p->impl->reset_grid(f_in, f_out, scale, mode, (hipStream_t)stream);
+    });
+}
+int hpup_plan_reset_adjoint(hpup_plan* p, const void* g, void* out, double scale, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        p->impl->reset_adjoint(g, out, scale, (hipStream_t)stream);
+    });
+}
+int hpup_plan_reset_eps_grad(hpup_plan* p, const void* f, const void* g, double scale, double* out, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        p->impl->reset_eps_grad(f, g, scale, out, (hipStream_t)stream);
     });
 }
 int hpup_plan_hybrid_step(hpup_plan* p, void* F, void* f, int nsteps, void* stream) {

@@ -38,6 +38,9 @@ template <class T> struct DriftDev {
     T c0[3];
     // the adjoint operator (transposed tables, damping through the V planes)
     int adj;
+    // 3-D adjoint: T3^H [(e3 - i b3 kc(k3)) F] from the scaled axis-3 pass
+    // (e3 of the adjoint tables is the conjugate of the forward one)
+    const cT<T>* s3;
     // gradient stage (adj only, gpart non-null): forward stage value y, the
     // sweep's running sum, the workgroups' partial sums (kGradMaxWG x 5), and
     // out[5] += gscale * sums
@@ -72,8 +75,11 @@ template <class T> struct MixDev {
 template <class T>
 void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
                         void* Fout, void* ACC, const StageCoef<T>& sc, bool bf16x3, hipStream_t st);
+// out = T3 in; scaled: out = T3 [(e3[r] - i b3 kc(k3)) in] (the 3-D adjoint's
+// axis-3 term, on an adjoint plan's tables)
 template <class T>
-void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf16x3, hipStream_t st);
+void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf16x3, hipStream_t st,
+                       bool scaled = false);
 
 // Omega DFTs as chirp-Toeplitz MFMA products (N <= 32).  bs: 96 entries, the
 // Toeplitz kernel c_d at [d + 32] for d in [-32, 32) and then the chirp p_k

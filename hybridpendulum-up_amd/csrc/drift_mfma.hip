@@ -425,11 +425,13 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
         // this lane's elements of the slab and of T3 F, issued together.  The
         // lanes' elements tile the 32 x 32 plane exactly once, so they also
         // fill plane A (zero in the padding)
-        cT<T> fo[4], Zv[4];
+        cT<T> fo[4], Zv[4], Sv[4];
+        const bool ldz = DIM == 3 && (!ADJ || D.has_gyro);  // the adjoint reads Z3 for its gyroscopic term only
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             fo[v] = *at_b(Fb, offb[v]);
-            Zv[v] = DIM == 3 ? ld_once<T>(at_b(Z3 + base, offb[v])) : czero<T>();
+            Zv[v] = ldz ? ld_once<T>(at_b(Z3 + base, offb[v])) : czero<T>();
+            Sv[v] = (ADJ && DIM == 3) ? ld_once<T>(at_b(D.s3 + base, offb[v])) : czero<T>();
         }
         cT<T> yv[4];
         if constexpr (GRAD) {
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
                 pd_re[k2w[v] * kPad + k1w] = ok[v] ? d.x : T(0);
                 pd_im[k2w[v] * kPad + k1w] = ok[v] ? d.y : T(0);
             }
-        } else if constexpr (!ADJ) {
+        } else if (!ADJ || (DIM == 3 && D.has_gyro)) {  // the 3-D adjoint's gyroscopic term needs T2^H F
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 pa_re[k2w[v] * kPad + k1w] = ok[v] ? fo[v].x : T(0);
@@ -542,6 +544,7 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
                 V1[v] = csub<T>(V1[v], cimul<T>(fo[v], D.b[0] * kc1));
                 V2[v] = csub<T>(V2[v], cimul<T>(fo[v], D.b[1] * kc2v[v]));
             }
+            if (DIM == 3 && D.has_gyro) gm.t2(pa_re, pa_im, P2, D.c0[1]);
         } else {
             gm.t1(pa_re, pa_im, P1, D.c0[0]);
             gm.t2(pa_re, pa_im, P2, D.c0[1]);
@@ -556,17 +559,25 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
         }
         // every term but T1 V1 + T2 V2, summed now so that P, Z and W retire:
         // S = W + b1 c1 P1 + b2 c2 P2 + (e3 + b3 c3) Z3
-        const cT<T> e3 = DIM == 3 ? kc_cplx<T>(as_const<T>((const T*)D.e3), r) : czero<T>();
         cT<T> S[4];
+        if constexpr (ADJ) {
+            // the damping went into the V planes; 3-D: the axis-3 term comes
+            // whole from the scaled axis-3 pass, T3^H [(e3^* - i b3 kc3) F]
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            cT<T> q = cadd<T>(Wv[v], cimul<T>(P1[v], D.b[0] * kc1));
-            q = cadd<T>(q, cimul<T>(P2[v], D.b[1] * kc2v[v]));
-            if (DIM == 3) {
-                q = cadd<T>(q, cmul<T>(Zv[v], e3));
-                q = cadd<T>(q, cimul<T>(Zv[v], D.b[2] * kc3));
+            for (int v = 0; v < 4; v++)
+                S[v] = DIM == 3 ? cadd<T>(Wv[v], Sv[v]) : Wv[v];
+        } else {
+            const cT<T> e3 = DIM == 3 ? kc_cplx<T>(as_const<T>((const T*)D.e3), r) : czero<T>();
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                cT<T> q = cadd<T>(Wv[v], cimul<T>(P1[v], D.b[0] * kc1));
+                q = cadd<T>(q, cimul<T>(P2[v], D.b[1] * kc2v[v]));
+                if (DIM == 3) {
+                    q = cadd<T>(q, cmul<T>(Zv[v], e3));
+                    q = cadd<T>(q, cimul<T>(Zv[v], D.b[2] * kc3));
+                }
+                S[v] = q;
             }
-            S[v] = q;
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {  // V planes, zero in the padding
@@ -706,7 +717,7 @@ __global__ __launch_bounds__(kGradRedThreads) void k_grad_reduce(const double* _
 // address and are zeroed), all eight issued before the first wait, with
 // 32-bit offsets from the row's base.  F and T3 F are touched once here:
 // FP64 streams them non-temporally (measured faster; FP32 measured slower).
-template <class T>
+template <class T, bool SCALED = false>
 __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* __restrict__ in,
                                                     cT<T>* __restrict__ out) {
     using MF = Mfma<T>;
@@ -728,6 +739,16 @@ __global__ __launch_bounds__(256) void k_axis3_mfma(DriftDev<T> D, const cT<T>* 
         x[i] = ok ? v : czero<T>();
     }
     tab.fill(D.kap + 2 * N, N, tid);
+    if constexpr (SCALED) {
+        // row a (= k3) of the input times e3[r] - i b3 kc(a)
+        const cT<T> e = kc_cplx<T>(as_const<T>((const T*)D.e3), r);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int a = w + 4 * i;
+            const T kca = a < N ? D.kc[a] : T(0);
+            x[i] = cmul<T>(x[i], cmk<T>(e.x, e.y - D.b[2] * kca));
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const int a = w + 4 * i;
@@ -908,8 +929,14 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     HPUP_REQUIRE(!grav || D.dim == 2, HPUP_ERR_UNSUPPORTED, "density-form gravity is a 2-D path");
     if (D.adj) {
         // model plans have coordinate-ramp circulants; only that form is built
-        HPUP_REQUIRE(D.dim == 2 && !grav && D.ramp, HPUP_ERR_UNSUPPORTED,
-                     "the adjoint operator is built for the reduced model on its own omega grid");
+        HPUP_REQUIRE(!grav && D.ramp, HPUP_ERR_UNSUPPORTED,
+                     "the adjoint operator is built for model plans on their own omega grid");
+        if (D.dim == 3) {
+            HPUP_REQUIRE(!D.gpart, HPUP_ERR_UNSUPPORTED, "parameter gradients: reduced model (dim 2) only");
+            HPUP_REQUIRE(D.s3, HPUP_ERR_ARG, "3-D adjoint stage needs its axis-3 term");
+            launch_fused_t<T, GemmF<T, true>, 3, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+            return;
+        }
         if (D.gpart) {
             HPUP_REQUIRE(D.gy && sc.deriv_only, HPUP_ERR_ARG, "gradient stage needs the forward stage value");
             const unsigned nwg = launch_fused_t<T, GemmF<T, true>, 2, false, true, true>(D, MX, Fin, Z3, F0, Fout, ACC,
@@ -939,11 +966,17 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
 }
 
 template <class T>
-void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf16x3, hipStream_t st) {
+void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf16x3, hipStream_t st, bool scaled) {
     HPUP_REQUIRE(D.N <= 32 && D.dim == 3, HPUP_ERR_UNSUPPORTED, "axis-3 MFMA kernel needs dim 3, N <= 32");
     const int64_t NN = (int64_t)D.N * D.N;
     HPUP_REQUIRE(D.nrc <= 65535, HPUP_ERR_UNSUPPORTED, "axis-3 kernel: more than 65535 rotation rows");
     const int nch = (int)((NN + 63) / 64);
+    if (scaled) {
+        HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "the adjoint operator has no 3xBF16 form");
+        dim3 grid((unsigned)nch, (unsigned)D.nrc);
+        hipLaunchKernelGGL((k_axis3_mfma<T, true>), grid, dim3(256), 0, st, D, (const cT<T>*)in, (cT<T>*)out);
+        return;
+    }
     if constexpr (sizeof(T) == 4) {
         if (bf16x3) {
             dim3 grid((unsigned)nch, (unsigned)D.nrc);
@@ -953,7 +986,7 @@ void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf1
     }
     HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 needs FP32 storage");
     dim3 grid((unsigned)nch, (unsigned)D.nrc);
-    hipLaunchKernelGGL(k_axis3_mfma<T>, grid, dim3(256), 0, st, D, (const cT<T>*)in, (cT<T>*)out);
+    hipLaunchKernelGGL((k_axis3_mfma<T, false>), grid, dim3(256), 0, st, D, (const cT<T>*)in, (cT<T>*)out);
 }
 
 // ------------------------------------------------- omega DFTs (chirp) --
@@ -1122,7 +1155,7 @@ template void launch_drift_fused<float>(const DriftDev<float>&, const MixDev<flo
                                         const void*, void*, void*, const StageCoef<float>&, bool, hipStream_t);
 template void launch_drift_fused<double>(const DriftDev<double>&, const MixDev<double>&, const void*, const void*,
                                          const void*, void*, void*, const StageCoef<double>&, bool, hipStream_t);
-template void launch_axis3_mfma<float>(const DriftDev<float>&, const void*, void*, bool, hipStream_t);
-template void launch_axis3_mfma<double>(const DriftDev<double>&, const void*, void*, bool, hipStream_t);
+template void launch_axis3_mfma<float>(const DriftDev<float>&, const void*, void*, bool, hipStream_t, bool);
+template void launch_axis3_mfma<double>(const DriftDev<double>&, const void*, void*, bool, hipStream_t, bool);
 
 }  // namespace hpup

@@ -34,6 +34,12 @@ struct ResetGeom {
     // out = base + scale dfc on the region's rows (base may be out); base
     // nullptr: out = scale dfc, zero elsewhere
     template <class T> void apply(const void* dfc, const void* base, void* out, double scale, hipStream_t st) const;
+    // acc (numR, nx) = the transpose of the jump term applied to the grid g
+    template <class T> void adjoint(const void* g, void* acc, hipStream_t st) const;
+    // out[0] += scale <g, d(jump term)/d eps f> over the region (part: eps_grad_partials() doubles)
+    template <class T>
+    void eps_grad(const void* f, const void* g, double* part, double scale, double* out, hipStream_t st) const;
+    int64_t eps_grad_partials() const;
 };
 
 }  // namespace hpup

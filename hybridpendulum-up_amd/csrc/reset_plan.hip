@@ -155,6 +155,191 @@ __global__ __launch_bounds__(256) void k_reset_apply(const T* __restrict__ dfc, 
     out[i] = base ? (T)((double)base[i] + d) : (T)d;
 }
 
+// The jump term's pieces at one (region row, omega point), in FP64: the
+// outflow mask, the pulled-back point with its interpolation corners and
+// weights (inflow mask false: no corners), and d(pull-back)/d(eps).
+template <int DIM> struct Jump {
+    static constexpr int NQ = DIM == 2 ? 4 : 8;
+    bool out_on, in_on;
+    int64_t q[NQ];
+    double w[NQ];
+    double dw[NQ];  // d w_q / d eps through the pulled-back point
+};
+
+template <int DIM>
+__device__ __forceinline__ void jump_at(const double* Rr, const double* t, const double* P, const ResetConsts& c,
+                                        int64_t ix, double deps_scale, Jump<DIM>& J) {
+#pragma clang fp contract(off)
+    const int N = c.N;
+    const int64_t i0 = ix % N, i1 = (ix / N) % N;
+    J.in_on = false;
+    if constexpr (DIM == 2) {
+        const double x1 = c.xg[i0], x2 = c.xg[i1];
+        const double o0 = Rr[0] * x1 + Rr[3] * x2;
+        double o1 = Rr[1] * x1 + Rr[4] * x2;
+        double o2 = Rr[2] * x1 + Rr[5] * x2;
+        J.out_on = !(o1 * P[2] - o2 * P[1] < 0.0);
+        const double dot = o1 * t[1] + o2 * t[2];
+        const double ot = c.kfac * dot;
+        o1 -= ot * t[1];
+        o2 -= ot * t[2];
+        const double a0 = Rr[0] * o0 + Rr[1] * o1 + Rr[2] * o2;
+        const double a1 = Rr[3] * o0 + Rr[4] * o1 + Rr[5] * o2;
+        const double vCo = o1 * P[2] - o2 * P[1];
+        if (vCo < 0.0 || a0 < c.xlo || a0 > c.xhi || a1 < c.xlo || a1 > c.xhi) return;
+        const int j1 = (int)((a0 / c.Lg + 0.5) * N);
+        const int j2 = (int)((a1 / c.Lg + 0.5) * N);
+        if (j1 < 0 || j1 > N - 2 || j2 < 0 || j2 > N - 2) return;
+        J.in_on = true;
+        const double g10 = c.xg[j1], g11 = c.xg[j1 + 1], g20 = c.xg[j2], g21 = c.xg[j2 + 1];
+        const double a = a0 - g10, b = g11 - a0, cc = a1 - g20, e = g21 - a1;
+        J.w[0] = b * e / c.dx2;
+        J.w[1] = a * e / c.dx2;
+        J.w[2] = b * cc / c.dx2;
+        J.w[3] = a * cc / c.dx2;
+        const int64_t q0 = j1 + (int64_t)j2 * N;
+        J.q[0] = q0;
+        J.q[1] = q0 + 1;
+        J.q[2] = q0 + N;
+        J.q[3] = q0 + N + 1;
+        // d(a0, a1)/d eps = deps_scale (o . t) (R^T t)_{0,1}, deps_scale = -dk/deps
+        const double da0 = deps_scale * dot * (Rr[1] * t[1] + Rr[2] * t[2]);
+        const double da1 = deps_scale * dot * (Rr[4] * t[1] + Rr[5] * t[2]);
+        J.dw[0] = (-e * da0 - b * da1) / c.dx2;
+        J.dw[1] = (e * da0 - a * da1) / c.dx2;
+        J.dw[2] = (-cc * da0 + b * da1) / c.dx2;
+        J.dw[3] = (cc * da0 + a * da1) / c.dx2;
+    } else {
+        const int64_t i2 = ix / ((int64_t)N * N);
+        const double Om[3] = {c.x0 + c.hx * (double)i0, c.x0 + c.hx * (double)i1, c.x0 + c.hx * (double)i2};
+        double w[3];
+        for (int a = 0; a < 3; a++) w[a] = Rr[a] * Om[0] + Rr[a + 3] * Om[1] + Rr[a + 6] * Om[2];
+        auto vc = [&](const double* v) {
+            const double x0 = v[1] * P[2] - v[2] * P[1], x1 = v[2] * P[0] - v[0] * P[2],
+                         x2 = v[0] * P[1] - v[1] * P[0];
+            return x0 * c.ew[0] + x1 * c.ew[1] + x2 * c.ew[2];
+        };
+        J.out_on = !(vc(w) < 0.0);
+        const double dot = w[0] * t[0] + w[1] * t[1] + w[2] * t[2];
+        const double fac = c.kfac * dot;
+        for (int a = 0; a < 3; a++) w[a] -= fac * t[a];
+        double Oo[3], wo[3];
+        for (int a = 0; a < 3; a++) Oo[a] = Rr[3 * a] * w[0] + Rr[3 * a + 1] * w[1] + Rr[3 * a + 2] * w[2];
+        for (int a = 0; a < 3; a++) wo[a] = Rr[a] * Oo[0] + Rr[a + 3] * Oo[1] + Rr[a + 6] * Oo[2];
+        for (int a = 0; a < 3; a++)
+            if (!(Oo[a] >= c.xlo && Oo[a] <= c.xhi)) return;
+        if (vc(wo) < 0.0) return;
+        J.in_on = true;
+        int64_t cl[3];
+        double s[3], ds[3];
+        for (int a = 0; a < 3; a++) {
+            int64_t ci = (int64_t)floor((Oo[a] - c.x0) / c.hx);
+            if (ci > N - 2) ci = N - 2;
+            cl[a] = ci;
+            s[a] = (Oo[a] - (c.x0 + c.hx * (double)ci)) / c.hx;
+            // d Oo_a / d eps = deps_scale (w . t) (R^T t)_a
+            ds[a] = deps_scale * dot * (Rr[3 * a] * t[0] + Rr[3 * a + 1] * t[1] + Rr[3 * a + 2] * t[2]) / c.hx;
+        }
+        for (int q = 0; q < 8; q++) {
+            const int b0 = q & 1, b1 = (q >> 1) & 1, b2 = (q >> 2) & 1;
+            const double u0 = b0 ? s[0] : 1 - s[0], u1 = b1 ? s[1] : 1 - s[1], u2 = b2 ? s[2] : 1 - s[2];
+            J.w[q] = u0 * u1 * u2;
+            J.dw[q] = (b0 ? ds[0] : -ds[0]) * u1 * u2 + u0 * (b1 ? ds[1] : -ds[1]) * u2 +
+                      u0 * u1 * (b2 ? ds[2] : -ds[2]);
+            J.q[q] = (cl[0] + b0) + N * ((cl[1] + b1) + (int64_t)N * (cl[2] + b2));
+        }
+    }
+}
+
+__device__ __forceinline__ void load_geo(const double* geo, int numR, int k, double* Rr, double* t, double* P,
+                                         double& lam) {
+#pragma unroll
+    for (int a = 0; a < 9; a++) Rr[a] = geo[(int64_t)(G_R + a) * numR + k];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        t[a] = geo[(int64_t)(G_T + a) * numR + k];
+        P[a] = geo[(int64_t)(G_PC + a) * numR + k];
+    }
+    lam = geo[(int64_t)G_LAM * numR + k];
+}
+
+// Transpose of the jump term, scattered into acc (numR, nx), zeroed first:
+// acc[k][x] += -lambda [out] g(x); acc[k][corner q of x] += J lambda w_q g(x).
+// The scatter targets collide across omega points, so it adds atomically
+// (the summation order, and so the last bits, vary between runs).
+template <class T, int DIM>
+__global__ __launch_bounds__(256) void k_reset_adj(const T* __restrict__ g, const double* __restrict__ geo,
+                                                   const int32_t* __restrict__ indR, int numR, int64_t nR,
+                                                   ResetConsts c, T* acc) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t ix = blockIdx.y;
+    if (k >= numR) return;
+    double Rr[9], t[3], P[3], lam;
+    load_geo(geo, numR, k, Rr, t, P, lam);
+    Jump<DIM> J;
+    jump_at<DIM>(Rr, t, P, c, ix, 0.0, J);
+    const int64_t iR = indR[k];
+    const double gv = (double)g[iR + nR * ix];
+    if (J.out_on) atomicAdd(acc + k + (int64_t)numR * ix, (T)(-lam * gv));
+    if (J.in_on) {
+        const double a = c.J * lam * gv;
+#pragma unroll
+        for (int q = 0; q < Jump<DIM>::NQ; q++) atomicAdd(acc + k + (int64_t)numR * J.q[q], (T)(a * J.w[q]));
+    }
+}
+
+// d/d eps of the jump term, against g: per block the sum of
+// g(x) [dJ/d eps lambda I(f) + J lambda dI(f)/d eps] over its (k, x) items,
+// written to part[block] (FP64, fixed order: lanes, then waves)
+template <class T, int DIM>
+__global__ __launch_bounds__(256) void k_reset_deps(const T* __restrict__ f, const T* __restrict__ g,
+                                                    const double* __restrict__ geo, const int32_t* __restrict__ indR,
+                                                    int numR, int64_t nR, ResetConsts c, double dJ, double dk,
+                                                    double* __restrict__ part) {
+    __shared__ double red[4];
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t ix = blockIdx.y;
+    double v = 0.0;
+    if (k < numR) {
+        double Rr[9], t[3], P[3], lam;
+        load_geo(geo, numR, k, Rr, t, P, lam);
+        Jump<DIM> J;
+        jump_at<DIM>(Rr, t, P, c, ix, -dk, J);
+        if (J.in_on) {
+            const int64_t iR = indR[k];
+            double I = 0.0, dI = 0.0;
+#pragma unroll
+            for (int q = 0; q < Jump<DIM>::NQ; q++) {
+                const double fq = (double)f[iR + nR * J.q[q]];
+                I += J.w[q] * fq;
+                dI += J.dw[q] * fq;
+            }
+            v = (double)g[iR + nR * ix] * lam * (dJ * I + c.J * dI);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out[0] += scale * sum of n partials, one block: strided sums per thread,
+// then a fixed pairwise tree (deterministic)
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int64_t n, double scale,
+                                                      double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] += scale * red[0];
+}
+
 template <class T> __global__ void k_fill_zero(T* p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = T(0);
@@ -274,6 +459,46 @@ void ResetGeom::apply(const void* dfc, const void* base, void* out, double scale
     HPUP_HIP(hipGetLastError());
 }
 
+template <class T> void ResetGeom::adjoint(const void* g, void* acc, hipStream_t st) const {
+    const int64_t n = (int64_t)numR * nx;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)acc, n);
+    if (dim == 2)
+        hipLaunchKernelGGL((k_reset_adj<T, 2>), grid_k(numR, nx), dim3(256), 0, st, (const T*)g, geo.as<double>(),
+                           indR.as<int32_t>(), numR, nR, consts, (T*)acc);
+    else
+        hipLaunchKernelGGL((k_reset_adj<T, 3>), grid_k(numR, nx), dim3(256), 0, st, (const T*)g, geo.as<double>(),
+                           indR.as<int32_t>(), numR, nR, consts, (T*)acc);
+    HPUP_HIP(hipGetLastError());
+}
+
+template <class T>
+void ResetGeom::eps_grad(const void* f, const void* g, double* part, double scale, double* out, hipStream_t st) const {
+    if (numR == 0) return;
+    const double eps = params.epsilon;
+    const double dk = -1.0 / (eps * eps);                                        // d((1 + eps)/eps)/d eps
+    const double dJ = (dim == 3 && params.jacobian) ? -1.0 / (eps * eps) : 0.0;  // d(1/eps)/d eps
+    const dim3 grid = grid_k(numR, nx);
+    if (dim == 2)
+        hipLaunchKernelGGL((k_reset_deps<T, 2>), grid, dim3(256), 0, st, (const T*)f, (const T*)g, geo.as<double>(),
+                           indR.as<int32_t>(), numR, nR, consts, dJ, dk, part);
+    else
+        hipLaunchKernelGGL((k_reset_deps<T, 3>), grid, dim3(256), 0, st, (const T*)f, (const T*)g, geo.as<double>(),
+                           indR.as<int32_t>(), numR, nR, consts, dJ, dk, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)part,
+                       (int64_t)grid.x * grid.y, scale, out);
+    HPUP_HIP(hipGetLastError());
+}
+
+int64_t ResetGeom::eps_grad_partials() const {
+    const dim3 grid = grid_k(std::max(numR, 1), nx);
+    return (int64_t)grid.x * grid.y;
+}
+
+template void ResetGeom::adjoint<double>(const void*, void*, hipStream_t) const;
+template void ResetGeom::adjoint<float>(const void*, void*, hipStream_t) const;
+template void ResetGeom::eps_grad<double>(const void*, const void*, double*, double, double*, hipStream_t) const;
+template void ResetGeom::eps_grad<float>(const void*, const void*, double*, double, double*, hipStream_t) const;
 template void ResetGeom::df<double>(const void*, void*, hipStream_t) const;
 template void ResetGeom::df<float>(const void*, void*, hipStream_t) const;
 template void ResetGeom::apply<double>(const void*, const void*, void*, double, hipStream_t) const;

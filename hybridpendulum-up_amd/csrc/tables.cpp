@@ -74,20 +74,27 @@ void host_wigner_d(double beta, int lmax, double* d) {
 // clebsch_gordan.m:1-41: per (l, m) the null vector of the tridiagonal
 // J^2 - l(l+1) block on m1 + m2 = m, last (largest m1) component positive,
 // unit norm; solved by LU with partial pivoting as MATLAB's linsolve does.
-// Adjoint of a reduced-model operator (for the gradient harness; no
-// counterpart in the reference).  With A = sum of (row mixing) o (omega
-// circulant) terms, (K T)^H = T^H K^H and the row mixing acts on the rotation
-// index while T_p acts on omega, so each factor is transposed on its own:
-// mixing r <- src with c becomes src <- r with conj(c) (negated for the
-// Clebsch-Gordan terms, which enter as -i c F), T_p's kernel becomes
-// conj(kappa_p(-d)), and the damping i b_p c_p T_p F becomes
-// T_p^H(-i b_p c_p F), applied by the kernel's adj path.
+// Adjoint of a model operator (for the gradient harness; no counterpart in
+// the reference).  With A = sum of (row mixing) o (omega circulant) terms,
+// (K T)^H = T^H K^H and the row mixing acts on the rotation index while T_p
+// acts on omega, so each factor is transposed on its own: mixing r <- src
+// with c becomes src <- r with conj(c) (negated for the Clebsch-Gordan terms,
+// which enter as -i c F), T_p's kernel becomes conj(kappa_p(-d)), and the
+// damping i b_p c_p T_p F becomes T_p^H(-i b_p c_p F), applied by the
+// kernel's adj path.  In 3-D the axis-3 term (e3 + i b3 kc3) T3 F becomes
+// T3^H [(e3^* - i b3 kc3) F] (the kernel's scaled axis-3 pass).
 DriftSpec adjoint_spec(const DriftSpec& s) {
-    HPUP_REQUIRE(s.dim == 2 && !s.adj, HPUP_ERR_UNSUPPORTED, "adjoint operator: reduced model (dim 2) only");
+    HPUP_REQUIRE((s.dim == 2 || s.dim == 3) && !s.adj, HPUP_ERR_UNSUPPORTED,
+                 "adjoint operator: omega dimension 2 or 3");
     DriftSpec a = s;
     a.adj = true;
     const int N = s.N, dim = s.dim;
     const int64_t nrc = nrc_of(s.B);
+    // 3-D: the diagonal u_3 term conjugates; the gyroscopic terms a_p c_q T_r
+    // become a_p c_q^* T_r^H = -a_p c_q T_r^H (c_q = i kc(k_q) is imaginary and
+    // commutes with T_r, r != q)
+    for (int64_t r = 0; r < (int64_t)a.e3.size() / 2; r++) a.e3[2 * r + 1] = -a.e3[2 * r + 1];
+    for (int p = 0; p < 3; p++) a.gyro[p] = -s.gyro[p];
     for (int p = 0; p < dim; p++)
         for (int j = 0; j < N; j++) {
             const int jm = (N - j) % N;

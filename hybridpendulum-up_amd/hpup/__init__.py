@@ -83,6 +83,7 @@ EXPORTS = [
     "hpup_pendulum_plot_getc", "hpup_gyrobias_propagate", "hpup_plan_create_operator",
     "hpup_plan_adjoint_stage", "hpup_plan_axpy_stage",
     "hpup_plan_set_reset", "hpup_plan_reset_region", "hpup_plan_reset", "hpup_plan_hybrid_step",
+    "hpup_plan_reset_adjoint", "hpup_plan_reset_eps_grad",
 ]
 
 
@@ -137,6 +138,10 @@ def lib():
                                       ctypes.c_int, ctypes.c_void_p]
         L.hpup_plan_hybrid_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.c_void_p]
+        L.hpup_plan_reset_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                              ctypes.c_void_p]
+        L.hpup_plan_reset_eps_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                               ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -420,8 +425,9 @@ class Plan:
 
     operator selects what the plan applies (no counterpart in the reference;
     used by the gradient harness in hpup.adjoint): 0 the model's operator A,
-    1 its adjoint A^H (dim 2; step then applies the adjoint of the forward
-    step), 2 + p the derivative dA/db_p, 2 + dim + i + dim j dA/dG_ij."""
+    1 its adjoint A^H (dim 2 or 3, N <= 32; step then applies the adjoint of
+    the forward step), 2 + p the derivative dA/db_p, 2 + dim + i + dim j
+    dA/dG_ij."""
 
     def __init__(self, B, dim, N=1, L=1.0, b=(0, 0, 0), G=None, Jt=None, dt=0.0, method="euler",
                  precision="fp64", operator=0):
@@ -611,6 +617,23 @@ class Plan:
                                      self._chk(out, self.ngrid, self.rdtype), float(scale), int(mode),
                                      self._stream()))
         return out
+
+    def reset_adjoint(self, g, out=None, scale=1.0):
+        """out = g + scale R^H g, R the reset's jump term (out may be g)."""
+        out = self.empty_grid() if out is None else out
+        _check(lib().hpup_plan_reset_adjoint(self._h, self._chk(g, self.ngrid, self.rdtype),
+                                             self._chk(out, self.ngrid, self.rdtype), float(scale), self._stream()))
+        return out
+
+    def reset_eps_grad(self, f, g, acc, scale=1.0):
+        """acc (CUDA float64, 1 element) += scale <g, dR/d epsilon f>."""
+        import torch
+        if not (acc.is_cuda and acc.dtype == torch.float64 and acc.numel() >= 1):
+            raise HpupError("acc must be a CUDA float64 tensor")
+        _check(lib().hpup_plan_reset_eps_grad(self._h, self._chk(f, self.ngrid, self.rdtype),
+                                              self._chk(g, self.ngrid, self.rdtype), float(scale),
+                                              ctypes.c_void_p(acc.data_ptr()), self._stream()))
+        return acc
 
     def hybrid_step(self, F, f, nsteps=1):
         """nsteps hybrid steps: integrator step, inverse transform, reset with

@@ -1,4 +1,5 @@
-"""Gradients through the spectral propagator of the reduced pendulum (dim 2).
+"""Gradients through the spectral propagator (the reduced pendulum's
+parameters, and the restitution coefficient of the hybrid step).
 
 No counterpart in the reference, which only propagates forward.  The drift
 operator A is linear and autonomous, and the plan evaluates one step of each
@@ -181,6 +182,75 @@ class TransformAdjoint:
         """A^H Fbar for coefficients Fbar (the plan's coefficient count)."""
         y = (Fbar.view(self.plan.nrc, -1) / self.lfac).reshape(-1).contiguous()
         return (self.plan.backward(y).view(self.gshape) * self.w).reshape(-1)
+
+
+class HybridGradient:
+    """Hybrid steps (pendulum_reduced_hybrid.m:224-234) and their adjoint
+    sweep, for gradients in the restitution coefficient epsilon (no
+    counterpart in the reference).  One step is
+
+        F'_k = P F_k  (the plan's integrator),  f_k = B F'_k,
+        g_k = f_k + dt R_eps f_k,               F_{k+1} = A g_k,
+
+    with B, A the inverse and forward transforms and R_eps the reset's jump
+    term, so with Fbar_{k+1} the incoming adjoint (dJ = Re<Fbar, dF>)
+
+        gbar_k = A^H Fbar_{k+1},   dJ/deps += dt <gbar_k, (dR/deps) f_k>,
+        fbar_k = gbar_k + dt R^H gbar_k,   Fbar_k = P^H B^H fbar_k,
+
+    P^H the adjoint plan's step (operator=1, dim 2 or 3), R^H and dR/deps the
+    plan's reset adjoint and restitution-derivative kernels.  The states F_k
+    are kept every `checkpoint` steps and recomputed on the way back; f_k is
+    recomputed from F_k (one step and one inverse transform)."""
+
+    def __init__(self, B, dim, N, L, reset, b=(0, 0, 0), G=None, Jt=None, dt=0.0, method="euler",
+                 precision="fp64"):
+        kw = dict(L=L, b=b, G=G, Jt=Jt, dt=dt, method=method, precision=precision)
+        self.dt = dt
+        self.reset = dict(reset)
+        self.fwd = Plan(B, dim, N, **kw)
+        self.fwd.set_reset(**self.reset)
+        self.adj = Plan(B, dim, N, operator=1, **kw)
+        self.ta = TransformAdjoint(self.fwd)
+
+    def _grid_before_reset(self, F):
+        Fp = self.fwd.step(F.clone(), 1)
+        return self.fwd.backward(Fp)
+
+    def run(self, F0, nsteps, dJ, checkpoint=None):
+        """Propagates F0 by nsteps hybrid steps and back-propagates dJ(F_K)
+        (the gradient of a real objective in the final coefficients).
+        Returns (F_K, grad_F0, {"epsilon": dJ/deps}); F0 is left unchanged."""
+        import torch
+        K = int(nsteps)
+        cp = max(1, int(checkpoint or K or 1))
+        checkpoints = [F0.clone()]
+        F = F0.clone()
+        f = self.fwd.empty_grid()
+        for k in range(K):
+            self.fwd.hybrid_step(F, f, 1)
+            if (k + 1) % cp == 0 and k + 1 < K:
+                checkpoints.append(F.clone())
+        FK = F
+        lam = dJ(FK).clone()
+        acc = torch.zeros(1, dtype=torch.float64, device=FK.device)
+        for seg in range(len(checkpoints) - 1, -1, -1):
+            k0 = seg * cp
+            k1 = min(K, k0 + cp)
+            states = [checkpoints[seg]]
+            for _ in range(k0, k1 - 1):
+                nxt = states[-1].clone()
+                self.fwd.hybrid_step(nxt, f, 1)
+                states.append(nxt)
+            for k in range(k1 - 1, k0 - 1, -1):
+                fk = self._grid_before_reset(states[k - k0])
+                gbar = self.ta.forward_adjoint(lam)
+                self.fwd.reset_eps_grad(fk, gbar, acc, self.dt)
+                del fk
+                fbar = self.fwd.reset_adjoint(gbar, gbar, self.dt)
+                lam = self.adj.step(self.ta.backward_adjoint(fbar).contiguous(), 1)
+            del states
+        return FK, lam, {"epsilon": float(acc.item())}
 
 
 def transform_functions(plan):

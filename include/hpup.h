@@ -263,9 +263,10 @@ int hpup_plan_destroy(hpup_plan* plan);
 /* Operator plans for the gradient harness.  No counterpart in the reference
  * (it has no adjoint); callers: gradient computations over the plan API.
  *   kind 0: the model's derivative operator A (as hpup_plan_create);
- *   kind 1: its adjoint A^H (reduced model, dim 2): hpup_plan_derivative applies
- *           A^H, hpup_plan_step applies P(dt A^H), the adjoint of the forward
- *           step P(dt A) for every integrator (P has real coefficients);
+ *   kind 1: its adjoint A^H (dim 2 or 3, N <= 32): hpup_plan_derivative
+ *           applies A^H, hpup_plan_step applies P(dt A^H), the adjoint of the
+ *           forward step P(dt A) for every integrator (P has real
+ *           coefficients); hpup_plan_adjoint_stage needs dim 2;
  *   kind 2 + p (p < dim): dA/db_p;  kind 2 + dim + i + dim j: dA/dG_ij
  *           (A is linear in b and G), for hpup_plan_derivative. */
 int hpup_plan_create_operator(const hpup_model* model, int kind, hpup_plan** plan);
@@ -377,6 +378,18 @@ int hpup_plan_reset_region(const hpup_plan* plan, int32_t* indR, int max_rows);
  *   mode 1: f_out = scale df(f_in)          (zero outside the region)
  * df is the reset law's jump term (outflow and inflow on the region's rows). */
 int hpup_plan_reset(hpup_plan* plan, const void* f_in, void* f_out, double scale, int mode, void* stream);
+/* Adjoint of the reset (no counterpart in the reference; callers: gradient
+ * computations through hybrid steps).  f_out = g + scale R^H g for the jump
+ * term R (its transpose under the plain grid inner product); f_out may be g.
+ * The transpose scatters with atomic adds: its last bits vary between runs. */
+int hpup_plan_reset_adjoint(hpup_plan* plan, const void* g, void* f_out, double scale, void* stream);
+/* Restitution derivative of the reset (no counterpart in the reference;
+ * callers: gradient computations): out_dev[0] += scale <g, (dR/d epsilon) f>
+ * for device grids f, g; out_dev is one device double.  The mask edges (box
+ * and contact-velocity tests) are held fixed: the derivative of the jump
+ * term where it is smooth in epsilon. */
+int hpup_plan_reset_eps_grad(hpup_plan* plan, const void* f, const void* g, double scale, double* out_dev,
+                             void* stream);
 /* nsteps hybrid steps (pendulum_reduced_hybrid.m:224-234): one step of the
  * plan's integrator on F_dev, f_dev = inverse transform, f_dev += dt df(f_dev),
  * F_dev = forward transform of f_dev.  f_dev (grid) receives the last step's

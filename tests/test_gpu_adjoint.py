@@ -222,7 +222,10 @@ def test_operator_kind_errors():
     with pytest.raises(hpup.HpupError):
         _plan(operator=8)
     with pytest.raises(hpup.HpupError):
-        hpup.Plan(4, 3, 8, L, precision="fp64", operator=1)  # the adjoint is built for dim 2
+        hpup.Plan(4, 3, 8, L, precision="fp64", operator=2 + 3 + 9)  # past the last parameter operator
+    big = hpup.Plan(3, 3, 40, L, precision="fp64", operator=1)  # the adjoint needs the fused path (N <= 32)
+    with pytest.raises(hpup.HpupError):
+        big.derivative(_rand(big.ncoef, big.cdtype, 1))
 
 
 def test_autograd_function():
