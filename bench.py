@@ -230,6 +230,117 @@ def hybrid_size_row(B=30, N=60, L=3.4, steps=3, precision="fp64"):
             "transform_backward_ms": ev[2].elapsed_time(ev[3]), "transform_forward_ms": ev[4].elapsed_time(ev[5])}
 
 
+# the large hybrid workload: B = 30 (60^3 rotation grid), 20^3 omega grid
+# over [-6, 6]^3, a 0.6 / 0.4 mixture of two matrix-Fisher x Gaussian
+# components (seeded rotations, F = U diag(12, 6, 2); Gaussians of unit width
+# at seeded means in [-1, 1]^3), the reflection reset with restitution 0.8 on
+# a wall tilted by 0.2 rad (probability-conserving 1/eps Jacobian), RK4 with
+# dt = 0.003
+HYB = dict(B=30, N=20, L=12.0, dt=0.003, eps=0.8, tilt=0.2,
+           reset=dict(d=0.12, h=0.2, r=0.025, theta_t=5 * 3.141592653589793 / 180, lambda_max=100.0 * 0.1173))
+
+
+def hybrid_plan(precision, B=None, N=None):
+    import numpy as np
+    import hpup
+    c = HYB
+    B, N = B or c["B"], N or c["N"]
+    p = hpup.Plan(B, 3, N, c["L"], b=(0.2, 0.2, 0.2), G=np.eye(3) * 0.1, Jt=np.diag([1.0, 0.93539, 0.24935]),
+                  dt=c["dt"], method="RK4", precision=precision)
+    p.set_reset(**c["reset"], epsilon=c["eps"], tilt=c["tilt"], jacobian=True)
+    return p
+
+
+def hybrid_initial(p):
+    import numpy as np
+    from hpup import density as D
+    rng = np.random.default_rng(5)
+    comps = [dict(s=(12, 6, 2), U=D.random_rotation(rng), mu=rng.uniform(-1, 1, 3), Sigma=np.eye(3))
+             for _ in range(2)]
+    f = D.mixture_density(p.B, 3, p.N, p.L, comps, (0.6, 0.4), dtype=p.rdtype)
+    F = p.forward(f)
+    del f
+    return F
+
+
+def hybrid_row(steps=3, err_steps=3):
+    """The large hybrid workload (HYB), FP32 plan: per hybrid step the RK4
+    step, the inverse transform, the reset (f += dt df on the grid) and the
+    forward transform, HIP events on the plan's stream; the FP32 coefficients
+    against an FP64 plan after err_steps hybrid steps; the restitution
+    gradient sweep (HybridGradient, checkpoint every step) against the
+    forward hybrid steps."""
+    import torch
+    from hpup.adjoint import HybridGradient
+    p = hybrid_plan("fp32")
+    F = hybrid_initial(p)
+    f = p.empty_grid()
+    p.hybrid_step(F, f, 1)  # warm-up
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(9)]
+    acc = [0.0] * 4
+    ev[8].record()
+    for _ in range(steps):
+        ev[0].record()
+        p.step(F, 1)
+        ev[1].record()
+        p.backward(F, f)
+        ev[2].record()
+        p.reset(f, f, p.model.dt, 0)
+        ev[3].record()
+        p.forward(f, F)
+        ev[4].record()
+        torch.cuda.synchronize()
+        for i in range(4):
+            acc[i] += ev[i].elapsed_time(ev[i + 1])
+    whole = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    whole[0].record()
+    p.hybrid_step(F, f, steps)
+    whole[1].record()
+    torch.cuda.synchronize()
+    step_ms = whole[0].elapsed_time(whole[1]) / steps
+    out = {"config": "hybrid SO(3)xR^3, B=%d, %d^3 omega grid over [-6,6]^3, 0.6/0.4 matrix-Fisher x Gaussian "
+                     "mixture, reflection reset eps=%.1f tilt=%.1f rad, RK4 dt=%.3f, FP32" %
+                     (HYB["B"], HYB["N"], HYB["eps"], HYB["tilt"], HYB["dt"]),
+           "coefficients": p.ncoef, "grid_points": p.ngrid, "region_rotations": int(len(p.reset_rows)),
+           "hybrid_step_ms": step_ms, "hybrid_steps_per_s": 1000.0 / step_ms,
+           "rk4_step_ms": acc[0] / steps, "transform_backward_ms": acc[1] / steps, "reset_ms": acc[2] / steps,
+           "transform_forward_ms": acc[3] / steps}
+    del F, f
+    torch.cuda.empty_cache()
+    # FP32 against FP64 after err_steps hybrid steps from the same initial coefficients
+    p64 = hybrid_plan("fp64")
+    F64 = hybrid_initial(p64)
+    F32 = F64.to(torch.complex64)
+    p64.hybrid_step(F64, p64.empty_grid(), err_steps)
+    g32 = p.empty_grid()
+    p.hybrid_step(F32, g32, err_steps)
+    torch.cuda.synchronize()
+    out["rel_l2_fp32_vs_fp64"] = float(torch.linalg.vector_norm(F32.to(torch.complex128) - F64) /
+                                       torch.linalg.vector_norm(F64))
+    out["rel_l2_after_steps"] = err_steps
+    del p64, F64, F32, g32
+    torch.cuda.empty_cache()
+    # restitution gradient sweep, FP32, K = 2 hybrid steps, checkpoint every step
+    K = 2
+    hg = HybridGradient(HYB["B"], 3, HYB["N"], HYB["L"], dict(HYB["reset"], epsilon=HYB["eps"], tilt=HYB["tilt"],
+                                                               jacobian=True),
+                        b=(0.2, 0.2, 0.2), G=[[0.1, 0, 0], [0, 0.1, 0], [0, 0, 0.1]],
+                        Jt=[[1.0, 0, 0], [0, 0.93539, 0], [0, 0, 0.24935]], dt=HYB["dt"], method="RK4",
+                        precision="fp32")
+    F0 = hybrid_initial(hg.fwd)
+    wv = F0.clone()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    _, _, g = hg.run(F0, K, lambda F: wv, checkpoint=1)
+    t1.record()
+    torch.cuda.synchronize()
+    out["restitution_gradient"] = g["epsilon"]
+    out["gradient_sweep_ms_per_step"] = t0.elapsed_time(t1) / K
+    out["gradient_over_forward"] = out["gradient_sweep_ms_per_step"] / step_ms
+    return out
+
+
 def row_timings(plan, F, reps=3):
     """The other rows of the path on this workload, timed with HIP events on
     torch's current stream (the plan's stream): inverse transform (coefficients
@@ -457,6 +568,12 @@ def main():
                 rows["hybrid_size"]["transform_forward_ms"]))
         except Exception as e:
             log("hybrid-size row failed:", repr(e))
+        try:
+            rows["hybrid"] = hybrid_row()
+            log("rows hybrid:", ", ".join("%s %s" % (k, ("%.4g" % v) if isinstance(v, float) else v)
+                                          for k, v in rows["hybrid"].items() if k != "config"))
+        except Exception as e:
+            log("hybrid row failed:", repr(e))
         try:
             rows["reset"] = reset_row()
             log("rows reset:", ", ".join("%s %s" % (k, ("%.2f" % v) if isinstance(v, float) else v)

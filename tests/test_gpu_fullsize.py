@@ -104,3 +104,48 @@ def test_gradient_sweep_full_size():
     _, gF1, grads1 = gr.run(F0, K, lambda F: w, checkpoint=1)
     assert torch.equal(gF1, gF0)
     assert np.array_equal(grads1["b"], grads["b"]) and np.array_equal(grads1["G"], grads["G"])
+
+
+def test_hybrid_full_size():
+    """The large hybrid workload (B = 30, 60^3 rotation grid, 20^3 omega grid
+    over [-6, 6]^3, 0.6/0.4 matrix-Fisher x Gaussian mixture, reflection reset
+    with eps = 0.8 on a tilted wall, RK4 dt = 0.003), FP32 plan: the hybrid
+    step is linear in F (every part is: drift, transforms and the reset for a
+    fixed eps; 1e-5 relative in FP32), the post-reset grid it returns
+    transforms to the coefficients it returns (1e-6), and the reset changes
+    the total probability by dt times the grid quadrature of df."""
+    import importlib.util
+    import os
+    import torch
+    _print("hybrid full size")
+    spec = importlib.util.spec_from_file_location(
+        "bench_module", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py"))
+    bm = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bm)
+    p = bm.hybrid_plan("fp32")
+    F1 = bm.hybrid_initial(p)
+    assert len(p.reset_rows) > 0
+    F2 = torch.roll(F1, 7)  # a second, different coefficient set
+    f = p.empty_grid()
+    H1 = p.hybrid_step(F1.clone(), f, 1)[0]
+    H2 = p.hybrid_step(F2.clone(), f, 1)[0]
+    H12 = p.hybrid_step(F1 + 2 * F2, f, 1)[0]
+    torch.cuda.synchronize()
+    assert _rel(H12, H1 + 2 * H2) < 1e-5
+    # F_{k+1} is the forward transform of the grid the step returns
+    F = F1.clone()
+    p.hybrid_step(F, f, 1)
+    assert _rel(p.forward(f), F) < 1e-6
+    # mass: the drift conserves it, the reset moves it by dt sum w df dv
+    m = p.mass_index
+    g = p.backward(p.step(F1.clone(), 1))
+    df = p.reset(g, mode=1)
+    from hpup import density as D
+    wgt = D.beta_weights(p.B)[1]
+    wj = torch.tensor(wgt, device="cuda").view(1, 1, 2 * p.B, 1)
+    dmass = float((df.double().view(-1, 2 * p.B, 2 * p.B, 2 * p.B) * wj).sum())
+    F = F1.clone()
+    p.hybrid_step(F, f, 1)
+    torch.cuda.synchronize()
+    got = float(F[m].real - F1[m].real)
+    assert abs(got - p.model.dt * dmass) <= 1e-4 * abs(F1[m].real) + 1e-3 * abs(got)
