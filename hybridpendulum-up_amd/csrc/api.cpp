@@ -255,7 +255,7 @@ struct PlanBase {
     virtual void step(void* F, int nsteps, hipStream_t st) = 0;
     virtual void derivative(const void* F, void* dF, hipStream_t st) = 0;
     virtual void adjoint_stage(const void* ybar, const void* y, double scale, void* out, void* fbar,
-                               int fbar_init, double* gout, hipStream_t st) = 0;
+                               int fbar_init, double* gout, hipStream_t st, bool noalloc = false) = 0;
     virtual void axpy_stage(const void* in, const void* F0, double scale, void* out, hipStream_t st) = 0;
     virtual void full_to_compact(const void* a, void* b, hipStream_t st) = 0;
     virtual void compact_to_full(const void* a, void* b, hipStream_t st) = 0;
@@ -505,19 +505,30 @@ template <class T> struct PlanT : PlanBase {
     // out = scale A^H ybar, fbar += out (when fbar is given; fbar = ybar + out
     // with fbar_init), gout[0..4] += scale Re<ybar, (dA/dtheta) y> for
     // theta = b_1, b_2, G_00, G_11 and G_01 (= the G_10 term).
+    // noalloc: the workgroups' partial sums go to the static device array
+    // (grad_scratch) instead of the plan's buffer, and nothing is allocated
     void adjoint_stage(const void* ybar, const void* y, double scale, void* out, void* fbar, int fbar_init,
-                       double* gout, hipStream_t st) override {
+                       double* gout, hipStream_t st, bool noalloc) override {
         HPUP_REQUIRE(has_drift && fused() && dr.view.adj, HPUP_ERR_UNSUPPORTED,
                      "adjoint_stage needs an adjoint operator plan (operator kind 1, N <= 32)");
         HPUP_REQUIRE(ybar && y && out && gout, HPUP_ERR_ARG, "null argument");
         HPUP_REQUIRE(ybar != out && y != out, HPUP_ERR_ARG, "adjoint stage cannot run in place");
         HPUP_REQUIRE(fbar != ybar && fbar != y && fbar != out, HPUP_ERR_ARG, "fbar must not alias the stage arrays");
-        ensure(gpart, sizeof(double) * kGradMaxWG * 5);
+        HPUP_REQUIRE(model.dim == 2, HPUP_ERR_UNSUPPORTED, "parameter gradients: reduced model (dim 2) only");
+        double* part;
+        if (noalloc) {
+            int dev = 0;
+            HPUP_HIP(hipGetDevice(&dev));
+            part = grad_scratch(dev);
+        } else {
+            ensure(gpart, sizeof(double) * kGradMaxWG * 5);
+            part = gpart.as<double>();
+        }
         DriftDev<T> D = dr.view;
         D.gy = y;
         D.gacc = fbar;
         D.gacc_init = fbar_init ? 1 : 0;
-        D.gpart = gpart.as<double>();
+        D.gpart = part;
         D.gout = gout;
         D.gscale = scale;
         launch_drift_fused<T>(D, dr.mix, ybar, Z3.p, nullptr, out, ACC.p,
@@ -788,6 +799,13 @@ int hpup_plan_axpy_stage(hpup_plan* p, const void* in, const void* F0, double sc
 int hpup_plan_adjoint_stage(hpup_plan* p, const void* ybar, const void* y, double scale, void* ybar_out,
                             void* fbar, int fbar_init, double* grad, void* stream) {
     return guard([&] { p->impl->adjoint_stage(ybar, y, scale, ybar_out, fbar, fbar_init, grad, (hipStream_t)stream); });
+}
+int hpup_plan_adjoint_stage_noalloc(hpup_plan* p, const void* ybar, const void* y, double scale, void* ybar_out,
+                                    void* fbar, int fbar_init, double* grad, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        p->impl->adjoint_stage(ybar, y, scale, ybar_out, fbar, fbar_init, grad, (hipStream_t)stream, true);
+    });
 }
 int hpup_plan_full_to_compact(hpup_plan* p, const void* a, void* b, void* stream) {
     return guard([&] { p->impl->full_to_compact(a, b, (hipStream_t)stream); });

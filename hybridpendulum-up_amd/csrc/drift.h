@@ -52,6 +52,10 @@ template <class T> struct DriftDev {
     double gscale;
 };
 constexpr int kGradMaxWG = 4096;
+constexpr int kMaxDevices = 64;
+// Device address of the gradient stage's static partial-sum array (kGradMaxWG
+// x 5 doubles) on `device`, which must be current; resolved once per device.
+double* grad_scratch(int device);
 
 // Integrator epilogue of the slab kernel:
 //   Fout = F0 + dt (ca ACC + cd dF);  ACC <- ea ACC + ed dF

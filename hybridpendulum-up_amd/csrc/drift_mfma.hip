@@ -21,12 +21,36 @@
 // k_axis3 computes Z3 = T3 F (circulant along the outermost omega axis) as the
 // GEMM C3 [F_re; F_im] over 64-column chunks of the k1,k2 plane.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "drift.h"
 #include "kernels_common.h"
 
 namespace hpup {
+
+// Partial sums of the gradient stage for callers that cannot allocate:
+// kGradMaxWG x 5 doubles (160 KiB), held by every device the library runs on
+// until the process ends.  Host code never uses this name as an address: the
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[kGradMaxWG * 5];
+
+double* grad_scratch(int device) {
+    static std::mutex mu;
+    static double* ptr[kMaxDevices] = {};
+    HPUP_REQUIRE(device >= 0 && device < kMaxDevices, HPUP_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[device]) {
+        int cur = -1;
+        HPUP_HIP(hipGetDevice(&cur));
+        HPUP_REQUIRE(cur == device, HPUP_ERR_ARG, "grad_scratch: the device must be current");
+        void* a = nullptr;
+        HPUP_HIP(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+        HPUP_REQUIRE(a != nullptr, HPUP_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        ptr[device] = static_cast<double*>(a);
+    }
+    return ptr[device];
+}
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));

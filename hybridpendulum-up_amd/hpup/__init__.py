@@ -83,7 +83,7 @@ EXPORTS = [
     "hpup_pendulum_plot_getc", "hpup_gyrobias_propagate", "hpup_plan_create_operator",
     "hpup_plan_adjoint_stage", "hpup_plan_axpy_stage",
     "hpup_plan_set_reset", "hpup_plan_reset_region", "hpup_plan_reset", "hpup_plan_hybrid_step",
-    "hpup_plan_reset_adjoint", "hpup_plan_reset_eps_grad",
+    "hpup_plan_reset_adjoint", "hpup_plan_reset_eps_grad", "hpup_plan_adjoint_stage_noalloc",
 ]
 
 
@@ -116,6 +116,7 @@ def lib():
         L.hpup_plan_adjoint_stage.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                               ctypes.c_void_p]
+        L.hpup_plan_adjoint_stage_noalloc.argtypes = L.hpup_plan_adjoint_stage.argtypes
         L.hpup_plan_destroy.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_coeffs.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_grid.argtypes = [ctypes.c_void_p]

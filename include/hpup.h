@@ -292,6 +292,18 @@ int hpup_plan_axpy_stage(hpup_plan* plan, const void* in, const void* F0, double
 int hpup_plan_adjoint_stage(hpup_plan* plan, const void* ybar, const void* y, double scale, void* ybar_out,
                             void* fbar, int fbar_init, double* grad, void* stream);
 
+/* hpup_plan_adjoint_stage without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same arguments, kernels and results
+ * as hpup_plan_adjoint_stage; the workgroups' partial sums go to a static
+ * device array of the library (4096 x 5 doubles per GPU, held until the
+ * process ends) instead of a plan buffer, and launches needing more than
+ * 4096 workgroups are rejected with HPUP_ERR_UNSUPPORTED.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race. */
+int hpup_plan_adjoint_stage_noalloc(hpup_plan* plan, const void* ybar, const void* y, double scale,
+                                    void* ybar_out, void* fbar, int fbar_init, double* grad, void* stream);
+
 /* Number of compact coefficients: nRc * nx, nRc = B(2B-1)(2B+1)/3. */
 int64_t hpup_plan_num_coeffs(const hpup_plan* plan);
 /* Number of grid points: (2B)^3 * nx. */
