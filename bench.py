@@ -101,14 +101,16 @@ def timed_steps(plan, F, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def gradient_row(B=15, N=30, L=10.0, K=20, precision="fp64", checkpoints=(1, 5, 20)):
+def gradient_row(B=15, N=30, L=10.0, K=20, precision="fp64", checkpoints=(1, 5, 20), dt=0.004):
     """Cost of the gradient sweep (hpup.adjoint, reduced model, RK4) against a
-    forward step on a B = 15, 30^2 omega grid: per step of the sweep beyond the
-    forward pass, for checkpoints every 1, 5 and 20 steps."""
+    forward step: per step of the sweep beyond the forward pass, for the given
+    checkpoint spacings (defaults: B = 15, 30^2 omega grid, every 1, 5 and 20
+    steps; the gradient workload itself is B = 10, 32^2 over [-4, 4]^2, 50
+    steps of 0.01, checkpoint every 10)."""
     import numpy as np
     import torch
     from hpup.adjoint import Gradient
-    gr = Gradient(B, N, L, b=(0.2, 0.2), G=np.eye(2) * 0.1, dt=0.004, method="rk4", precision=precision)
+    gr = Gradient(B, N, L, b=(0.2, 0.2), G=np.eye(2) * 0.1, dt=dt, method="rk4", precision=precision)
     g = torch.Generator(device="cpu").manual_seed(5)
     F0 = (torch.randn(gr.fwd.ncoef, generator=g, dtype=torch.complex128) * 1e-3).to(gr.fwd.cdtype).cuda()
     w = torch.randn(gr.fwd.ncoef, generator=g, dtype=torch.complex128).to(gr.fwd.cdtype).cuda()
@@ -257,7 +259,7 @@ def hybrid_initial(p):
     rng = np.random.default_rng(5)
     comps = [dict(s=(12, 6, 2), U=D.random_rotation(rng), mu=rng.uniform(-1, 1, 3), Sigma=np.eye(3))
              for _ in range(2)]
-    f = D.mixture_density(p.B, 3, p.N, p.L, comps, (0.6, 0.4), dtype=p.rdtype)
+    f = D.mixture_density(p.B, 3, p.N, p.model.L, comps, (0.6, 0.4), dtype=p.rdtype)
     F = p.forward(f)
     del f
     return F
@@ -559,6 +561,12 @@ def main():
             for key in ("gradient", "gradient_fp32"):
                 log("rows %s: sweep / forward step %s (checkpoint 1, 5, 20)" % (key, ", ".join(
                     "%.2f" % rows[key]["sweep_over_forward_checkpoint_%d" % cp] for cp in (1, 5, 20))))
+            for prec in ("fp64", "fp32"):
+                key = "gradient_workload" + ("" if prec == "fp64" else "_fp32")
+                rows[key] = gradient_row(B=10, N=32, L=8.0, K=50, precision=prec, checkpoints=(1, 10), dt=0.01)
+                log("rows %s: sweep / forward step %.2f (checkpoint 10), %.2f (checkpoint 1); forward %.3f ms"
+                    % (key, rows[key]["sweep_over_forward_checkpoint_10"], rows[key]["sweep_over_forward_checkpoint_1"],
+                       rows[key]["forward_step_ms"]))
         except Exception as e:  # an optional row must not hide the headline result
             log("gradient row failed:", repr(e))
         try:

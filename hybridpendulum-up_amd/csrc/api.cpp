@@ -257,6 +257,9 @@ struct PlanBase {
     virtual void adjoint_stage(const void* ybar, const void* y, double scale, void* out, void* fbar,
                                int fbar_init, double* gout, hipStream_t st, bool noalloc = false) = 0;
     virtual void axpy_stage(const void* in, const void* F0, double scale, void* out, hipStream_t st) = 0;
+    virtual void step_stages(const void* F, void* const* ys, void* Fnext, hipStream_t st) = 0;
+    virtual void adjoint_step(const void* F, const void* const* ys, const void* lam, void* Fbar, double* gout,
+                              hipStream_t st) = 0;
     virtual void full_to_compact(const void* a, void* b, hipStream_t st) = 0;
     virtual void compact_to_full(const void* a, void* b, hipStream_t st) = 0;
     // device pass of get_stat: rotation marginal, first omega moment per
@@ -287,7 +290,7 @@ template <class T> struct PlanT : PlanBase {
     DriftTables<T> dr;
     bool has_drift = false;
     bool bf16x3 = false;  // circulants as 3xBF16 split products (FP32 storage)
-    DevBuf G, Wb, V1, V2, Z3, S3, Fs, Fs2, ACC, tmp, st_work, st_out, gpart, dfc, epart;
+    DevBuf G, Wb, V1, V2, Z3, S3, Fs, Fs2, ACC, tmp, st_work, st_out, gpart, dfc, epart, Ya, Yb;
 
     void init_xform(const double* d, const double* wv) {
         const hpup_model& m = model;
@@ -500,6 +503,52 @@ template <class T> struct PlanT : PlanBase {
         ensure_drift_buffers(false);
         stage(in, F0, out, StageCoef<T>{(T)scale, 0, 1, 0, 0, 1, 0, 0, 0}, st);
         HPUP_HIP(hipGetLastError());
+    }
+    // The Horner coefficients c_j of the plan's integrator: one step is
+    // y_j = F + c_j dt A y_{j-1}, j = 1..s, y_0 = F, F_next = y_s
+    int horner(double* c) const {
+        switch (model.method) {
+        case HPUP_EULER: c[0] = 1.0; return 1;
+        case HPUP_MIDPOINT:
+        case HPUP_RK2: c[0] = 0.5; c[1] = 1.0; return 2;
+        case HPUP_RK4: c[0] = 0.25; c[1] = 1.0 / 3.0; c[2] = 0.5; c[3] = 1.0; return 4;
+        default: throw Error{HPUP_ERR_METHOD, "unknown integrator"};
+        }
+    }
+    // ys[0..s-2] = y_1..y_{s-1} of one step from F; Fnext (nullable) = y_s
+    void step_stages(const void* F, void* const* ys, void* Fnext, hipStream_t st) override {
+        HPUP_REQUIRE(!den && !dr.view.adj, HPUP_ERR_UNSUPPORTED, "step stages: a forward model plan");
+        HPUP_REQUIRE(F && (ys || model.method == HPUP_EULER), HPUP_ERR_ARG, "null argument");
+        double c[4];
+        const int ns = horner(c);
+        const void* prev = F;
+        for (int j = 1; j < ns; j++) {
+            HPUP_REQUIRE(ys[j - 1] && ys[j - 1] != F, HPUP_ERR_ARG, "stage buffers must be distinct from F");
+            axpy_stage(prev, F, c[j - 1] * model.dt, ys[j - 1], st);
+            prev = ys[j - 1];
+        }
+        if (Fnext) {
+            HPUP_REQUIRE(Fnext != prev, HPUP_ERR_ARG, "F_next must not alias the last stage value");
+            axpy_stage(prev, F, c[ns - 1] * model.dt, Fnext, st);
+        }
+    }
+    // The whole reverse step on an adjoint plan from the forward stage values
+    // y_0 = F, y_1..y_{s-1} = ys: ybar_s = lam, ybar_{j-1} = c_j dt A^H ybar_j,
+    // Fbar = sum_j ybar_j, gout += the parameter terms of every stage
+    void adjoint_step(const void* F, const void* const* ys, const void* lam, void* Fbar, double* gout,
+                      hipStream_t st) override {
+        HPUP_REQUIRE(F && lam && Fbar && gout && (ys || model.method == HPUP_EULER), HPUP_ERR_ARG, "null argument");
+        double c[4];
+        const int ns = horner(c);
+        ensure(Ya, cbytes());
+        ensure(Yb, cbytes());
+        const void* ybar = lam;
+        for (int j = ns - 1; j >= 0; j--) {
+            const void* y = j == 0 ? F : ys[j - 1];
+            void* out = (ybar == Ya.p) ? Yb.p : Ya.p;
+            adjoint_stage(ybar, y, c[j] * model.dt, out, Fbar, j == ns - 1 ? 1 : 0, gout, st, false);
+            ybar = out;
+        }
     }
     // One reverse-mode stage of the gradient sweep on an adjoint plan:
     // out = scale A^H ybar, fbar += out (when fbar is given; fbar = ybar + out
@@ -799,6 +848,19 @@ int hpup_plan_axpy_stage(hpup_plan* p, const void* in, const void* F0, double sc
 int hpup_plan_adjoint_stage(hpup_plan* p, const void* ybar, const void* y, double scale, void* ybar_out,
                             void* fbar, int fbar_init, double* grad, void* stream) {
     return guard([&] { p->impl->adjoint_stage(ybar, y, scale, ybar_out, fbar, fbar_init, grad, (hipStream_t)stream); });
+}
+int hpup_plan_step_stages(hpup_plan* p, const void* F, void* const* ys, void* F_next, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        p->impl->step_stages(F, ys, F_next, (hipStream_t)stream);
+    });
+}
+int hpup_plan_adjoint_step(hpup_plan* p, const void* F, const void* const* ys, const void* lam, void* Fbar,
+                           double* grad, void* stream) {
+    return guard([&] {
+        HPUP_REQUIRE(p, HPUP_ERR_ARG, "null argument");
+        p->impl->adjoint_step(F, ys, lam, Fbar, grad, (hipStream_t)stream);
+    });
 }
 int hpup_plan_adjoint_stage_noalloc(hpup_plan* p, const void* ybar, const void* y, double scale, void* ybar_out,
                                     void* fbar, int fbar_init, double* grad, void* stream) {

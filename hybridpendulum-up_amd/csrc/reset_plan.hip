@@ -15,7 +15,11 @@
 // dim 3 restates the same law with a full body angular velocity and a wall
 // normal e_w = (cos tau, 0, sin tau) (tau = 0 is the reference's wall):
 // theta = asin(r3 . e_w), PC = (h - r tan theta) r3 + r sec theta e_w,
-// t = r3 x e_w / |r3 x e_w|, vC(omega) = (omega x PC) . e_w, trilinear
+// t = r3 x e_w / |r3 x e_w|, vC(omega) = (omega x PC) . e_w = sigma (omega . t)
+// with sigma = (PC x e_w) . t, so that vC(omega-) = -vC(omega)/eps and one
+// sign decides both contact tests, independently of eps (outflow where it is
+// positive, inflow where it is negative, neither on the switching plane
+// where the map is the identity); trilinear
 // interpolation, and J = 1/eps (|det g|, which conserves probability) or 1.
 //
 // Layouts: the plan's grid (nR, nx) with the rotation index fastest; the
@@ -35,6 +39,14 @@ namespace {
 
 // per-region geometry, structure of arrays: [c * numR + k]
 enum { G_R = 0, G_T = 9, G_PC = 12, G_LAM = 15, G_N = 16 };
+
+// sigma = (PC x e_w) . t = (h - r tan theta) |r3 x e_w|: the contact velocity
+// (omega x PC) . e_w equals sigma (omega . t)
+__device__ __forceinline__ double contact_sigma(const double* P, const double* ew, const double* t) {
+#pragma clang fp contract(off)
+    const double c0 = P[1] * ew[2] - P[2] * ew[1], c1 = P[2] * ew[0] - P[0] * ew[2], c2 = P[0] * ew[1] - P[1] * ew[0];
+    return c0 * t[0] + c1 * t[1] + c2 * t[2];
+}
 
 template <class T, int DIM>
 __global__ __launch_bounds__(256) void k_reset_df(const T* __restrict__ f, const double* __restrict__ geo,
@@ -97,25 +109,23 @@ __global__ __launch_bounds__(256) void k_reset_df(const T* __restrict__ f, const
         double w[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) w[a] = Rr[a] * Om[0] + Rr[a + 3] * Om[1] + Rr[a + 6] * Om[2];
-        auto vc = [&](const double* v) {
-            const double x0 = v[1] * P[2] - v[2] * P[1], x1 = v[2] * P[0] - v[0] * P[2],
-                         x2 = v[0] * P[1] - v[1] * P[0];
-            return x0 * c.ew[0] + x1 * c.ew[1] + x2 * c.ew[2];
-        };
-        out = vc(w) < 0.0 ? 0.0 : -lam * (double)f[iR + nR * ix];
+        // vC(omega) = sigma (omega . t) and vC(omega-) = -vC(omega)/eps: one
+        // sign decides both contact tests (outflow > 0, inflow < 0; on the
+        // switching plane, where the map is the identity, neither)
+        const double sigma = contact_sigma(P, c.ew, t);
+        const double vq = sigma * (w[0] * t[0] + w[1] * t[1] + w[2] * t[2]);
+        out = vq > 0.0 ? -lam * (double)f[iR + nR * ix] : 0.0;
         const double fac = c.kfac * (w[0] * t[0] + w[1] * t[1] + w[2] * t[2]);
 #pragma unroll
         for (int a = 0; a < 3; a++) w[a] -= fac * t[a];
-        double Oo[3], wo[3];
+        double Oo[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) Oo[a] = Rr[3 * a] * w[0] + Rr[3 * a + 1] * w[1] + Rr[3 * a + 2] * w[2];
-#pragma unroll
-        for (int a = 0; a < 3; a++) wo[a] = Rr[a] * Oo[0] + Rr[a + 3] * Oo[1] + Rr[a + 6] * Oo[2];
         bool inside = true;
 #pragma unroll
         for (int a = 0; a < 3; a++)
             if (!(Oo[a] >= c.xlo && Oo[a] <= c.xhi)) inside = false;
-        if (inside && !(vc(wo) < 0.0)) {
+        if (inside && vq < 0.0) {
             int64_t cl[3];
             double s[3];
 #pragma unroll
@@ -214,21 +224,16 @@ __device__ __forceinline__ void jump_at(const double* Rr, const double* t, const
         const double Om[3] = {c.x0 + c.hx * (double)i0, c.x0 + c.hx * (double)i1, c.x0 + c.hx * (double)i2};
         double w[3];
         for (int a = 0; a < 3; a++) w[a] = Rr[a] * Om[0] + Rr[a + 3] * Om[1] + Rr[a + 6] * Om[2];
-        auto vc = [&](const double* v) {
-            const double x0 = v[1] * P[2] - v[2] * P[1], x1 = v[2] * P[0] - v[0] * P[2],
-                         x2 = v[0] * P[1] - v[1] * P[0];
-            return x0 * c.ew[0] + x1 * c.ew[1] + x2 * c.ew[2];
-        };
-        J.out_on = !(vc(w) < 0.0);
         const double dot = w[0] * t[0] + w[1] * t[1] + w[2] * t[2];
+        const double vq = contact_sigma(P, c.ew, t) * dot;
+        J.out_on = vq > 0.0;
         const double fac = c.kfac * dot;
         for (int a = 0; a < 3; a++) w[a] -= fac * t[a];
-        double Oo[3], wo[3];
+        double Oo[3];
         for (int a = 0; a < 3; a++) Oo[a] = Rr[3 * a] * w[0] + Rr[3 * a + 1] * w[1] + Rr[3 * a + 2] * w[2];
-        for (int a = 0; a < 3; a++) wo[a] = Rr[a] * Oo[0] + Rr[a + 3] * Oo[1] + Rr[a + 6] * Oo[2];
         for (int a = 0; a < 3; a++)
             if (!(Oo[a] >= c.xlo && Oo[a] <= c.xhi)) return;
-        if (vc(wo) < 0.0) return;
+        if (!(vq < 0.0)) return;
         J.in_on = true;
         int64_t cl[3];
         double s[3], ds[3];

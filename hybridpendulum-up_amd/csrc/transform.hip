@@ -547,20 +547,75 @@ template <class T, int DIR> __device__ __forceinline__ void fft30(cT<T>* x) {
             for (int k3 = 0; k3 < 5; k3++) x[(15 * k1 + 10 * k2 + 6 * k3) % 30] = a[k1][k2][k3];
 }
 
-// Inverse (alpha, gamma) stage at B = 15: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}
+template <class T, int DIR> __device__ __forceinline__ void dft4(cT<T>* x) {
+    const cT<T> a = cadd<T>(x[0], x[2]), b = csub<T>(x[0], x[2]);
+    const cT<T> c = cadd<T>(x[1], x[3]), d = irot<T, DIR>(csub<T>(x[1], x[3]));
+    x[0] = cadd<T>(a, c);
+    x[2] = csub<T>(a, c);
+    x[1] = cadd<T>(b, d);
+    x[3] = csub<T>(b, d);
+}
+
+// 60 = 4 * 3 * 5, Good-Thomas with the input map n = 15 n1 + 20 n2 + 12 n3
+// (mod 60) and the output map k = 45 k1 + 40 k2 + 36 k3 (mod 60), the CRT
+// idempotents (45 = 1 mod 4, 40 = 1 mod 3, 36 = 1 mod 5, each 0 modulo the
+// other factors): then n k = 15 n1 k1 + 20 n2 k2 + 12 n3 k3 (mod 60), a
+// twiddle-free 4 x 3 x 5 transform.
+template <class T, int DIR> __device__ __forceinline__ void fft60(cT<T>* x) {
+    cT<T> a[4][3][5];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; n1++)
+#pragma unroll
+        for (int n2 = 0; n2 < 3; n2++)
+#pragma unroll
+            for (int n3 = 0; n3 < 5; n3++) a[n1][n2][n3] = x[(15 * n1 + 20 * n2 + 12 * n3) % 60];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; n2++)
+#pragma unroll
+        for (int n3 = 0; n3 < 5; n3++) {
+            cT<T> v[4] = {a[0][n2][n3], a[1][n2][n3], a[2][n2][n3], a[3][n2][n3]};
+            dft4<T, DIR>(v);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) a[k1][n2][n3] = v[k1];
+        }
+#pragma unroll
+    for (int n1 = 0; n1 < 4; n1++)
+#pragma unroll
+        for (int n3 = 0; n3 < 5; n3++) dft3<T, DIR>(a[n1][0][n3], a[n1][1][n3], a[n1][2][n3]);
+#pragma unroll
+    for (int n1 = 0; n1 < 4; n1++)
+#pragma unroll
+        for (int n2 = 0; n2 < 3; n2++) dft5<T, DIR>(a[n1][n2]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++)
+#pragma unroll
+        for (int k2 = 0; k2 < 3; k2++)
+#pragma unroll
+            for (int k3 = 0; k3 < 5; k3++) x[(45 * k1 + 40 * k2 + 36 * k3) % 60] = a[k1][k2][k3];
+}
+
+// the length-2B transform of the register-row FFT kernels (2B = 30 or 60)
+template <class T, int DIR, int NF> __device__ __forceinline__ void fftn(cT<T>* x) {
+    if constexpr (NF == 30) fft30<T, DIR>(x);
+    else fft60<T, DIR>(x);
+}
+
+// Inverse (alpha, gamma) stage at 2B = 30 or 60: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}
 // (item (m, xl): its G row, frequency n at index n mod 30), then
 // f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a} (item (g, xl): its H column).
 // Row stride (elements) of the FFT kernels' H tile [m][g][xl]: padded so that
 // the lanes of a wave, which write or read one g of eight (FP32) or four
 // (FP64) rows m at once, spread over all LDS banks (2-way for 8-byte and
 // 4-way for 16-byte elements, the minimum for a wave's 512 or 1024 bytes)
-template <class T> __host__ __device__ constexpr int fft_sh(int XC) { return 30 * XC + (sizeof(T) == 4 ? 8 : 4); }
+template <class T> __host__ __device__ constexpr int fft_sh(int XC, int n2 = 30) {
+    return n2 * XC + (sizeof(T) == 4 ? 8 : 4);
+}
 
-template <class T, int XC, int NT>
-__global__ __launch_bounds__(NT) void k_inv_ag_fft30(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
-                                                     int nxc) {
+template <class T, int B, int XC, int NT>
+__global__ __launch_bounds__(NT) void k_inv_ag_fft(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
+                                                   int nxc) {
     extern __shared__ char smem_raw[];
-    constexpr int B = 15, n2 = 30, M = 29, SH = fft_sh<T>(XC);
+    constexpr int n2 = 2 * B, M = 2 * B - 1, SH = fft_sh<T>(XC, n2);
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl], row stride SH
     const int w = xcd_major(blockIdx.x, gridDim.x);
     const int j = w / nxc;
@@ -579,7 +634,7 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft30(XformDev<T> X, const cT<T>*
             v[(ni - (B - 1) + n2) % n2] = ld ? g : czero<T>();
         }
         v[B] = czero<T>();
-        fft30<T, -1>(v);
+        fftn<T, -1, n2>(v);
 #pragma unroll
         for (int g = 0; g < n2; g++) sH[mi * SH + g * XC + xl] = v[g];
     }
@@ -591,7 +646,7 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft30(XformDev<T> X, const cT<T>*
 #pragma unroll
         for (int mi = 0; mi < M; mi++) v[(mi - (B - 1) + n2) % n2] = sH[mi * SH + g * XC + xl];
         v[B] = czero<T>();
-        fft30<T, -1>(v);
+        fftn<T, -1, n2>(v);
         if (x < nx) {
             T* fo = f + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x));
 #pragma unroll
@@ -600,14 +655,14 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft30(XformDev<T> X, const cT<T>*
     }
 }
 
-// Forward (alpha, gamma) stage at B = 15: H[m][g] = sum_a f[a][g] e^{+i m alpha_a},
+// Forward (alpha, gamma) stage at 2B = 30 or 60: H[m][g] = sum_a f[a][g] e^{+i m alpha_a},
 // m >= 0 (item (g, xl): its f row), then G[m][n] = sum_g H[m][g] e^{+i n gamma_g}
 // and G[-m][-n] = conj (item (m, xl): its H row).
-template <class T, int XC, int NT>
-__global__ __launch_bounds__(NT) void k_fwd_ag_fft30(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
-                                                     int wstride, cT<T>* __restrict__ G, int nxc) {
+template <class T, int B, int XC, int NT>
+__global__ __launch_bounds__(NT) void k_fwd_ag_fft(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
+                                                   int wstride, cT<T>* __restrict__ G, int nxc) {
     extern __shared__ char smem_raw[];
-    constexpr int B = 15, n2 = 30, lmax = 14, M = 29, rp = n2 + 1, SH = fft_sh<T>(XC);
+    constexpr int n2 = 2 * B, lmax = B - 1, M = 2 * B - 1, rp = n2 + 1, SH = fft_sh<T>(XC, n2);
     T* sf = reinterpret_cast<T*>(smem_raw);          // [xl][g][a]  XC*n2*rp
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m>=0][g][xl], row stride SH, aliases sf
     const int w = xcd_major(blockIdx.x, gridDim.x);
@@ -636,7 +691,7 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_fft30(XformDev<T> X, const T* __r
     }
     __syncthreads();  // sH overwrites sf
     if (actA) {
-        fft30<T, 1>(v);
+        fftn<T, 1, n2>(v);
 #pragma unroll
         for (int m = 0; m < B; m++) sH[m * SH + gA * XC + xlA] = v[m];
     }
@@ -646,7 +701,7 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_fft30(XformDev<T> X, const T* __r
         const int64_t x = x0 + xl;
 #pragma unroll
         for (int g = 0; g < n2; g++) v[g] = sH[m * SH + g * XC + xl];
-        fft30<T, 1>(v);
+        fftn<T, 1, n2>(v);
         if (x < nx) {
             cT<T>* Gx = G + g_off(B, 0, j, x);
 #pragma unroll
@@ -685,17 +740,39 @@ constexpr int kFftInvXC = HPUP_FFT_INV_XC, kFftFwdXC = HPUP_FFT_FWD_XC, kFftNT =
 constexpr int kFwdXC = 8;
 template <class T> constexpr int inv_xc() { return sizeof(T) == 8 ? 4 : 8; }
 
+template <class T, int B, int XC, int NT>
+static void fwd_ag_fft(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
+    constexpr int n2 = 2 * B;
+    static_assert(n2 * XC <= NT, "one alpha row per thread");
+    const int nxc = (int)((X.nx + XC - 1) / XC);
+    HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+    const size_t sh = std::max(sizeof(T) * XC * n2 * (n2 + 1), sizeof(cT<T>) * B * fft_sh<T>(XC, n2));
+    allow_lds((const void*)k_fwd_ag_fft<T, B, XC, NT>, sh);
+    hipLaunchKernelGGL((k_fwd_ag_fft<T, B, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X, (const T*)f,
+                       (const T*)wR, wstride, (cT<T>*)G, nxc);
+}
+
+template <class T, int B, int XC, int NT>
+static void inv_ag_fft(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
+    constexpr int n2 = 2 * B, M = 2 * B - 1;
+    const int nxc = (int)((X.nx + XC - 1) / XC);
+    HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+    const size_t sh = sizeof(cT<T>) * M * fft_sh<T>(XC, n2);
+    allow_lds((const void*)k_inv_ag_fft<T, B, XC, NT>, sh);
+    hipLaunchKernelGGL((k_inv_ag_fft<T, B, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X,
+                       (const cT<T>*)G, (T*)f, nxc);
+}
+
 template <class T>
 void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
     const int n2 = 2 * X.B;
     if (n2 == 30) {
-        constexpr int XC = kFftFwdXC, NT = kFftNT;
-        const int nxc = (int)((X.nx + XC - 1) / XC);
-        HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
-        const size_t sh = std::max(sizeof(T) * XC * n2 * (n2 + 1), sizeof(cT<T>) * X.B * fft_sh<T>(XC));
-        allow_lds((const void*)k_fwd_ag_fft30<T, XC, NT>, sh);
-        hipLaunchKernelGGL((k_fwd_ag_fft30<T, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X, (const T*)f,
-                           (const T*)wR, wstride, (cT<T>*)G, nxc);
+        fwd_ag_fft<T, 15, kFftFwdXC, kFftNT>(X, f, wR, wstride, G, st);
+        return;
+    }
+    if (n2 == 60) {  // 2B = 60: two (FP64) or four (FP32) omega points per block
+        if constexpr (sizeof(T) == 8) fwd_ag_fft<T, 30, 2, 128>(X, f, wR, wstride, G, st);
+        else fwd_ag_fft<T, 30, 4, 256>(X, f, wR, wstride, G, st);
         return;
     }
     if (n2 <= kAgMax) {
@@ -736,13 +813,12 @@ template <class T>
 void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
     const int n2 = 2 * X.B, M = 2 * X.B - 1;
     if (n2 == 30) {
-        constexpr int XC = kFftInvXC * (sizeof(T) == 8 ? 1 : 2), NT = kFftNT;
-        const int nxc = (int)((X.nx + XC - 1) / XC);
-        HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
-        const size_t sh = sizeof(cT<T>) * M * fft_sh<T>(XC);
-        allow_lds((const void*)k_inv_ag_fft30<T, XC, NT>, sh);
-        hipLaunchKernelGGL((k_inv_ag_fft30<T, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X,
-                           (const cT<T>*)G, (T*)f, nxc);
+        inv_ag_fft<T, 15, kFftInvXC * (sizeof(T) == 8 ? 1 : 2), kFftNT>(X, G, f, st);
+        return;
+    }
+    if (n2 == 60) {
+        if constexpr (sizeof(T) == 8) inv_ag_fft<T, 30, 2, 128>(X, G, f, st);
+        else inv_ag_fft<T, 30, 4, 256>(X, G, f, st);
         return;
     }
     if (n2 <= kAgMax) {

@@ -84,6 +84,7 @@ EXPORTS = [
     "hpup_plan_adjoint_stage", "hpup_plan_axpy_stage",
     "hpup_plan_set_reset", "hpup_plan_reset_region", "hpup_plan_reset", "hpup_plan_hybrid_step",
     "hpup_plan_reset_adjoint", "hpup_plan_reset_eps_grad", "hpup_plan_adjoint_stage_noalloc",
+    "hpup_plan_step_stages", "hpup_plan_adjoint_step",
 ]
 
 
@@ -117,6 +118,10 @@ def lib():
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                               ctypes.c_void_p]
         L.hpup_plan_adjoint_stage_noalloc.argtypes = L.hpup_plan_adjoint_stage.argtypes
+        L.hpup_plan_step_stages.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.hpup_plan_adjoint_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.hpup_plan_destroy.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_coeffs.argtypes = [ctypes.c_void_p]
         L.hpup_plan_num_grid.argtypes = [ctypes.c_void_p]
@@ -538,6 +543,27 @@ class Plan:
                                              1 if fbar_init else 0, ctypes.c_void_p(grad.data_ptr()),
                                              self._stream()))
         return out
+
+    def step_stages(self, F, ys, F_next=None):
+        """Forward model plans: ys (s - 1 coefficient tensors) receive the Horner
+        stage values y_1..y_{s-1} of one step from F; F_next (optional) y_s."""
+        arr = (ctypes.c_void_p * max(1, len(ys)))(*[self._chk(y, self.ncoef, self.cdtype) for y in ys])
+        _check(lib().hpup_plan_step_stages(self._h, self._chk(F, self.ncoef, self.cdtype), arr,
+                                           self._chk(F_next, self.ncoef, self.cdtype) if F_next is not None else None,
+                                           self._stream()))
+        return ys
+
+    def adjoint_step(self, F, ys, lam, grad, Fbar=None):
+        """Adjoint plans: the reverse of one step from its stage values (F, ys)
+        and the incoming adjoint lam; returns Fbar, accumulates grad (device
+        float64, 5)."""
+        Fbar = self.empty_coeffs() if Fbar is None else Fbar
+        arr = (ctypes.c_void_p * max(1, len(ys)))(*[self._chk(y, self.ncoef, self.cdtype) for y in ys])
+        _check(lib().hpup_plan_adjoint_step(self._h, self._chk(F, self.ncoef, self.cdtype), arr,
+                                            self._chk(lam, self.ncoef, self.cdtype),
+                                            self._chk(Fbar, self.ncoef, self.cdtype),
+                                            ctypes.c_void_p(grad.data_ptr()), self._stream()))
+        return Fbar
 
     @property
     def mass_index(self):

@@ -21,9 +21,10 @@ matrix G: dA/db_p = i K_p T_p (the damping term) and dA/dG_ij is the diagonal
 factor of the diffusion term.  Each reverse stage is one fused kernel
 (Plan.adjoint_stage) that applies A^H and reduces all five parameter terms;
 the forward stage values come from the same fused stage kernel
-(Plan.axpy_stage).  The forward states are kept at checkpoints
-every `checkpoint` steps and each step's stage values are recomputed from
-them on the way back.
+(Plan.step_stages).  The forward states are kept at checkpoints every
+`checkpoint` steps; on the way back each segment is recomputed from its
+checkpoint once, keeping every stage value, and each step's reverse stages
+run in one call (Plan.adjoint_step).
 """
 import numpy as np
 
@@ -47,26 +48,6 @@ class Gradient:
         self.fwd = Plan(B, 2, N, **kw)
         self.adj = Plan(B, 2, N, operator=1, **kw)
 
-    def _stages(self, F):
-        """y_0 .. y_{s-1} of one step from F (the plan's fused stage kernel)."""
-        ys = [F]
-        for c in HORNER[self.method][:-1]:
-            ys.append(self.fwd.axpy_stage(ys[-1], F, c * self.dt))
-        return ys
-
-    def _step_back(self, F, lam, acc):
-        """Adjoint of one step taken from F; returns Fbar.  acc (device,
-        float64, 5) accumulates the b_1, b_2, G_00, G_11, G_01 terms: each
-        stage is one fused kernel (A^H ybar and the five reductions)."""
-        import torch
-        ys = self._stages(F)
-        c = HORNER[self.method]
-        Fbar = torch.empty_like(lam)  # the last stage starts it at lam
-        ybar = lam
-        for j in range(len(c) - 1, -1, -1):
-            ybar = self.adj.adjoint_stage(ybar, ys[j], c[j] * self.dt, acc, fbar=Fbar, fbar_init=j == len(c) - 1)
-        return Fbar
-
     def run(self, F0, nsteps, dJ, checkpoint=None):
         """Propagates F0 by nsteps steps and back-propagates the gradient.
 
@@ -87,17 +68,25 @@ class Gradient:
         FK = F
         lam = dJ(FK).clone()
         acc = torch.zeros(5, dtype=torch.float64, device=FK.device)
+        ns = len(HORNER[self.method])
         for seg in range(len(checkpoints) - 1, -1, -1):
             k0 = seg * cp
             k1 = min(K, k0 + cp)
-            states = [checkpoints[seg]]
-            for _ in range(k0, k1 - 1):
-                nxt = states[-1].clone()
-                self.fwd.step(nxt, 1)
-                states.append(nxt)
+            # the segment's steps from its checkpoint, keeping every stage value
+            # (the last step's result is the next checkpoint, not recomputed)
+            stored = []
+            F = checkpoints[seg]
+            for k in range(k0, k1):
+                ys = [torch.empty_like(F) for _ in range(ns - 1)]
+                nxt = torch.empty_like(F) if k < k1 - 1 else None
+                self.fwd.step_stages(F, ys, nxt)
+                stored.append((F, ys))
+                F = nxt
             for k in range(k1 - 1, k0 - 1, -1):
-                lam = self._step_back(states[k - k0], lam, acc)
-            del states
+                Fk, ys = stored[k - k0]
+                lam = self.adj.adjoint_step(Fk, ys, lam, acc)
+                stored[k - k0] = None
+            del stored
         acc = acc.cpu().numpy()
         G = np.array([[acc[2], acc[4]], [acc[4], acc[3]]])  # G_10 and G_01 enter A as their sum
         return FK, lam, {"b": acc[:2].copy(), "G": G}

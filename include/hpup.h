@@ -292,6 +292,23 @@ int hpup_plan_axpy_stage(hpup_plan* plan, const void* in, const void* F0, double
 int hpup_plan_adjoint_stage(hpup_plan* plan, const void* ybar, const void* y, double scale, void* ybar_out,
                             void* fbar, int fbar_init, double* grad, void* stream);
 
+/* The stage values of one step of a forward model plan (no counterpart in the
+ * reference; callers: gradient sweeps that keep them for the reverse pass).
+ * With y_0 = F and the Horner stages y_j = F + c_j dt A y_{j-1} (c = 1 Euler;
+ * 1/2, 1 midpoint and RK2; 1/4, 1/3, 1/2, 1 RK4), writes y_1..y_{s-1} to
+ * ys[0..s-2] (s - 1 device arrays, none of them F) and, when F_next is not
+ * NULL, the step's result y_s to F_next. */
+int hpup_plan_step_stages(hpup_plan* plan, const void* F, void* const* ys, void* F_next, void* stream);
+/* The reverse of one step on an adjoint plan (kind 1, dim 2; no counterpart
+ * in the reference; callers: gradient sweeps): from the forward stage values
+ * F = y_0 and ys = y_1..y_{s-1} (as hpup_plan_step_stages writes them) and
+ * the incoming adjoint lam, runs the s reverse stages of
+ * hpup_plan_adjoint_stage back to back: Fbar (device, coefficient count)
+ * receives the adjoint of F, grad (5 device doubles) accumulates the
+ * parameter terms.  Fbar must not alias F, ys or lam. */
+int hpup_plan_adjoint_step(hpup_plan* adjoint_plan, const void* F, const void* const* ys, const void* lam,
+                           void* Fbar, double* grad, void* stream);
+
 /* hpup_plan_adjoint_stage without any allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops).  Same arguments, kernels and results

@@ -18,8 +18,16 @@
  *   omega- = omega - ((1+eps)/eps)(omega.t) t,  Omega- = R^T omega-
  *                                                  (getOmega.cu:99-137, the 'old' map g)
  *   vC(Omega) = (R Omega x PC) . e_w              (getLambda.cu:234-266)
- *   df(R, Omega) = J lambda(R) f(R, Omega-) [vC(Omega-) >= 0, Omega- in the box]
- *                 - lambda(R) f(R, Omega) [vC(Omega) >= 0]
+ *             = sigma (omega . t),  sigma = (PC x e_w) . t = (h - r tan theta) |r3 x e_w|
+ *   vC(Omega-) = -vC(Omega)/eps (the law of restitution), so both contact tests
+ *   are decided by the sign of sigma (omega . t): outflow where it is > 0,
+ *   inflow where it is < 0.  On the switching plane omega . t = 0 the map is
+ *   the identity and nothing moves (neither term: with J = 1/eps, counting
+ *   both would create mass (1/eps - 1) lambda f on the lattice points of that
+ *   plane).  Evaluated this way the tests do not depend on eps, even where
+ *   omega . t is at rounding level
+ *   df(R, Omega) = J lambda(R) f(R, Omega-) [vC(Omega-) > 0, Omega- in the box]
+ *                 - lambda(R) f(R, Omega) [vC(Omega) > 0]
  *                                                  (propagate.cu:123-229, collision.tex:160-190)
  * with f(R, Omega-) by trilinear interpolation on the omega grid (the 3-D
  * form of getIndRule.cu:50-90's bilinear rule, same box test) and
@@ -56,10 +64,11 @@ void ref_reset3_tables(int B, const double* R, double dwall, double h, double r,
     }
 }
 
-static double contact_velocity(const double* w, const double* P, const double* ew)
+/* sigma = (PC x e_w) . t: vC(omega) = sigma (omega . t) */
+static double contact_sigma(const double* P, const double* ew, const double* t)
 {
-    double c[3] = {w[1] * P[2] - w[2] * P[1], w[2] * P[0] - w[0] * P[2], w[0] * P[1] - w[1] * P[0]};
-    return c[0] * ew[0] + c[1] * ew[1] + c[2] * ew[2];
+    double c[3] = {P[1] * ew[2] - P[2] * ew[1], P[2] * ew[0] - P[0] * ew[2], P[0] * ew[1] - P[1] * ew[0]};
+    return c[0] * t[0] + c[1] * t[1] + c[2] * t[2];
 }
 
 /* Omega- = g(R, Omega) for one rotation (body frame, 3 components) */
@@ -86,18 +95,20 @@ void ref_reset3_df(int B, int N, double L, const double* R, const double* f, con
         if (lam[iR] == 0.0) continue;
         const double* Rr = R + 9 * iR;
         const double* P = PC + 3 * iR;
+        const double* tt = t + 3 * iR;
+        const double sigma = contact_sigma(P, ew, tt);
         for (long ix = 0; ix < nx; ix++) {
             long i[3] = {ix % N, (ix / N) % N, ix / ((long)N * N)};
-            double Om[3], w[3], Oo[3], wo[3];
+            double Om[3], w[3], Oo[3];
             for (int a = 0; a < 3; a++) Om[a] = x0 + hx * i[a];
             for (int a = 0; a < 3; a++) w[a] = Rr[a] * Om[0] + Rr[a + 3] * Om[1] + Rr[a + 6] * Om[2];
-            double out = contact_velocity(w, P, ew) < 0.0 ? 0.0 : -lam[iR] * f[iR + nR * ix];
-            ref_reset3_pullback(Rr, t + 3 * iR, epsilon, Om, Oo);
-            for (int a = 0; a < 3; a++) wo[a] = Rr[a] * Oo[0] + Rr[a + 3] * Oo[1] + Rr[a + 6] * Oo[2];
+            const double vq = sigma * (w[0] * tt[0] + w[1] * tt[1] + w[2] * tt[2]);
+            double out = vq > 0.0 ? -lam[iR] * f[iR + nR * ix] : 0.0;
+            ref_reset3_pullback(Rr, tt, epsilon, Om, Oo);
             int inside = 1;
             for (int a = 0; a < 3; a++)
                 if (!(Oo[a] >= xlo && Oo[a] <= xhi)) inside = 0;
-            if (inside && !(contact_velocity(wo, P, ew) < 0.0)) {
+            if (inside && vq < 0.0) {
                 long c[3];
                 double s[3];
                 for (int a = 0; a < 3; a++) {

@@ -54,7 +54,8 @@ def test_operator_adjoint_identity_3d(B, N, L, gyro, precision, tol):
     scale = float(Ax.abs().pow(2).sum().sqrt() * y.abs().pow(2).sum().sqrt())
     assert abs(lhs - rhs) <= tol * scale, (lhs, rhs)
     # the operator is not self-adjoint, so the identity is not trivial
-    assert abs(_dot(A.derivative(y), x) - _dot(y, A.derivative(x))) > 100 * tol * scale
+    if precision == "fp64":
+        assert abs(_dot(A.derivative(y), x) - _dot(y, A.derivative(x))) > 1e6 * tol * scale
 
 
 @pytest.mark.parametrize("method", ["euler", "midpoint", "RK2", "RK4"])

@@ -10,9 +10,9 @@ counterpart in the reference, which only propagates forward).  Pinned by:
   CPU composition of the same four functions (the reference's path); the F0
   gradient against the exact change of the linear objective.
 
-The density sits well inside a wide omega box (L = 12, unit widths), so the
-box test of the pull-back, whose edge moves with epsilon, meets densities of
-order e^-18.  Tolerances: 1e-12 on the identity, 2e-6 relative on the
+The density sits well inside a wide omega box (L = 16, unit widths), so the
+box test of the pull-back, whose edge moves with epsilon, meets densities
+below e^-18.  Tolerances: 1e-12 on the identity, 2e-6 relative on the
 differences (h = 1e-5)."""
 import numpy as np
 import pytest
@@ -20,7 +20,7 @@ import pytest
 import hpup
 import oracle as O
 from hpup.adjoint import HybridGradient
-from helpers import density, operators
+from helpers import density, omega_grid, operators
 
 pytestmark = pytest.mark.gpu
 
@@ -58,15 +58,19 @@ def test_reset_adjoint_identity(dim, tilt, jac):
     assert float(Rx.abs().max()) > 0
 
 
-@pytest.mark.parametrize("dim,tilt,jac", [(2, 0.0, False), (3, 0.25, True)])
+@pytest.mark.parametrize("dim,tilt,jac", [(2, 0.0, False), (3, 0.25, True), (3, -0.1, False)])
 def test_restitution_derivative(dim, tilt, jac):
     import torch
     _print("restitution derivative", dim, tilt, jac)
-    B, N, L, eps, h = 3, 12, 12.0, 0.8, 1e-5
+    B, N, L, eps, h = 3, 16, 16.0, 0.8, 1e-5
     f = density(B, dim, N, L, (6, 3, 1), seed=21, Sigma=np.eye(dim))
     p = hpup.Plan(B, dim, N, L, precision="fp64")
     ft = _grid(p, f)
-    g = _rand_grid(p, 3)
+    # random weights under a Gaussian window: the pull-back's box edge, which
+    # moves with eps, then meets weights and densities below 1e-13
+    x = omega_grid(dim, N, L)
+    win = np.exp(-0.125 * (x ** 2).sum(axis=0))
+    g = _rand_grid(p, 3) * _grid(p, np.broadcast_to(win, f.shape))
 
     def val(e):
         p.set_reset(**PAR, epsilon=e, tilt=tilt, jacobian=jac)
@@ -97,7 +101,7 @@ def _oracle_hybrid(B, N, L, F, ops, bt, G, dt, method, eps, tilt, K):
 def test_hybrid_restitution_gradient():
     import torch
     _print("hybrid restitution gradient")
-    B, N, L, dt, K, method = 3, 8, 12.0, 0.01, 3, "RK4"
+    B, N, L, dt, K, method = 3, 10, 16.0, 0.01, 3, "RK4"
     eps, tilt, h = 0.8, 0.25, 1e-5
     bt, G = np.array([0.2, 0.15, 0.25]), np.diag([0.1, 0.2, 0.05])
     f0 = density(B, 3, N, L, (8, 4, 2), seed=31, Sigma=np.eye(3))

@@ -68,8 +68,9 @@ def test_gateway_transforms_on_metric_grids(B, dim, N):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
-@pytest.mark.parametrize("B,dim,N", [(3, 3, 30), (4, 2, 32)])
+@pytest.mark.parametrize("B,dim,N", [(3, 3, 30), (4, 2, 32), (30, 3, 4), (30, 2, 6)])
 def test_plan_transforms_on_metric_grids(B, dim, N, precision, tol):
+    # B = 30 (2B = 60): the prime-factor 4 x 3 x 5 FFT kernels of the angular stages
     import torch
     _print("case plan transforms", B, dim, N, precision)
     L = 10.0
