@@ -504,14 +504,19 @@ template <class T> struct PlanT : PlanBase {
         stage(in, F0, out, StageCoef<T>{(T)scale, 0, 1, 0, 0, 1, 0, 0, 0}, st);
         HPUP_HIP(hipGetLastError());
     }
-    // The Horner coefficients c_j of the plan's integrator: one step is
-    // y_j = F + c_j dt A y_{j-1}, j = 1..s, y_0 = F, F_next = y_s
+    // The Horner stage scales h_j = c_j dt of the plan's integrator: one step
+    // is y_j = F + h_j A y_{j-1}, j = 1..s, y_0 = F, F_next = y_s; rounded
+    // exactly as the step's own stages form them (dt / 4, dt / 3, ... in the
+    // plan precision), so stage values taken here equal the step's bit for bit
     int horner(double* c) const {
+        const T dt = (T)model.dt;
         switch (model.method) {
-        case HPUP_EULER: c[0] = 1.0; return 1;
+        case HPUP_EULER: c[0] = (double)dt; return 1;
         case HPUP_MIDPOINT:
-        case HPUP_RK2: c[0] = 0.5; c[1] = 1.0; return 2;
-        case HPUP_RK4: c[0] = 0.25; c[1] = 1.0 / 3.0; c[2] = 0.5; c[3] = 1.0; return 4;
+        case HPUP_RK2: c[0] = (double)(dt / 2); c[1] = (double)dt; return 2;
+        case HPUP_RK4:
+            c[0] = (double)(dt / 4); c[1] = (double)(dt / 3); c[2] = (double)(dt / 2); c[3] = (double)dt;
+            return 4;
         default: throw Error{HPUP_ERR_METHOD, "unknown integrator"};
         }
     }
@@ -524,12 +529,12 @@ template <class T> struct PlanT : PlanBase {
         const void* prev = F;
         for (int j = 1; j < ns; j++) {
             HPUP_REQUIRE(ys[j - 1] && ys[j - 1] != F, HPUP_ERR_ARG, "stage buffers must be distinct from F");
-            axpy_stage(prev, F, c[j - 1] * model.dt, ys[j - 1], st);
+            axpy_stage(prev, F, c[j - 1], ys[j - 1], st);
             prev = ys[j - 1];
         }
         if (Fnext) {
             HPUP_REQUIRE(Fnext != prev, HPUP_ERR_ARG, "F_next must not alias the last stage value");
-            axpy_stage(prev, F, c[ns - 1] * model.dt, Fnext, st);
+            axpy_stage(prev, F, c[ns - 1], Fnext, st);
         }
     }
     // The whole reverse step on an adjoint plan from the forward stage values
@@ -546,7 +551,7 @@ template <class T> struct PlanT : PlanBase {
         for (int j = ns - 1; j >= 0; j--) {
             const void* y = j == 0 ? F : ys[j - 1];
             void* out = (ybar == Ya.p) ? Yb.p : Ya.p;
-            adjoint_stage(ybar, y, c[j] * model.dt, out, Fbar, j == ns - 1 ? 1 : 0, gout, st, false);
+            adjoint_stage(ybar, y, c[j], out, Fbar, j == ns - 1 ? 1 : 0, gout, st, false);
             ybar = out;
         }
     }

@@ -14,7 +14,7 @@ struct ResetConsts {
     int N;
     double xg[kResetMaxN];  // omega grid values along an axis
     double x0, hx;          // 3-D: grid origin and spacing
-    double xlo, xhi;        // box of valid pull-backs
+    double xlo, xhi;        // 2-D: getOmega's box of valid pull-backs
     double Lg, dx2;         // 2-D: getIndRule's box length and cell area
     double kfac;            // (1 + eps) / eps
     double J;               // jump-term Jacobian factor (1/eps or 1)

@@ -19,8 +19,10 @@
 // with sigma = (PC x e_w) . t, so that vC(omega-) = -vC(omega)/eps and one
 // sign decides both contact tests, independently of eps (outflow where it is
 // positive, inflow where it is negative, neither on the switching plane
-// where the map is the identity); trilinear
-// interpolation, and J = 1/eps (|det g|, which conserves probability) or 1.
+// where the map is the identity); trilinear interpolation of the grid values
+// extended by zero (the inflow fades to zero over the cell beyond the grid's
+// edge instead of stopping at the edge, so it is continuous in eps), and
+// J = 1/eps (|det g|, which conserves probability) or 1.
 //
 // Layouts: the plan's grid (nR, nx) with the rotation index fastest; the
 // region's rows k (rotations with theta >= theta0 - theta_t, ascending) and a
@@ -121,27 +123,27 @@ __global__ __launch_bounds__(256) void k_reset_df(const T* __restrict__ f, const
         double Oo[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) Oo[a] = Rr[3 * a] * w[0] + Rr[3 * a + 1] * w[1] + Rr[3 * a + 2] * w[2];
-        bool inside = true;
+        // trilinear interpolation of the grid values extended by zero: cells
+        // c in [-1, N-1], corners outside the grid weigh nothing
+        int64_t cl[3];
+        double s[3];
+        bool support = true;
 #pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (!(Oo[a] >= c.xlo && Oo[a] <= c.xhi)) inside = false;
-        if (inside && vq < 0.0) {
-            int64_t cl[3];
-            double s[3];
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                int64_t ci = (int64_t)floor((Oo[a] - c.x0) / c.hx);
-                if (ci > N - 2) ci = N - 2;
-                cl[a] = ci;
-                s[a] = (Oo[a] - (c.x0 + c.hx * (double)ci)) / c.hx;
-            }
+        for (int a = 0; a < 3; a++) {
+            const double cf = floor((Oo[a] - c.x0) / c.hx);
+            if (!(cf >= -1.0 && cf <= (double)(N - 1))) support = false;
+            cl[a] = support ? (int64_t)cf : 0;
+            s[a] = (Oo[a] - (c.x0 + c.hx * (double)cl[a])) / c.hx;
+        }
+        if (support && vq < 0.0) {
             double fv[8], wt[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int b0 = q & 1, b1 = (q >> 1) & 1, b2 = (q >> 2) & 1;
+                const int64_t j0 = cl[0] + b0, j1 = cl[1] + b1, j2 = cl[2] + b2;
+                const bool in = j0 >= 0 && j0 < N && j1 >= 0 && j1 < N && j2 >= 0 && j2 < N;
                 wt[q] = (b0 ? s[0] : 1 - s[0]) * (b1 ? s[1] : 1 - s[1]) * (b2 ? s[2] : 1 - s[2]);
-                const int64_t xq = (cl[0] + b0) + N * ((cl[1] + b1) + (int64_t)N * (cl[2] + b2));
-                fv[q] = (double)f[iR + nR * xq];
+                fv[q] = in ? (double)f[iR + nR * (j0 + N * (j1 + (int64_t)N * j2))] : 0.0;
             }
             double v = 0.0;
 #pragma unroll
@@ -231,27 +233,29 @@ __device__ __forceinline__ void jump_at(const double* Rr, const double* t, const
         for (int a = 0; a < 3; a++) w[a] -= fac * t[a];
         double Oo[3];
         for (int a = 0; a < 3; a++) Oo[a] = Rr[3 * a] * w[0] + Rr[3 * a + 1] * w[1] + Rr[3 * a + 2] * w[2];
-        for (int a = 0; a < 3; a++)
-            if (!(Oo[a] >= c.xlo && Oo[a] <= c.xhi)) return;
         if (!(vq < 0.0)) return;
-        J.in_on = true;
         int64_t cl[3];
         double s[3], ds[3];
         for (int a = 0; a < 3; a++) {
-            int64_t ci = (int64_t)floor((Oo[a] - c.x0) / c.hx);
-            if (ci > N - 2) ci = N - 2;
-            cl[a] = ci;
-            s[a] = (Oo[a] - (c.x0 + c.hx * (double)ci)) / c.hx;
+            const double cf = floor((Oo[a] - c.x0) / c.hx);
+            if (!(cf >= -1.0 && cf <= (double)(N - 1))) return;  // beyond the zero-extended support
+            cl[a] = (int64_t)cf;
+            s[a] = (Oo[a] - (c.x0 + c.hx * (double)cl[a])) / c.hx;
             // d Oo_a / d eps = deps_scale (w . t) (R^T t)_a
             ds[a] = deps_scale * dot * (Rr[3 * a] * t[0] + Rr[3 * a + 1] * t[1] + Rr[3 * a + 2] * t[2]) / c.hx;
         }
+        J.in_on = true;
         for (int q = 0; q < 8; q++) {
             const int b0 = q & 1, b1 = (q >> 1) & 1, b2 = (q >> 2) & 1;
+            const int64_t j0 = cl[0] + b0, j1 = cl[1] + b1, j2 = cl[2] + b2;
+            const bool in = j0 >= 0 && j0 < N && j1 >= 0 && j1 < N && j2 >= 0 && j2 < N;
             const double u0 = b0 ? s[0] : 1 - s[0], u1 = b1 ? s[1] : 1 - s[1], u2 = b2 ? s[2] : 1 - s[2];
-            J.w[q] = u0 * u1 * u2;
-            J.dw[q] = (b0 ? ds[0] : -ds[0]) * u1 * u2 + u0 * (b1 ? ds[1] : -ds[1]) * u2 +
-                      u0 * u1 * (b2 ? ds[2] : -ds[2]);
-            J.q[q] = (cl[0] + b0) + N * ((cl[1] + b1) + (int64_t)N * (cl[2] + b2));
+            // corners outside the grid carry the zero extension: no weight, no derivative
+            J.w[q] = in ? u0 * u1 * u2 : 0.0;
+            J.dw[q] = in ? (b0 ? ds[0] : -ds[0]) * u1 * u2 + u0 * (b1 ? ds[1] : -ds[1]) * u2 +
+                               u0 * u1 * (b2 ? ds[2] : -ds[2])
+                         : 0.0;
+            J.q[q] = in ? j0 + N * (j1 + (int64_t)N * j2) : 0;
         }
     }
 }
@@ -383,8 +387,6 @@ void ResetGeom::build(int B, int dim_, int N_, double L, const hpup_reset_params
         c.kfac = (1.0 + p.epsilon) / p.epsilon;
         c.J = 1.0;
     } else {
-        c.xlo = c.x0 + 1e-10;
-        c.xhi = c.x0 + c.hx * (N - 1) - 1e-10;
         c.kfac = (1.0 + p.epsilon) / p.epsilon;
         c.J = p.jacobian ? 1.0 / p.epsilon : 1.0;
     }

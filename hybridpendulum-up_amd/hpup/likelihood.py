@@ -51,7 +51,7 @@ def sample_mf_isotropic(rng, R, kappa):
     of the angle), drawn by inverting its tabulated distribution function."""
     n = len(R)
     t = np.linspace(0.0, math.pi, 200001)
-    logp = 2 * kappa * (np.cos(t) - 1) + np.log1p(-np.cos(t) + 1e-300)
+    logp = 2 * kappa * (np.cos(t) - 1) + np.log(np.maximum(1.0 - np.cos(t), 1e-300))
     p = np.exp(logp - logp.max())
     cdf = np.concatenate([[0.0], np.cumsum(0.5 * (p[1:] + p[:-1]) * np.diff(t))])
     cdf /= cdf[-1]

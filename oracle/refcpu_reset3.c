@@ -29,8 +29,11 @@
  *   df(R, Omega) = J lambda(R) f(R, Omega-) [vC(Omega-) > 0, Omega- in the box]
  *                 - lambda(R) f(R, Omega) [vC(Omega) > 0]
  *                                                  (propagate.cu:123-229, collision.tex:160-190)
- * with f(R, Omega-) by trilinear interpolation on the omega grid (the 3-D
- * form of getIndRule.cu:50-90's bilinear rule, same box test) and
+ * with f(R, Omega-) by trilinear interpolation of the grid values extended by
+ * zero outside the omega grid (the 3-D form of getIndRule.cu:50-90's bilinear
+ * rule; where the reference drops a pull-back outside [x_0, x_{N-1}] whole,
+ * the zero extension lets the inflow fade to zero over the outermost cell,
+ * so it stays continuous in eps) and
  * J = 1/eps (|det| of the map g, which conserves probability) or J = 1 (the
  * reference's reduced law omits it).
  */
@@ -87,7 +90,7 @@ void ref_reset3_df(int B, int N, double L, const double* R, const double* f, con
 {
     long nR = 8L * B * B * B, nx = (long)N * N * N;
     double ew[3] = {cos(tilt), 0.0, sin(tilt)};
-    double hx = L / N, x0 = -L / 2, xlo = x0 + 1e-10, xhi = x0 + hx * (N - 1) - 1e-10;
+    double hx = L / N, x0 = -L / 2;
     double J = jacobian ? 1.0 / epsilon : 1.0;
     memset(df, 0, sizeof(double) * nR * nx);
 #pragma omp parallel for schedule(static)
@@ -105,23 +108,22 @@ void ref_reset3_df(int B, int N, double L, const double* R, const double* f, con
             const double vq = sigma * (w[0] * tt[0] + w[1] * tt[1] + w[2] * tt[2]);
             double out = vq > 0.0 ? -lam[iR] * f[iR + nR * ix] : 0.0;
             ref_reset3_pullback(Rr, tt, epsilon, Om, Oo);
-            int inside = 1;
-            for (int a = 0; a < 3; a++)
-                if (!(Oo[a] >= xlo && Oo[a] <= xhi)) inside = 0;
-            if (inside && vq < 0.0) {
-                long c[3];
-                double s[3];
-                for (int a = 0; a < 3; a++) {
-                    c[a] = (long)floor((Oo[a] - x0) / hx);
-                    if (c[a] > N - 2) c[a] = N - 2;
-                    s[a] = (Oo[a] - (x0 + hx * c[a])) / hx;
-                }
+            long c[3];
+            double s[3];
+            int support = 1;
+            for (int a = 0; a < 3; a++) {
+                c[a] = (long)floor((Oo[a] - x0) / hx);  /* cell [x_c, x_{c+1}], c in [-1, N-1] */
+                if (!(c[a] >= -1 && c[a] <= N - 1)) support = 0;
+                s[a] = (Oo[a] - (x0 + hx * c[a])) / hx;
+            }
+            if (support && vq < 0.0) {
                 double v = 0.0;
                 for (int q = 0; q < 8; q++) {
                     int b0 = q & 1, b1 = (q >> 1) & 1, b2 = (q >> 2) & 1;
+                    long j0 = c[0] + b0, j1 = c[1] + b1, j2 = c[2] + b2;
+                    if (j0 < 0 || j0 >= N || j1 < 0 || j1 >= N || j2 < 0 || j2 >= N) continue;  /* zero outside */
                     double wt = (b0 ? s[0] : 1 - s[0]) * (b1 ? s[1] : 1 - s[1]) * (b2 ? s[2] : 1 - s[2]);
-                    long xq = (c[0] + b0) + N * ((c[1] + b1) + (long)N * (c[2] + b2));
-                    v += wt * f[iR + nR * xq];
+                    v += wt * f[iR + nR * (j0 + N * (j1 + (long)N * j2))];
                 }
                 out += J * lam[iR] * v;
             }

@@ -12,8 +12,10 @@ counterpart in the reference, which only propagates forward).  Pinned by:
 
 The density sits well inside a wide omega box (L = 16, unit widths), so the
 box test of the pull-back, whose edge moves with epsilon, meets densities
-below e^-18.  Tolerances: 1e-12 on the identity, 2e-6 relative on the
-differences (h = 1e-5)."""
+below e^-18, and the differences are taken at a generic eps (0.77), where no
+pulled-back lattice point sits on a cell face (the interpolant's kinks).
+Tolerances: 1e-12 on the identity, 2e-6 relative on the differences
+(h = 1e-6 for one reset, 1e-5 through hybrid steps)."""
 import numpy as np
 import pytest
 
@@ -62,7 +64,9 @@ def test_reset_adjoint_identity(dim, tilt, jac):
 def test_restitution_derivative(dim, tilt, jac):
     import torch
     _print("restitution derivative", dim, tilt, jac)
-    B, N, L, eps, h = 3, 16, 16.0, 0.8, 1e-5
+    # a generic eps and grid spacing: at eps = 0.8 on a unit-spaced grid, pull-backs
+    # of lattice points land exactly on cell faces, where the interpolant has kinks
+    B, N, L, eps, h = 3, 14, 16.0, 0.77, 1e-6
     f = density(B, dim, N, L, (6, 3, 1), seed=21, Sigma=np.eye(dim))
     p = hpup.Plan(B, dim, N, L, precision="fp64")
     ft = _grid(p, f)
@@ -102,7 +106,7 @@ def test_hybrid_restitution_gradient():
     import torch
     _print("hybrid restitution gradient")
     B, N, L, dt, K, method = 3, 10, 16.0, 0.01, 3, "RK4"
-    eps, tilt, h = 0.8, 0.25, 1e-5
+    eps, tilt, h = 0.77, 0.25, 1e-5
     bt, G = np.array([0.2, 0.15, 0.25]), np.diag([0.1, 0.2, 0.05])
     f0 = density(B, 3, N, L, (8, 4, 2), seed=31, Sigma=np.eye(3))
     d = O.wigner_table(B)

@@ -17,7 +17,7 @@ from hpup.likelihood import MeasurementLikelihood, sample_mf_isotropic, sample_s
 pytestmark = pytest.mark.gpu
 
 B, N, L, K, DT, CP = 10, 32, 8.0, 50, 0.01, 10
-S0 = (8.0, 4.0, 2.0)
+S0 = (4.0, 2.0, 1.0)  # resolved at B = 10: the noisy measurements' likelihoods stay positive
 B0, G0 = 0.2, (0.1, 0.1)
 
 

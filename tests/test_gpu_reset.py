@@ -89,8 +89,17 @@ def test_getFcL_and_noisy_propagate(setup):
         a, b = dens(k, dest, int(idx[t, k, d1, d2])), dens(k, dest, int(idx_o[t, k, d1, d2]))
         assert abs(a - b) <= 1e-13 * max(a, b), (t, k, dest, a, b)
     print("  top-nD slots holding a tied source: %d of %d" % (len(diff), idx.size), flush=True)
-    same = idx == idx_o
-    assert np.abs(fcL[same] - fcL_o[same]).max() <= 1e-12 * np.abs(fcL_o).max()
+    # a tied slot moves a density from one source's normalizer to another's,
+    # so every weight of those two sources (in that region point) changes;
+    # all other weights agree to rounding
+    touched = set()
+    for t, k, d1, d2 in diff:
+        touched.add((int(k), int(idx[t, k, d1, d2])))
+        touched.add((int(k), int(idx_o[t, k, d1, d2])))
+    kk = np.broadcast_to(np.arange(numR).reshape(1, numR, 1, 1), idx.shape)
+    clean = (idx == idx_o) & ~np.vectorize(lambda k, s: (int(k), int(s)) in touched)(kk, idx_o)
+    assert clean.sum() > 0.99 * idx.size
+    assert np.abs(fcL[clean] - fcL_o[clean]).max() <= 1e-12 * np.abs(fcL_o).max()
     f = density(B, 2, N, L, (8, 4, 2), seed=4, Sigma=np.eye(2) * 0.5)
     df = hpup.pendulum_reduced_discrete_propagate(True, f, lam, indR, fcL_o, idx_o)
     df_o = O.discrete_noise(B, N, f, lam, indR, fcL_o, idx_o)

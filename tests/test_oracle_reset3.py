@@ -11,8 +11,10 @@ propagate.cu) with a full body angular velocity and a tilted wall.
   changes omega only along the tangent t, and the forward map (numpy) undoes
   it;
 * the jump term is exact on densities trilinear in Omega (trilinear
-  interpolation reproduces them), and with the Jacobian 1/eps it conserves
-  each rotation's probability up to interpolation error.
+  interpolation reproduces them) inside the grid, fades over the outermost
+  cell as the interpolation of the grid values extended by zero, and with the
+  Jacobian 1/eps it conserves each rotation's probability up to
+  interpolation error.
 """
 import math
 
@@ -138,7 +140,20 @@ def test_jump_term_exact_for_trilinear_densities(jacobian):
         vc = lambda q: np.cross(q.T, PC[:, iR]) @ ew
         inside = np.all((Oo >= lo) & (Oo <= hi), axis=0) & (vc(wo) >= 0)
         pred = -lam[iR] * f[iR] * (vc(w) >= 0)
+        # inside the grid the interpolation reproduces the trilinear density
         pred[inside] += J * lam[iR] * val(Oo[:, inside])[iR]
+        # within one cell beyond the grid: the grid values extended by zero
+        c = np.floor((Oo + L / 2) / h).astype(int)
+        edge = ~inside & (vc(wo) >= 0) & np.all((c >= -1) & (c <= N - 1), axis=0)
+        s = (Oo + L / 2) / h - c
+        fl = f[iR]
+        for q in range(8):
+            b = [(q >> a) & 1 for a in range(3)]
+            jj = [c[a] + b[a] for a in range(3)]
+            ok_c = edge & np.all([(jj[a] >= 0) & (jj[a] < N) for a in range(3)], axis=0)
+            wq = np.prod([s[a] if b[a] else 1 - s[a] for a in range(3)], axis=0)
+            idx = np.where(ok_c, jj[0] + N * (jj[1] + N * jj[2]), 0)
+            pred += np.where(ok_c, J * lam[iR] * wq * fl[idx], 0.0)
         inflows += int(inside.sum())
         # points whose contact velocity is zero to rounding (omega on the
         # plane of PC and e_w) sit on the switching surface: not compared
