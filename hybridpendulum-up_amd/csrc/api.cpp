@@ -670,6 +670,17 @@ template <class T> struct PlanT : PlanBase {
         HPUP_REQUIRE(fin && out, HPUP_ERR_ARG, "null argument");
         HPUP_REQUIRE(mode == 0 || mode == 1, HPUP_ERR_ARG, "reset mode must be 0 or 1");
         HPUP_REQUIRE(mode == 0 || fin != out, HPUP_ERR_ARG, "reset mode 1 cannot run in place");
+        if (reset->rows_fit(sizeof(T))) {  // LDS row-group kernel, in place on the region's rows
+            if (mode == 0) {
+                if (out != fin)
+                    HPUP_HIP(hipMemcpyAsync(out, fin, sizeof(T) * (size_t)ngrid, hipMemcpyDeviceToDevice, st));
+                reset->rows_pass<T>(out, out, scale, true, st);
+            } else {
+                reset->zero<T>(out, st);
+                reset->rows_pass<T>(fin, out, scale, false, st);
+            }
+            return;
+        }
         ensure(dfc, sizeof(T) * (size_t)std::max<int64_t>(1, (int64_t)reset->numR * nx));
         reset->df<T>(fin, dfc.p, st);
         if (mode == 0) {
@@ -683,9 +694,13 @@ template <class T> struct PlanT : PlanBase {
     void reset_adjoint(const void* g, void* out, double scale, hipStream_t st) override {
         HPUP_REQUIRE(reset, HPUP_ERR_ARG, "the plan has no reset (hpup_plan_set_reset)");
         HPUP_REQUIRE(g && out, HPUP_ERR_ARG, "null argument");
-        ensure(dfc, sizeof(T) * (size_t)std::max<int64_t>(1, (int64_t)reset->numR * nx));
-        reset->adjoint<T>(g, dfc.p, st);
         if (out != g) HPUP_HIP(hipMemcpyAsync(out, g, sizeof(T) * (size_t)ngrid, hipMemcpyDeviceToDevice, st));
+        if (reset->rows_fit(2 * sizeof(T))) {  // LDS row-group kernel (LDS atomics)
+            reset->adjoint_rows<T>(out, out, scale, st);
+            return;
+        }
+        ensure(dfc, sizeof(T) * (size_t)std::max<int64_t>(1, (int64_t)reset->numR * nx));
+        reset->adjoint<T>(out, dfc.p, st);
         reset->apply<T>(dfc.p, out, out, scale, st);
     }
     // out[0] (device) += scale <g, dR/d eps f>

@@ -34,6 +34,14 @@ struct ResetGeom {
     // out = base + scale dfc on the region's rows (base may be out); base
     // nullptr: out = scale dfc, zero elsewhere
     template <class T> void apply(const void* dfc, const void* base, void* out, double scale, hipStream_t st) const;
+    // whether a row of `per` bytes per omega point fits the row-group kernels
+    bool rows_fit(size_t per) const;
+    template <class T> void zero(void* out, hipStream_t st) const;
+    // Row-group forms (LDS-staged rows; false when one row does not fit):
+    // fout's region rows = (accumulate ? fin : 0) + scale df(fin) (fout may be fin)
+    template <class T> bool rows_pass(const void* fin, void* fout, double scale, bool accumulate, hipStream_t st) const;
+    // out's region rows = g + scale R^H g (out may be g)
+    template <class T> bool adjoint_rows(const void* g, void* out, double scale, hipStream_t st) const;
     // acc (numR, nx) = the transpose of the jump term applied to the grid g
     template <class T> void adjoint(const void* g, void* acc, hipStream_t st) const;
     // out[0] += scale <g, d(jump term)/d eps f> over the region (part: eps_grad_partials() doubles)

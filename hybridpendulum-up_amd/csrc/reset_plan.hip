@@ -29,9 +29,11 @@
 // compact df (numR, nx) with k fastest.  The geometry is FP64 in every
 // precision (IEEE operations in the oracle's order: contraction off), so the
 // masks and interpolation weights do not depend on the plan's precision.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
+#include "drift.h"
 #include "hpup_internal.h"
 #include "reset_plan.h"
 
@@ -349,6 +351,185 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
     if (threadIdx.x == 0) out[0] += scale * red[0];
 }
 
+// ------------------------------------------------ row-group kernels -----
+//
+// One workgroup per group of up to RG consecutive region rows holds the rows'
+// grid values for all nx omega points in LDS, so the interpolation gathers
+// (corners anywhere in the omega grid) read LDS instead of scattered global
+// addresses; the staging loads run with the row index fastest, the grid's
+// own fastest index, so neighbouring lanes read neighbouring rotations.
+// Groups are numbered XCD-major: the groups that run at once on one XCD are
+// neighbours and share the cache lines of their rows in that XCD's L2.
+
+__device__ __forceinline__ int xcd_major_r(int b, int nb) {
+    const int per = nb >> 3;
+    if (b >= per << 3) return b;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <class T>
+__device__ __forceinline__ void stage_rows(const T* __restrict__ src, const int32_t* __restrict__ indR, int k0, int nr,
+                                           int64_t nR, int64_t nx, T* rows) {
+    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
+        const int r = (int)(e % nr);
+        const int64_t x = e / nr;
+        rows[(int64_t)r * nx + x] = src[indR[k0 + r] + nR * x];
+    }
+}
+
+__device__ __forceinline__ void stage_geo(const double* __restrict__ geo, int numR, int k0, int nr, double* sg) {
+    for (int e = threadIdx.x; e < G_N * nr; e += blockDim.x) sg[e] = geo[(int64_t)(e / nr) * numR + k0 + e % nr];
+}
+
+// jump term at (row r of the group, omega point ix) from the staged rows
+template <class T, int DIM>
+__device__ __forceinline__ double jump_lds(const T* rows, const double* sg, int r, int nr, int64_t nx, int64_t ix,
+                                           const ResetConsts& c) {
+#pragma clang fp contract(off)
+    double Rr[9], t[3], P[3];
+#pragma unroll
+    for (int a = 0; a < 9; a++) Rr[a] = sg[(G_R + a) * nr + r];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        t[a] = sg[(G_T + a) * nr + r];
+        P[a] = sg[(G_PC + a) * nr + r];
+    }
+    const double lam = sg[G_LAM * nr + r];
+    const T* row = rows + (int64_t)r * nx;
+    Jump<DIM> J;
+    jump_at<DIM>(Rr, t, P, c, ix, 0.0, J);
+    double out = J.out_on ? -(double)row[ix] * lam : 0.0;
+    if (J.in_on) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < Jump<DIM>::NQ; q++) v += (double)row[J.q[q]] * J.w[q];
+        out = DIM == 2 ? v * lam + out : out + c.J * lam * v;
+    }
+    return out;
+}
+
+// out rows = (accumulate ? staged + scale df : scale df) on the group's rows
+template <class T, int DIM>
+__global__ __launch_bounds__(1024) void k_reset_rows(const T* f_in, T* f_out, const double* __restrict__ geo,
+                                                     const int32_t* __restrict__ indR, int numR, int64_t nR, int64_t nx,
+                                                     int RG, ResetConsts c, double scale, int accumulate) {
+    extern __shared__ double smem_rr[];
+    double* sg = smem_rr;                       // G_N x RG
+    T* rows = (T*)(smem_rr + G_N * RG);          // RG x nx
+    const int k0 = xcd_major_r(blockIdx.x, gridDim.x) * RG;
+    const int nr = min(RG, numR - k0);
+    stage_geo(geo, numR, k0, nr, sg);
+    stage_rows<T>(f_in, indR, k0, nr, nR, nx, rows);
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
+        const int r = (int)(e % nr);
+        const int64_t x = e / nr;
+        const double d = jump_lds<T, DIM>(rows, sg, r, nr, nx, x, c);
+        const double base = accumulate ? (double)rows[(int64_t)r * nx + x] : 0.0;
+        f_out[indR[k0 + r] + nR * x] = (T)(base + scale * d);
+    }
+}
+
+// out rows = g + scale R^H g on the group's rows: the transpose scatters into
+// an LDS accumulator with atomic adds (only this group's rows are targets)
+template <class T, int DIM>
+__global__ __launch_bounds__(1024) void k_reset_adj_rows(const T* g_in, T* g_out, const double* __restrict__ geo,
+                                                         const int32_t* __restrict__ indR, int numR, int64_t nR,
+                                                         int64_t nx, int RG, ResetConsts c, double scale) {
+    extern __shared__ double smem_ra[];
+    double* sg = smem_ra;
+    T* rows = (T*)(smem_ra + G_N * RG);
+    T* acc = rows + (int64_t)RG * nx;
+    const int k0 = xcd_major_r(blockIdx.x, gridDim.x) * RG;
+    const int nr = min(RG, numR - k0);
+    stage_geo(geo, numR, k0, nr, sg);
+    stage_rows<T>(g_in, indR, k0, nr, nR, nx, rows);
+    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) acc[e] = T(0);
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
+        const int r = (int)(e % nr);
+        const int64_t x = e / nr;
+        double Rr[9], t[3], P[3];
+        for (int a = 0; a < 9; a++) Rr[a] = sg[(G_R + a) * nr + r];
+        for (int a = 0; a < 3; a++) {
+            t[a] = sg[(G_T + a) * nr + r];
+            P[a] = sg[(G_PC + a) * nr + r];
+        }
+        const double lam = sg[G_LAM * nr + r];
+        Jump<DIM> J;
+        jump_at<DIM>(Rr, t, P, c, x, 0.0, J);
+        const double gv = (double)rows[(int64_t)r * nx + x];
+        T* ar = acc + (int64_t)r * nx;
+        if (J.out_on) atomicAdd(ar + x, (T)(-lam * gv));
+        if (J.in_on) {
+            const double a = c.J * lam * gv;
+#pragma unroll
+            for (int q = 0; q < Jump<DIM>::NQ; q++)
+                if (J.w[q] != 0.0) atomicAdd(ar + J.q[q], (T)(a * J.w[q]));
+        }
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
+        const int r = (int)(e % nr);
+        const int64_t x = e / nr;
+        g_out[indR[k0 + r] + nR * x] = (T)((double)rows[(int64_t)r * nx + x] + scale * (double)acc[(int64_t)r * nx + x]);
+    }
+}
+
+// <g, (dR/d eps) f> over the group's rows, f staged, g read in place; the
+// block's sum (lanes, then waves, fixed order) goes to part[group]
+template <class T, int DIM>
+__global__ __launch_bounds__(1024) void k_reset_deps_rows(const T* __restrict__ f, const T* __restrict__ g,
+                                                          const double* __restrict__ geo,
+                                                          const int32_t* __restrict__ indR, int numR, int64_t nR,
+                                                          int64_t nx, int RG, ResetConsts c, double dJ, double dk,
+                                                          double* __restrict__ part) {
+    extern __shared__ double smem_rd[];
+    __shared__ double red[16];
+    double* sg = smem_rd;
+    T* rows = (T*)(smem_rd + G_N * RG);
+    const int grp = xcd_major_r(blockIdx.x, gridDim.x);
+    const int k0 = grp * RG;
+    const int nr = min(RG, numR - k0);
+    stage_geo(geo, numR, k0, nr, sg);
+    stage_rows<T>(f, indR, k0, nr, nR, nx, rows);
+    __syncthreads();
+    double v = 0.0;
+    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
+        const int r = (int)(e % nr);
+        const int64_t x = e / nr;
+        double Rr[9], t[3], P[3];
+        for (int a = 0; a < 9; a++) Rr[a] = sg[(G_R + a) * nr + r];
+        for (int a = 0; a < 3; a++) {
+            t[a] = sg[(G_T + a) * nr + r];
+            P[a] = sg[(G_PC + a) * nr + r];
+        }
+        const double lam = sg[G_LAM * nr + r];
+        Jump<DIM> J;
+        jump_at<DIM>(Rr, t, P, c, x, -dk, J);
+        if (J.in_on) {
+            const T* row = rows + (int64_t)r * nx;
+            double I = 0.0, dI = 0.0;
+#pragma unroll
+            for (int q = 0; q < Jump<DIM>::NQ; q++) {
+                const double fq = (double)row[J.q[q]];
+                I += J.w[q] * fq;
+                dI += J.dw[q] * fq;
+            }
+            v += (double)g[indR[k0 + r] + nR * x] * lam * (dJ * I + c.J * dI);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
+        part[grp] = s;
+    }
+}
+
 template <class T> __global__ void k_fill_zero(T* p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = T(0);
@@ -443,6 +624,64 @@ void ResetGeom::build(int B, int dim_, int N_, double L, const hpup_reset_params
     region = ind;
 }
 
+// rows per group of the row-group kernels: as many as fit `per` bytes of LDS
+// per omega point into 128 KiB, at most 8; 0 if not even one row fits
+static int rows_per_group(int64_t nx, size_t per) {
+    const int64_t budget = 128 * 1024 - G_N * 8 * 8;
+    const int64_t rg = budget / (nx * (int64_t)per);
+    return (int)std::min<int64_t>(rg, 8);
+}
+
+bool ResetGeom::rows_fit(size_t per) const { return rows_per_group(nx, per) >= 1; }
+
+template <class T> void ResetGeom::zero(void* out, hipStream_t st) const {
+    const int64_t n = nR * nx;
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)out, n);
+    HPUP_HIP(hipGetLastError());
+}
+template void ResetGeom::zero<double>(void*, hipStream_t) const;
+template void ResetGeom::zero<float>(void*, hipStream_t) const;
+
+template <class T>
+bool ResetGeom::rows_pass(const void* fin, void* fout, double scale, bool accumulate, hipStream_t st) const {
+    const int RG = rows_per_group(nx, sizeof(T));
+    if (RG < 1) return false;
+    if (numR == 0) return true;
+    const int ng = (numR + RG - 1) / RG;
+    const size_t sh = sizeof(double) * G_N * RG + sizeof(T) * (size_t)RG * nx;
+    if (dim == 2) {
+        allow_lds((const void*)k_reset_rows<T, 2>, sh);
+        hipLaunchKernelGGL((k_reset_rows<T, 2>), dim3(ng), dim3(1024), sh, st, (const T*)fin, (T*)fout,
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale, accumulate ? 1 : 0);
+    } else {
+        allow_lds((const void*)k_reset_rows<T, 3>, sh);
+        hipLaunchKernelGGL((k_reset_rows<T, 3>), dim3(ng), dim3(1024), sh, st, (const T*)fin, (T*)fout,
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale, accumulate ? 1 : 0);
+    }
+    HPUP_HIP(hipGetLastError());
+    return true;
+}
+
+template <class T>
+bool ResetGeom::adjoint_rows(const void* g, void* out, double scale, hipStream_t st) const {
+    const int RG = rows_per_group(nx, 2 * sizeof(T));
+    if (RG < 1) return false;
+    if (numR == 0) return true;
+    const int ng = (numR + RG - 1) / RG;
+    const size_t sh = sizeof(double) * G_N * RG + 2 * sizeof(T) * (size_t)RG * nx;
+    if (dim == 2) {
+        allow_lds((const void*)k_reset_adj_rows<T, 2>, sh);
+        hipLaunchKernelGGL((k_reset_adj_rows<T, 2>), dim3(ng), dim3(1024), sh, st, (const T*)g, (T*)out,
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale);
+    } else {
+        allow_lds((const void*)k_reset_adj_rows<T, 3>, sh);
+        hipLaunchKernelGGL((k_reset_adj_rows<T, 3>), dim3(ng), dim3(1024), sh, st, (const T*)g, (T*)out,
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale);
+    }
+    HPUP_HIP(hipGetLastError());
+    return true;
+}
+
 template <class T> void ResetGeom::df(const void* f, void* dfc, hipStream_t st) const {
     if (numR == 0) return;
     if (dim == 2)
@@ -485,6 +724,23 @@ void ResetGeom::eps_grad(const void* f, const void* g, double* part, double scal
     const double eps = params.epsilon;
     const double dk = -1.0 / (eps * eps);                                        // d((1 + eps)/eps)/d eps
     const double dJ = (dim == 3 && params.jacobian) ? -1.0 / (eps * eps) : 0.0;  // d(1/eps)/d eps
+    const int RG = rows_per_group(nx, sizeof(T));
+    if (RG >= 1) {  // LDS row groups
+        const int ng = (numR + RG - 1) / RG;
+        const size_t sh = sizeof(double) * G_N * RG + sizeof(T) * (size_t)RG * nx;
+        if (dim == 2) {
+            allow_lds((const void*)k_reset_deps_rows<T, 2>, sh);
+            hipLaunchKernelGGL((k_reset_deps_rows<T, 2>), dim3(ng), dim3(1024), sh, st, (const T*)f, (const T*)g,
+                               geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, dJ, dk, part);
+        } else {
+            allow_lds((const void*)k_reset_deps_rows<T, 3>, sh);
+            hipLaunchKernelGGL((k_reset_deps_rows<T, 3>), dim3(ng), dim3(1024), sh, st, (const T*)f, (const T*)g,
+                               geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, dJ, dk, part);
+        }
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)part, (int64_t)ng, scale, out);
+        HPUP_HIP(hipGetLastError());
+        return;
+    }
     const dim3 grid = grid_k(numR, nx);
     if (dim == 2)
         hipLaunchKernelGGL((k_reset_deps<T, 2>), grid, dim3(256), 0, st, (const T*)f, (const T*)g, geo.as<double>(),
@@ -499,9 +755,13 @@ void ResetGeom::eps_grad(const void* f, const void* g, double* part, double scal
 
 int64_t ResetGeom::eps_grad_partials() const {
     const dim3 grid = grid_k(std::max(numR, 1), nx);
-    return (int64_t)grid.x * grid.y;
+    return std::max<int64_t>((int64_t)grid.x * grid.y, std::max(numR, 1));  // per-point or row-group partials
 }
 
+template bool ResetGeom::rows_pass<double>(const void*, void*, double, bool, hipStream_t) const;
+template bool ResetGeom::rows_pass<float>(const void*, void*, double, bool, hipStream_t) const;
+template bool ResetGeom::adjoint_rows<double>(const void*, void*, double, hipStream_t) const;
+template bool ResetGeom::adjoint_rows<float>(const void*, void*, double, hipStream_t) const;
 template void ResetGeom::adjoint<double>(const void*, void*, hipStream_t) const;
 template void ResetGeom::adjoint<float>(const void*, void*, hipStream_t) const;
 template void ResetGeom::eps_grad<double>(const void*, const void*, double*, double, double*, hipStream_t) const;

@@ -13,6 +13,7 @@
 
 #include "drift.h"
 #include "kernels_common.h"
+#include "mfma.h"
 
 namespace hpup {
 
@@ -120,6 +121,110 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
             F[d_ridx(B, l0 + t, m, n) * X.nx + x] = acc[t];
         }
     }
+}
+
+// The beta stages as MFMA GEMMs per (m, n) block over the omega points.
+// Forward: F^l_mn(x) = sum_j Wf[mn][l][j] G[mn][j][x] -- A = Wf (rows l, RT
+// tiles of 16; K = j in KS steps of 4), B = G (K = j, columns x), one real
+// GEMM on the real and one on the imaginary part of G.  A workgroup of four
+// waves takes one (m, n) and one kGX-block of omega points (each wave a
+// 16-point column tile), whose G chunk [j][x % kGX] is contiguous; rows l <
+// max(|m|, |n|) and the padding carry zero weights and are not stored.
+template <class T, int RT, int KS>
+__global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T>* __restrict__ G,
+                                                       cT<T>* __restrict__ F) {
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc;
+    const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
+    const int mn = blockIdx.y;
+    const int m = mn % M - lmax, n = mn / M - lmax;
+    const int lmin = max(abs(m), abs(n));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    const int64_t nx = X.nx;
+    const int64_t x = (int64_t)blockIdx.x * kGX + 16 * w + lr;  // this lane's column
+    const bool xok = x < nx;
+    const cT<T>* Gp = G + g_off(B, mn, 0, xok ? x : nx - 1);
+    cT<T> g[KS];
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        const int j = 4 * s + lq;
+        const cT<T> v = Gp[(int64_t)(j < n2 ? j : 0) * kGX];
+        g[s] = (j < n2 && xok) ? v : czero<T>();
+    }
+    const T* W = X.Wf + (size_t)mn * B * n2;
+    acc_t re[RT], im[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+        re[rt] = MF::zero();
+        im[rt] = MF::zero();
+        const int l = 16 * rt + lr;
+        const bool lok = l >= lmin && l < B;
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int j = 4 * s + lq;
+            const T a = (lok && j < n2) ? W[(size_t)l * n2 + j] : T(0);
+            re[rt] = MF::fma(a, g[s].x, re[rt]);
+            im[rt] = MF::fma(a, g[s].y, im[rt]);
+        }
+    }
+    if (!xok) return;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int l = 16 * rt + MF::orow(lq, v);
+            if (l >= lmin && l < B) F[d_ridx(B, l, m, n) * nx + x] = cmk<T>(re[rt][v], im[rt][v]);
+        }
+}
+
+// Inverse: G[mn][j][x] = sum_l Wi[mn][j][l] F^l_mn(x) -- A = Wi (rows j, RT
+// tiles; K = l in KS steps), B = the coefficient rows of (m, n).
+template <class T, int RT, int KS>
+__global__ __launch_bounds__(256) void k_beta_inv_mfma(XformDev<T> X, const cT<T>* __restrict__ F,
+                                                       cT<T>* __restrict__ G) {
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc;
+    const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
+    const int mn = blockIdx.y;
+    const int m = mn % M - lmax, n = mn / M - lmax;
+    const int lmin = max(abs(m), abs(n));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    const int64_t nx = X.nx;
+    const int64_t x = (int64_t)blockIdx.x * kGX + 16 * w + lr;
+    const bool xok = x < nx;
+    const int64_t xc = xok ? x : nx - 1;
+    cT<T> f[KS];
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        const int l = 4 * s + lq;
+        const bool lok = l >= lmin && l < B;
+        const cT<T> v = F[d_ridx(B, lok ? l : lmax, m, n) * nx + xc];
+        f[s] = (lok && xok) ? v : czero<T>();
+    }
+    const T* W = X.Wi + (size_t)mn * n2 * B;
+    acc_t re[RT], im[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+        re[rt] = MF::zero();
+        im[rt] = MF::zero();
+        const int j = 16 * rt + lr;
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int l = 4 * s + lq;
+            const T a = (j < n2 && l >= lmin && l < B) ? W[(size_t)j * B + l] : T(0);
+            re[rt] = MF::fma(a, f[s].x, re[rt]);
+            im[rt] = MF::fma(a, f[s].y, im[rt]);
+        }
+    }
+    if (!xok) return;
+    cT<T>* Gp = G + g_off(B, mn, 0, x);
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int j = 16 * rt + MF::orow(lq, v);
+            if (j < n2) Gp[(int64_t)j * kGX] = cmk<T>(re[rt][v], im[rt][v]);
+        }
 }
 
 // ---------------------------------------------------------- inverse -------
@@ -793,8 +898,27 @@ void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstr
     hipLaunchKernelGGL(k_fwd_ag<T>, grid, dim3(256), sh, st, X, (const T*)f, (const T*)wR, wstride, (cT<T>*)G, XC);
 }
 
+// MFMA beta stages for B <= 32 (row tiles and K steps fixed per bandlimit
+// class); larger bandlimits keep the FMA kernels
+template <class T, int RT, int KS>
+static void beta_fwd_mfma(const XformDev<T>& X, const void* G, void* F, hipStream_t st) {
+    const int M = 2 * X.B - 1;
+    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)(M * M));
+    hipLaunchKernelGGL((k_beta_fwd_mfma<T, RT, KS>), grid, dim3(256), 0, st, X, (const cT<T>*)G, (cT<T>*)F);
+}
+template <class T, int RT, int KS>
+static void beta_inv_mfma(const XformDev<T>& X, const void* F, void* G, hipStream_t st) {
+    const int M = 2 * X.B - 1;
+    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)(M * M));
+    hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS>), grid, dim3(256), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
+}
+
 template <class T>
 void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t st) {
+    static_assert(kGX == 64, "one workgroup of four 16-point tiles per G block");
+    if (X.B <= 8) return beta_fwd_mfma<T, 1, 4>(X, G, F, st);     // l < 16, j < 16
+    if (X.B <= 16) return beta_fwd_mfma<T, 1, 8>(X, G, F, st);    // l < 16, j < 32
+    if (X.B <= 32) return beta_fwd_mfma<T, 2, 16>(X, G, F, st);   // l < 32, j < 64
     const int M = 2 * X.B - 1;
     dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)(M * M));
     hipLaunchKernelGGL(k_beta_fwd<T>, grid, dim3(256), sizeof(T) * X.B * 2 * X.B, st, X, (const cT<T>*)G,
@@ -803,6 +927,9 @@ void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t s
 
 template <class T>
 void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t st) {
+    if (X.B <= 8) return beta_inv_mfma<T, 1, 2>(X, F, G, st);     // j < 16, l < 8
+    if (X.B <= 16) return beta_inv_mfma<T, 2, 4>(X, F, G, st);    // j < 32, l < 16
+    if (X.B <= 32) return beta_inv_mfma<T, 4, 8>(X, F, G, st);    // j < 64, l < 32
     const int M = 2 * X.B - 1;
     dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)(M * M));
     hipLaunchKernelGGL(k_beta_inv<T>, grid, dim3(256), sizeof(T) * X.B * 2 * X.B, st, X, (const cT<T>*)F,
