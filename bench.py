@@ -200,10 +200,9 @@ def reset_row(B=15, N=32, L=3.4, nD=10, reps=3):
 
 def hybrid_size_row(B=30, N=60, L=3.4, steps=3, precision="fp64"):
     """The reduced hybrid driver's own grid (bandlimit 30, 60^2 omega grid over
-    [-1.7, 1.7]^2): one RK4 step of the continuous part (N > 32 runs the
-    LDS-tiled FMA drift kernels) and the inverse and forward transforms
-    (2B = 60 > 32: the LDS transform kernels), device-resident, HIP events
-    on the plan's stream."""
+    [-1.7, 1.7]^2): one RK4 step of the continuous part (the 64-point MFMA
+    drift kernel) and the inverse and forward transforms, device-resident,
+    HIP events on the plan's stream."""
     import numpy as np
     import torch
     import hpup
@@ -227,9 +226,15 @@ def hybrid_size_row(B=30, N=60, L=3.4, steps=3, precision="fp64"):
     p.forward(g, F)
     ev[5].record()
     torch.cuda.synchronize()
-    return {"config": "reduced hybrid driver grid, B=%d, %d^2 omega grid, RK4 dt=0.0025, %s" % (B, N, precision),
-            "coefficients": p.ncoef, "rk4_step_ms": ev[0].elapsed_time(ev[1]) / steps,
-            "transform_backward_ms": ev[2].elapsed_time(ev[3]), "transform_forward_ms": ev[4].elapsed_time(ev[5])}
+    row = {"config": "reduced hybrid driver grid, B=%d, %d^2 omega grid, RK4 dt=0.0025, %s" % (B, N, precision),
+           "coefficients": p.ncoef, "rk4_step_ms": ev[0].elapsed_time(ev[1]) / steps,
+           "transform_backward_ms": ev[2].elapsed_time(ev[3]), "transform_forward_ms": ev[4].elapsed_time(ev[5])}
+    for name, ms, nbytes in p.profile(F, 2):
+        if ms > 0 and name.startswith("k_drift"):
+            row["drift_kernel"] = name
+            row["drift_kernel_ms"] = ms
+            row["drift_kernel_tb_s"] = nbytes / ms / 1e9
+    return row
 
 
 # the large hybrid workload: B = 30 (60^3 rotation grid), 20^3 omega grid
@@ -518,6 +523,28 @@ def main():
         except Exception as e:  # an optional report must not hide the headline result
             log("output errors failed:", repr(e))
 
+    # the other mixed-precision variant: 3xBF16 split-product circulants
+    # (three bf16 pieces per operand, six products, FP32 accumulation), its
+    # step time and its error after the workload's 200 steps
+    split_row = None
+    if args.precision == "fp32" and not args.no_rows:
+        try:
+            pb = make_plan(cfg, "bf16x3", args.method)
+            Fb = F0.to(torch.complex64)
+            msb = timed_steps(pb, Fb, K, W)
+            Fbe = Fb if K == ERR_STEPS else F0.to(torch.complex64)
+            if K != ERR_STEPS:
+                pb.step(Fbe, ERR_STEPS)
+            torch.cuda.synchronize()
+            eb = float(torch.linalg.vector_norm(Fbe.to(torch.complex128) - F64e) / torch.linalg.vector_norm(F64e))
+            split_row = {"ms_per_step": msb, "steps_per_s": 1000.0 / msb, "speedup_vs_fp64": ms64 / msb,
+                         "rel_l2_vs_fp64": eb, "rel_l2_vs_fp64_after_steps": ERR_STEPS}
+            log("bf16x3 %.3f ms/step, rel L2 vs fp64 after %d steps %.2e" % (msb, ERR_STEPS, eb))
+            del pb, Fb, Fbe
+            torch.cuda.empty_cache()
+        except Exception as e:
+            log("bf16x3 row failed:", repr(e))
+
     # per-kernel timing with HIP events on the stream the kernels run on
     Fp = Fm.clone()
     prof = pm.profile(Fp, args.profile_reps)
@@ -571,9 +598,11 @@ def main():
             log("gradient row failed:", repr(e))
         try:
             rows["hybrid_size"] = hybrid_size_row()
-            log("rows hybrid_size: RK4 step %.2f ms, backward %.2f ms, forward %.2f ms" % (
-                rows["hybrid_size"]["rk4_step_ms"], rows["hybrid_size"]["transform_backward_ms"],
-                rows["hybrid_size"]["transform_forward_ms"]))
+            hs = rows["hybrid_size"]
+            log("rows hybrid_size: RK4 step %.2f ms, backward %.2f ms, forward %.2f ms, %s %.3f ms per launch "
+                "(%.2f TB/s)" % (hs["rk4_step_ms"], hs["transform_backward_ms"], hs["transform_forward_ms"],
+                                 hs.get("drift_kernel", "?"), hs.get("drift_kernel_ms", 0.0),
+                                 hs.get("drift_kernel_tb_s", 0.0)))
         except Exception as e:
             log("hybrid-size row failed:", repr(e))
         try:
@@ -624,6 +653,7 @@ def main():
         "rel_l2_vs_fp64_after_steps": ERR_STEPS,
         "output_errors_vs_fp64": out_err,
         "fp64_mass_drift": mass_drift,
+        "bf16x3": split_row,
         "roofline": roof,
         "kernels_ms": {n: t for n, t, _ in prof},
         "rows": rows,

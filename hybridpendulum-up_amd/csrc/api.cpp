@@ -325,9 +325,10 @@ template <class T> struct PlanT : PlanBase {
         launch_inv_ag<T>(xf.view, G.p, f, st);
         HPUP_HIP(hipGetLastError());
     }
-    // MFMA kernels cover omega grids up to 32 points per axis; larger grids use
-    // the LDS-tiled FMA kernels (three passes).
-    bool fused() const { return model.N <= 32 || model.dim == 0; }
+    // MFMA kernels cover omega grids up to 32 points per axis, and up to 64 for
+    // the reduced model (dim 2); larger grids use the LDS-tiled FMA kernels
+    // (three passes).
+    bool fused() const { return model.N <= 32 || model.dim == 0 || (model.dim == 2 && model.N <= 64); }
 
     // Density-form gravity (pendulum_reduced_den.m:453-516): the term is
     // -sum_p c(k_p) fftSO3R(f mR_p) on the grid density f, with f the caller's
@@ -482,7 +483,7 @@ template <class T> struct PlanT : PlanBase {
     }
     void ensure_drift_buffers(bool rk) {
         HPUP_REQUIRE(has_drift, HPUP_ERR_ARG, "plan has no drift operator (dim must be 2 or 3)");
-        HPUP_REQUIRE(fused() || !bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 drift needs an omega grid of at most 32");
+        HPUP_REQUIRE(model.N <= 32 || !bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 drift needs an omega grid of at most 32");
         if (!fused()) { ensure(Wb, cbytes()); ensure(V1, cbytes()); ensure(V2, cbytes()); }
         if (model.dim == 3) ensure(Z3, cbytes());
         if (rk) ensure(Fs, cbytes());
@@ -624,8 +625,9 @@ template <class T> struct PlanT : PlanBase {
     }
     const char* profile_name(int i) const override {
         static const char* fz[2] = {"k_axis3_mfma", "k_drift_fused"};
+        static const char* fz64[2] = {"k_axis3_mfma", "k_drift_fused64"};
         static const char* names[3] = {"k_rotmix", "k_axis_outer", "k_drift_slab"};
-        if (fused()) return (i >= 0 && i < 2) ? fz[i] : "";
+        if (fused()) return (i >= 0 && i < 2) ? (model.N > 32 ? fz64 : fz)[i] : "";
         return (i >= 0 && i < 3) ? names[i] : "";
     }
     // Algorithmic HBM bytes per launch of kernel i, averaged over the stages

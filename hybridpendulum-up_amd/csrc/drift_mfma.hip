@@ -15,12 +15,13 @@
 // accumulator v hold element (k2, k1) = (16 k2t + orow(l >> 4, v), 16 k1t + (l & 15)).
 //
 // The GEMMs are a policy: GemmF (FP32 or FP64 MFMA 16x16x4, K = 4 per
-// instruction) or GemmBf16x3 (v_mfma_f32_16x16x32_bf16 on hi/lo bf16 splits of
-// both operands, hi*hi + hi*lo + lo*hi accumulated in FP32).
+// instruction) or GemmBf16x3 (v_mfma_f32_16x16x32_bf16 on three-piece bf16
+// splits of both operands, the six products of order <= 2 accumulated in FP32).
 //
 // k_axis3 computes Z3 = T3 F (circulant along the outermost omega axis) as the
 // GEMM C3 [F_re; F_im] over 64-column chunks of the k1,k2 plane.
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <type_traits>
 
@@ -62,17 +63,18 @@ constexpr int kPad = 34;
 // ------------------------------------------------------------ policies ---
 
 // Circulant fragments from LDS.  Entry (o, a) of a circulant with kernel
-// kappa is kappa[(o - a) mod N]; the table holds, for d = o - a + 32 in
-// [0, 64), the pair (re, im) of kappa[(o - a) mod N] and the pair (-im, re)
+// kappa is kappa[(o - a) mod N]; the table holds, for d = o - a + TILE in
+// [0, 2 TILE), the pair (re, im) of kappa[(o - a) mod N] and the pair (-im, re)
 // used for the imaginary input half.  Rows o >= N or columns a >= N read
 // finite values that meet zero-padded data or are discarded, so the table
-// needs no padding logic.  CircTab<T>::fill builds it for one axis.
-template <class T> struct CircTab {
-    cT<T> re_in[64], im_in[64];
-    cT<T> rim[64];  // (-Im kappa, Im kappa): the real circulant of the ramp path
+// needs no padding logic.  CircTab<T, TILE>::fill builds it for one axis
+// (threads 0 .. 2 TILE - 1 of the workgroup).
+template <class T, int TILE = 32> struct CircTab {
+    cT<T> re_in[2 * TILE], im_in[2 * TILE];
+    cT<T> rim[2 * TILE];  // (-Im kappa, Im kappa): the real circulant of the ramp path
     __device__ __forceinline__ void fill(const cT<T>* kap, int N, int tid) {
-        if (tid < 64) {
-            const int d = tid - 32;
+        if (tid < 2 * TILE) {
+            const int d = tid - TILE;
             const cT<T> c = kap[((d % N) + N) % N];
             re_in[tid] = c;
             im_in[tid] = cmk<T>(-c.y, c.x);
@@ -88,36 +90,42 @@ template <class T> __device__ __forceinline__ T sum_lq(T x) {
     return x;
 }
 
-// FP32 / FP64 MFMA; fragments read from LDS.  General circulants: 16
-// reduction steps of K = 4 over the (re | im) stacked slab.  RAMP (kappa =
-// c0 + i (real circulant)): 8 steps of two MFMAs (-Im C . X_im, Im C . X_re)
-// plus c0 times the axis sums of X, which the operand loads accumulate on the
-// way (the four lq lanes of a row or column hold a quarter each).
-template <class T, bool RAMP = false> struct GemmF {
+// FP32 / FP64 MFMA; fragments read from LDS, planes of TILE rows at stride
+// TP = TILE + 2.  General circulants: TILE / 2 reduction steps of K = 4 over
+// the (re | im) stacked slab.  RAMP (kappa = c0 + i (real circulant)): TILE / 4
+// steps of two MFMAs (-Im C . X_im, Im C . X_re) plus c0 times the axis sums
+// of X, which the operand loads accumulate on the way (the four lq lanes of a
+// row or column hold a quarter each).
+template <class T, bool RAMP = false, int TILE = 32> struct GemmF {
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
-    const CircTab<T>* c1;
-    const CircTab<T>* c2;
+    static constexpr int TP = TILE + 2;  // plane row stride
+    static constexpr int KS = TILE / 4;  // K = 4 steps per re / im half
+    const CircTab<T, TILE>* c1;
+    const CircTab<T, TILE>* c2;
     int k2t, k1t, lr, lq, b1, b2;
     static __device__ __forceinline__ int orow(int q, int v) { return MF::orow(q, v); }
-    __device__ __forceinline__ void init(const CircTab<T>* tab1, const CircTab<T>* tab2, int k2t_, int k1t_,
-                                         int lr_, int lq_) {
+    // T2's reduction index of step u (< KS) in each re/im half
+    static __device__ __forceinline__ int k2of(int u) { return (u & 7) + 32 * (u >> 3); }
+    __device__ __forceinline__ void init(const CircTab<T, TILE>* tab1, const CircTab<T, TILE>* tab2, int k2t_,
+                                         int k1t_, int lr_, int lq_) {
         c1 = tab1; c2 = tab2;
         k2t = k2t_; k1t = k1t_; lr = lr_; lq = lq_;
-        // T1 reduces over k = 4 (s & 7) + lq, T2 over k = 8 lq + (s & 7) (in each
+        // T1 reduces over k = 4 u + lq, T2 over k = 8 lq + k2of(u) (in each
         // re/im half): the order of a GEMM's reduction is free, and these
-        // orders make the data reads conflict-free.  d = o - k + 32.
-        b1 = 16 * k1t + lr + 32 - lq;
-        b2 = 16 * k2t + lr + 32 - 8 * lq;
+        // orders make the data reads conflict-free (TP = 2 mod 32).
+        // d = o - k + TILE.
+        b1 = 16 * k1t + lr + TILE - lq;
+        b2 = 16 * k2t + lr + TILE - 8 * lq;
     }
     // Y = X C1^T along k1 (slab rows as the A operand); c0: Re kappa_1 (RAMP)
     __device__ __forceinline__ void t1(const T* pr, const T* pi, cT<T>* out, T c0 = T(0)) const {
         acc_t are = MF::zero(), aim = MF::zero();
-        const int row = (16 * k2t + lr) * kPad;
+        const int row = (16 * k2t + lr) * TP;
         if constexpr (RAMP) {
             T sr = T(0), si = T(0);
 #pragma unroll 1
-            for (int s = 0; s < 8; s++) {
+            for (int s = 0; s < KS; s++) {
                 const int col = 4 * s + lq;
                 const T xr = pr[row + col], xi = pi[row + col];
                 const cT<T> f = c1->rim[b1 - 4 * s];
@@ -126,8 +134,8 @@ template <class T, bool RAMP = false> struct GemmF {
                 sr += xr;
                 si += xi;
             }
-            // row 16 k2t + lr summed over its 32 columns; output row orow(lq, v)
-            // is held by the lanes with lr = orow(lq, v)
+            // row 16 k2t + lr summed over its TILE columns; output row
+            // orow(lq, v) is held by the lanes with lr = orow(lq, v)
             sr = sum_lq<T>(sr);
             si = sum_lq<T>(si);
 #pragma unroll
@@ -138,9 +146,10 @@ template <class T, bool RAMP = false> struct GemmF {
             return;
         }
 #pragma unroll 4
-        for (int s = 0; s < 16; s++) {
-            const T av = (s < 8 ? pr : pi)[row + ((4 * s + lq) & 31)];
-            const cT<T> f = (s < 8 ? c1->re_in : c1->im_in)[b1 - 4 * (s & 7)];
+        for (int s = 0; s < 2 * KS; s++) {
+            const int u = s & (KS - 1);
+            const T av = (s < KS ? pr : pi)[row + 4 * u + lq];
+            const cT<T> f = (s < KS ? c1->re_in : c1->im_in)[b1 - 4 * u];
             are = MF::fma(av, f.x, are);
             aim = MF::fma(av, f.y, aim);
         }
@@ -154,16 +163,16 @@ template <class T, bool RAMP = false> struct GemmF {
         if constexpr (RAMP) {
             T sr = T(0), si = T(0);
 #pragma unroll 1
-            for (int s = 0; s < 8; s++) {
-                const int rw = (8 * lq + s) * kPad + col;
+            for (int s = 0; s < KS; s++) {
+                const int rw = (8 * lq + k2of(s)) * TP + col;
                 const T xr = pr[rw], xi = pi[rw];
-                const cT<T> f = c2->rim[b2 - s];
+                const cT<T> f = c2->rim[b2 - k2of(s)];
                 are = MF::fma(f.x, xi, are);
                 aim = MF::fma(f.y, xr, aim);
                 sr += xr;
                 si += xi;
             }
-            // column 16 k1t + lr summed over its 32 rows: this lane's own column
+            // column 16 k1t + lr summed over its TILE rows: this lane's own column
             sr = sum_lq<T>(sr);
             si = sum_lq<T>(si);
 #pragma unroll
@@ -171,9 +180,10 @@ template <class T, bool RAMP = false> struct GemmF {
             return;
         }
 #pragma unroll 4
-        for (int s = 0; s < 16; s++) {
-            const T bv = (s < 8 ? pr : pi)[(8 * lq + (s & 7)) * kPad + col];
-            const cT<T> f = (s < 8 ? c2->re_in : c2->im_in)[b2 - (s & 7)];
+        for (int s = 0; s < 2 * KS; s++) {
+            const int u = s & (KS - 1);
+            const T bv = (s < KS ? pr : pi)[(8 * lq + k2of(u)) * TP + col];
+            const cT<T> f = (s < KS ? c2->re_in : c2->im_in)[b2 - k2of(u)];
             are = MF::fma(f.x, bv, are);
             aim = MF::fma(f.y, bv, aim);
         }
@@ -186,20 +196,40 @@ __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// x = hi + lo + O(2^-18 |x|), hi and lo bf16 (round to nearest even)
-__device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
+// x = hi + mid + lo + O(2^-26 |x|), each piece bf16 (round to nearest even):
+// every piece holds the leading 8 bits of what the previous ones left
+__device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& mid, bf16x8& lo) {
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         const __bf16 h = (__bf16)v[t];
+        const float r1 = v[t] - (float)h;
+        const __bf16 m = (__bf16)r1;
         hi[t] = h;
-        lo[t] = (__bf16)(v[t] - (float)h);
+        mid[t] = m;
+        lo[t] = (__bf16)(r1 - (float)m);
     }
 }
 
-// 3xBF16: two reduction steps of K = 32 (re half, im half); the 16x16x32 bf16
-// instruction takes A[l&15][8(l>>4) + t] and B[8(l>>4) + t][l&15], t < 8.
+// sum of the six split products a_i b_j with i + j <= 2 (hi = 0, mid = 1,
+// lo = 2), smallest first: every term of order 2^-16 or larger relative to
+// a b, so the product is good to about 2^-24, the FP32 rounding itself
+__device__ __forceinline__ f32x4 mfma_split6(bf16x8 ah, bf16x8 am, bf16x8 al, bf16x8 bh, bf16x8 bm, bf16x8 bl,
+                                             f32x4 c) {
+    c = mfma_bf16(al, bh, c);
+    c = mfma_bf16(am, bm, c);
+    c = mfma_bf16(ah, bl, c);
+    c = mfma_bf16(am, bh, c);
+    c = mfma_bf16(ah, bm, c);
+    return mfma_bf16(ah, bh, c);
+}
+
+// 3xBF16: each FP32 operand split into three bf16 pieces, the six products
+// of total order <= 2 (mfma_split6) accumulated in FP32; two reduction steps
+// of K = 32 (re half, im half); the 16x16x32 bf16 instruction takes
+// A[l&15][8(l>>4) + t] and B[8(l>>4) + t][l&15], t < 8.
 struct GemmBf16x3 {
-    bf16x8 a2re_h[2], a2re_l[2], a2im_h[2], a2im_l[2], b1re_h[2], b1re_l[2], b1im_h[2], b1im_l[2];
+    bf16x8 a2re_h[2], a2re_m[2], a2re_l[2], a2im_h[2], a2im_m[2], a2im_l[2];
+    bf16x8 b1re_h[2], b1re_m[2], b1re_l[2], b1im_h[2], b1im_m[2], b1im_l[2];
     int k2t, k1t, lr, lq;
     static __device__ __forceinline__ int orow(int q, int v) { return 4 * q + v; }
     __device__ __forceinline__ void init(const CircTab<float>* tab1, const CircTab<float>* tab2, int k2t_, int k1t_,
@@ -221,10 +251,10 @@ struct GemmBf16x3 {
                 v[2][t] = ent(tab1, 16 * k1t + lr, k, false);
                 v[3][t] = ent(tab1, 16 * k1t + lr, k, true);
             }
-            split8(v[0], a2re_h[s], a2re_l[s]);
-            split8(v[1], a2im_h[s], a2im_l[s]);
-            split8(v[2], b1re_h[s], b1re_l[s]);
-            split8(v[3], b1im_h[s], b1im_l[s]);
+            split8(v[0], a2re_h[s], a2re_m[s], a2re_l[s]);
+            split8(v[1], a2im_h[s], a2im_m[s], a2im_l[s]);
+            split8(v[2], b1re_h[s], b1re_m[s], b1re_l[s]);
+            split8(v[3], b1im_h[s], b1im_m[s], b1im_l[s]);
         }
     }
     __device__ __forceinline__ void t1(const float* pr, const float* pi, float2* out, float = 0.f) const {
@@ -236,14 +266,10 @@ struct GemmBf16x3 {
             float v[8];
 #pragma unroll
             for (int t = 0; t < 8; t++) v[t] = p[t];
-            bf16x8 h, l;
-            split8(v, h, l);
-            are = mfma_bf16(l, b1re_h[s], are);
-            are = mfma_bf16(h, b1re_l[s], are);
-            are = mfma_bf16(h, b1re_h[s], are);
-            aim = mfma_bf16(l, b1im_h[s], aim);
-            aim = mfma_bf16(h, b1im_l[s], aim);
-            aim = mfma_bf16(h, b1im_h[s], aim);
+            bf16x8 h, m, l;
+            split8(v, h, m, l);
+            are = mfma_split6(h, m, l, b1re_h[s], b1re_m[s], b1re_l[s], are);
+            aim = mfma_split6(h, m, l, b1im_h[s], b1im_m[s], b1im_l[s], aim);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) out[v] = cmk<float>(are[v], aim[v]);
@@ -257,14 +283,10 @@ struct GemmBf16x3 {
             float v[8];
 #pragma unroll
             for (int t = 0; t < 8; t++) v[t] = p[t * kPad];
-            bf16x8 h, l;
-            split8(v, h, l);
-            are = mfma_bf16(a2re_l[s], h, are);
-            are = mfma_bf16(a2re_h[s], l, are);
-            are = mfma_bf16(a2re_h[s], h, are);
-            aim = mfma_bf16(a2im_l[s], h, aim);
-            aim = mfma_bf16(a2im_h[s], l, aim);
-            aim = mfma_bf16(a2im_h[s], h, aim);
+            bf16x8 h, m, l;
+            split8(v, h, m, l);
+            are = mfma_split6(a2re_h[s], a2re_m[s], a2re_l[s], h, m, l, are);
+            aim = mfma_split6(a2im_h[s], a2im_m[s], a2im_l[s], h, m, l, aim);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) out[v] = cmk<float>(are[v], aim[v]);
@@ -685,6 +707,246 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
     }
 }
 
+// --------------------------------------------- fused kernel, 64 points ---
+//
+// Omega grids of 33 to 64 points per axis, reduced model (dim 2), forward
+// operator.  The slab of rotation row r is its whole (zero-padded) 64 x 64
+// omega plane; a workgroup of 16 waves covers it, wave w owning the 16 x 16
+// output tile (k2t, k1t) = (w >> 2, w & 3), lane l and accumulator v element
+// (16 k2t + orow(l >> 4, v), 16 k1t + (l & 15)), four per lane as in
+// k_drift_fused, whose terms and epilogue this kernel shares.  The circulants
+// are real 128 x 128 GEMMs on the (re | im) stacked plane.
+//
+// LDS: FP32 keeps three plane pairs (slab, V1, V2: two barriers per slab, as
+// k_drift_fused).  FP64 has room for two pairs only (4 x 64 x 66 x 8 B =
+// 135 KB of the 160 KB) and alternates their roles X, Y between slabs, with
+// three barriers per slab:
+//   F -> X; sync; P1, P2 from X; V2 -> Y; sync; Q2 from Y; V1 -> X; sync; Q1 from X.
+// The next slab writes its F into Y (read last by Q2, before the third sync)
+// and its V2 into X (read last by Q1, before the next slab's first sync).
+constexpr int kFused64Threads = 1024;
+template <class T> struct Fused64 {
+    static constexpr int pairs = sizeof(T) == 4 ? 3 : 2;
+    static constexpr int plane = 64 * GemmF<T, true, 64>::TP;  // elements of one real plane
+    static constexpr size_t lds = 2 * sizeof(CircTab<T, 64>) + (size_t)2 * pairs * plane * sizeof(T);
+};
+
+template <class T, bool RAMP, bool GRAV>
+__global__ __launch_bounds__(kFused64Threads, 1) void k_drift_fused64(DriftDev<T> D, MixDev<T> MX,
+                                                                      const cT<T>* Fin, const cT<T>* F0,
+                                                                      cT<T>* Fout, cT<T>* ACC, StageCoef<T> sc,
+                                                                      int nslab) {
+    using G = GemmF<T, RAMP, 64>;
+    constexpr int TP = G::TP, PL = Fused64<T>::plane, NPAIR = Fused64<T>::pairs;
+    constexpr int NC = 4;  // per source: -i CG_1, -i CG_2, -u_1, -u_2
+    // source rows whose loads are issued together: one for FP64, whose
+    // register budget at 16 waves per CU (128 VGPRs) holds no more
+    constexpr int MB = sizeof(T) == 8 ? 1 : MixBatch<T>::n;
+    extern __shared__ __align__(16) unsigned char smem64[];
+    CircTab<T, 64>* tabs = reinterpret_cast<CircTab<T, 64>*>(smem64);
+    T* planes = reinterpret_cast<T*>(smem64 + 2 * sizeof(CircTab<T, 64>));  // pair i: re at 2 i PL, im at (2 i + 1) PL
+
+    const int N = D.N, NN = N * N;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int k2t = w >> 2, k1t = w & 3, lr = lane & 15, lq = lane >> 4;
+    tabs[0].fill(D.kap, N, tid);
+    tabs[1].fill(D.kap + N, N, tid);
+    __syncthreads();
+    G gm;
+    gm.init(&tabs[0], &tabs[1], k2t, k1t, lr, lq);
+
+    // this lane's omega factors: kc of its elements and the diffusion
+    // D(k) = G_00 kd(k1) + G_11 kd(k2) - (G_01 + G_10) kc(k1) kc(k2)
+    const int k1 = 16 * k1t + lr;
+    const bool k1ok = k1 < N;
+    const T kc1 = k1ok ? D.kc[k1] : T(0), kd1 = k1ok ? D.kd[k1] : T(0);
+    const T g00 = D.G[0], g11 = D.G[3], g01 = D.G[1] + D.G[2];
+    T kc2v[4], ddv[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int k2 = 16 * k2t + G::orow(lq, v);
+        const bool ok = k1ok && k2 < N;
+        kc2v[v] = ok ? D.kc[k2] : T(0);
+        const T kd2 = ok ? D.kd[k2] : T(0);
+        ddv[v] = g00 * kd1 + g11 * kd2 - g01 * kc1 * kc2v[v];
+    }
+
+    // XCD x walks its own contiguous share of the rows (see k_drift_fused)
+    const int nxcd = gridDim.x >= 8 ? 8 : 1;
+    const int per = (int)gridDim.x / nxcd;
+    const int xcd = (int)blockIdx.x % nxcd;
+    const int hi = (int)((int64_t)nslab * (xcd + 1) / nxcd);
+    int par = 0;
+    for (int r = (int)((int64_t)nslab * xcd / nxcd) + (int)blockIdx.x / nxcd; r < hi; r += per) {
+        int zo;  // 0, opaque: the indices below are recomputed per slab (see k_drift_fused)
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
+        const int tidz = tid + zo;
+        const int lanez = tidz & 63, wz = tidz >> 6;
+        const int k1w = 16 * (wz & 3) + (lanez & 15), lqz = lanez >> 4;
+        int k2w[4];
+        unsigned offb[4];
+        bool ok[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            k2w[v] = 16 * (wz >> 2) + G::orow(lqz, v);
+            ok[v] = k1w < N && k2w[v] < N;
+            offb[v] = (unsigned)((k2w[v] < N ? k2w[v] : N - 1) * N + (k1w < N ? k1w : N - 1)) *
+                      (unsigned)sizeof(cT<T>);
+        }
+        const int64_t base = (int64_t)r * NN;
+        T* xre = planes + 2 * (NPAIR == 3 ? 0 : par) * PL;
+        T* xim = xre + PL;
+        T* yre = planes + 2 * (NPAIR == 3 ? 1 : 1 - par) * PL;
+        T* yim = yre + PL;
+        T* zre = planes + 2 * (NPAIR == 3 ? 2 : 0) * PL;  // three pairs only
+        T* zim = zre + PL;
+
+        cT<T> fo[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) fo[v] = *at_b(Fin + base, offb[v]);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {  // the lanes' elements tile the 64 x 64 plane once
+            xre[k2w[v] * TP + k1w] = ok[v] ? fo[v].x : T(0);
+            xim[k2w[v] * TP + k1w] = ok[v] ? fo[v].y : T(0);
+        }
+
+        // W starts as D(k) F - sum_p c_p (grid gravity term); the mixing adds
+        // the rotation terms
+        cT<T> V1[4], V2[4], Wv[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            V1[v] = czero<T>();
+            V2[v] = czero<T>();
+            Wv[v] = cscale<T>(fo[v], ddv[v]);
+            if constexpr (GRAV) {
+                Wv[v] = csub<T>(Wv[v], cimul<T>(*at_b(D.grav[0] + base, offb[v]), kc1));
+                Wv[v] = csub<T>(Wv[v], cimul<T>(*at_b(D.grav[1] + base, offb[v]), kc2v[v]));
+            }
+        }
+        const int TS = MX.TS;
+        const kc_ptr<int32_t> srcT = as_const<int32_t>(MX.src) + (int64_t)r * TS;
+        const kc_ptr<T> coefT = as_const<T>((const T*)MX.coef) + 2 * (int64_t)r * TS * NC;
+        for (int t0 = 0; t0 < TS; t0 += MB) {
+            int src[MB];
+            cT<T> fv[MB][4];
+#pragma unroll
+            for (int u = 0; u < MB; u++) src[u] = t0 + u < TS ? srcT[t0 + u] : -1;
+#pragma unroll
+            for (int u = 0; u < MB; u++) {
+                const cT<T>* Fs = Fin + (int64_t)(src[u] < 0 ? 0 : (src[u] & 0xffffff)) * NN;
+#pragma unroll
+                for (int v = 0; v < 4; v++) fv[u][v] = *at_b(Fs, offb[v]);
+            }
+#pragma unroll
+            for (int u = 0; u < MB; u++) {
+                if (src[u] < 0) continue;
+                const int fl = src[u] >> 24;
+                const int64_t cb = (int64_t)(t0 + u) * NC;
+                if (fl & 1) {
+                    const cT<T> g0 = kc_cplx<T>(coefT, cb), g1 = kc_cplx<T>(coefT, cb + 1);
+                    const cT<T> e = cscale<T>(g0, kc1);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) cfma<T>(Wv[v], cadd<T>(e, cscale<T>(g1, kc2v[v])), fv[u][v]);
+                }
+                if (fl & 2) {
+                    const cT<T> cu1 = kc_cplx<T>(coefT, cb + 2);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) cfma<T>(V1[v], cu1, fv[u][v]);
+                }
+                if (fl & 4) {
+                    const cT<T> cu2 = kc_cplx<T>(coefT, cb + 3);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) cfma<T>(V2[v], cu2, fv[u][v]);
+                }
+            }
+        }
+        __syncthreads();  // plane X complete; the planes the previous slab's GEMMs read are consumed
+
+        // V planes (zero in the padding) written before the P GEMMs, so that
+        // the V registers retire early: FP32 both (planes Y, Z), FP64 V2 (Y)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            yre[k2w[v] * TP + k1w] = ok[v] ? (NPAIR == 3 ? V1[v].x : V2[v].x) : T(0);
+            yim[k2w[v] * TP + k1w] = ok[v] ? (NPAIR == 3 ? V1[v].y : V2[v].y) : T(0);
+            if constexpr (NPAIR == 3) {
+                zre[k2w[v] * TP + k1w] = ok[v] ? V2[v].x : T(0);
+                zim[k2w[v] * TP + k1w] = ok[v] ? V2[v].y : T(0);
+            }
+        }
+        // S = W + b1 c1 P1 + b2 c2 P2
+        cT<T> P[4], S[4];
+        gm.t1(xre, xim, P, D.c0[0]);
+#pragma unroll
+        for (int v = 0; v < 4; v++) S[v] = cadd<T>(Wv[v], cimul<T>(P[v], D.b[0] * kc1));
+        gm.t2(xre, xim, P, D.c0[1]);
+#pragma unroll
+        for (int v = 0; v < 4; v++) S[v] = cadd<T>(S[v], cimul<T>(P[v], D.b[1] * kc2v[v]));
+        cT<T> Q1[4], Q2[4];
+        if constexpr (NPAIR == 3) {
+            __syncthreads();  // V planes complete; plane X consumed
+            gm.t1(yre, yim, Q1, D.c0[0]);
+            gm.t2(zre, zim, Q2, D.c0[1]);
+        } else {
+            __syncthreads();  // V2 in Y complete; plane X consumed
+            gm.t2(yre, yim, Q2, D.c0[1]);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                xre[k2w[v] * TP + k1w] = ok[v] ? V1[v].x : T(0);
+                xim[k2w[v] * TP + k1w] = ok[v] ? V1[v].y : T(0);
+            }
+            __syncthreads();  // V1 in X complete
+            gm.t1(xre, xim, Q1, D.c0[0]);
+            par ^= 1;
+        }
+
+        // epilogue (as k_drift_fused): all loads issued together and consumed
+        // before the first store; stores masked per element
+        cT<T> q[4], f0v[4], acv[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) q[v] = cadd<T>(cadd<T>(Q1[v], Q2[v]), S[v]);
+        if (sc.deriv_only) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (ok[v]) st_once<T>(at_b(Fout + base, offb[v]), cscale<T>(q[v], sc.dt));
+        } else {
+            if (sc.read_f0) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) f0v[v] = ld_once<T>(at_b(F0 + base, offb[v]));
+            } else if constexpr (sizeof(T) == 8) {
+                // the slab itself, read again (an L2 hit): FP64 does not keep
+                // it in registers across the slab
+#pragma unroll
+                for (int v = 0; v < 4; v++) f0v[v] = ld_once<T>(at_b(Fin + base, offb[v]));
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) f0v[v] = fo[v];
+            }
+            if (sc.read_acc) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) acv[v] = ld_once<T>(at_b(ACC + base, offb[v]));
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) acv[v] = czero<T>();
+            }
+            cT<T> fn[4], an[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const cT<T> upd = cadd<T>(cscale<T>(acv[v], sc.ca), cscale<T>(q[v], sc.cd));
+                fn[v] = cadd<T>(f0v[v], cscale<T>(upd, sc.dt));
+                an[v] = cadd<T>(cscale<T>(acv[v], sc.ea), cscale<T>(q[v], sc.ed));
+                asm volatile("" : "+v"(fn[v].x), "+v"(fn[v].y), "+v"(an[v].x), "+v"(an[v].y));
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (ok[v]) {
+                    st_once<T>(at_b(Fout + base, offb[v]), fn[v]);
+                    if (sc.write_acc) st_once<T>(at_b(ACC + base, offb[v]), an[v]);
+                }
+            }
+        }
+    }
+}
+
 // out[i] += scale * sum over the workgroups' partials, one workgroup of
 // kGradRedThreads: thread t sums partials t, t + kGradRedThreads, ... in that
 // order, then a fixed pairwise tree combines the threads, so the result does
@@ -846,8 +1108,8 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
         const float* p = (s == 0 ? sre : sim) + 8 * lq * CP + 16 * w + lr;
 #pragma unroll
         for (int t = 0; t < 8; t++) bv[t] = p[t * CP];
-        bf16x8 bh, bl;
-        split8(bv, bh, bl);
+        bf16x8 bh, bm, bl;
+        split8(bv, bh, bm, bl);
 #pragma unroll
         for (int it = 0; it < 4; it++) {
             float av[8];
@@ -857,11 +1119,9 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
                 const float2 c = (k < 32 ? tab.re_in : tab.im_in)[16 * (it & 1) + lr + 32 - (k & 31)];
                 av[t] = (it >> 1) ? c.y : c.x;
             }
-            bf16x8 ah, al;
-            split8(av, ah, al);
-            acc[it] = mfma_bf16(al, bh, acc[it]);
-            acc[it] = mfma_bf16(ah, bl, acc[it]);
-            acc[it] = mfma_bf16(ah, bh, acc[it]);
+            bf16x8 ah, am, al;
+            split8(av, ah, am, al);
+            acc[it] = mfma_split6(ah, am, al, bh, bm, bl, acc[it]);
         }
     }
     const int j = 64 * c + 16 * w + lr;
@@ -877,20 +1137,42 @@ __global__ __launch_bounds__(256) void k_axis3_bf16x3(DriftDev<float> D, const f
 
 // ---------------------------------------------------------- launchers ---
 
-// Workgroups of the persistent fused kernel: enough to fill every CU at the
-// kernel's occupancy, never more than there are slabs.
-template <class K> static unsigned persistent_grid(K kernel, int64_t nslab) {
-    static int cached = 0;  // one value per kernel instantiation
+// Workgroups of a persistent fused kernel: enough to fill every CU at the
+// kernel's occupancy, never more than there are slabs.  The resident count is
+// looked up once per (kernel, device).
+template <class K> static unsigned persistent_grid(K kernel, int64_t nslab, int threads = 256, size_t lds = 0) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> resident;
+    int dev = 0;
+    HPUP_HIP(hipGetDevice(&dev));
+    int cached = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = resident.find({(const void*)kernel, dev});
+        if (it != resident.end()) cached = it->second;
+    }
     if (!cached) {
-        int dev = 0, ncu = 0, per_cu = 0;
-        HPUP_HIP(hipGetDevice(&dev));
+        int ncu = 0, per_cu = 0;
         HPUP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HPUP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+        HPUP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
         cached = std::max(1, ncu * std::max(per_cu, 1));
+        std::lock_guard<std::mutex> lk(mu);
+        resident[{(const void*)kernel, dev}] = cached;
     }
     int64_t g = std::min<int64_t>(nslab, cached);
     if (g >= 8) g -= g % 8;  // whole workgroups per XCD
     return (unsigned)g;
+}
+
+template <class T, bool RAMP, bool GRAV>
+static void launch_fused64_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* F0, void* Fout,
+                             void* ACC, const StageCoef<T>& sc, int64_t nslab, hipStream_t st) {
+    auto kern = k_drift_fused64<T, RAMP, GRAV>;
+    const size_t sh = Fused64<T>::lds;
+    allow_lds((const void*)kern, sh);
+    const unsigned grid = persistent_grid(kern, nslab, kFused64Threads, sh);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kFused64Threads), sh, st, D, MX, (const cT<T>*)Fin, (const cT<T>*)F0,
+                       (cT<T>*)Fout, (cT<T>*)ACC, sc, (int)nslab);
 }
 
 template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false>
@@ -909,11 +1191,24 @@ static unsigned launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const 
 template <class T>
 void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
                         void* Fout, void* ACC, const StageCoef<T>& sc, bool bf16x3, hipStream_t st) {
-    HPUP_REQUIRE(D.N <= 32, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs N <= 32");
     HPUP_REQUIRE(D.dim == 2 || D.dim == 3, HPUP_ERR_UNSUPPORTED, "fused MFMA drift needs dim 2 or 3");
+    HPUP_REQUIRE(D.N <= 32 || (D.dim == 2 && D.N <= 64), HPUP_ERR_UNSUPPORTED,
+                 "fused MFMA drift needs N <= 32 (N <= 64 for dim 2)");
     const int64_t nslab = D.nrc * (D.dim == 3 ? D.N : 1);
     HPUP_REQUIRE(nslab < ((int64_t)1 << 30), HPUP_ERR_UNSUPPORTED, "fused MFMA drift: too many slabs");
     const bool grav = D.grav[0] || D.grav[1] || D.grav[2];
+    if (D.N > 32) {
+        HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 drift needs N <= 32");
+        HPUP_REQUIRE(!D.adj, HPUP_ERR_UNSUPPORTED, "the adjoint operator needs N <= 32");
+        if (D.ramp) {
+            if (grav) launch_fused64_t<T, true, true>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+            else launch_fused64_t<T, true, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+        } else {
+            if (grav) launch_fused64_t<T, false, true>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+            else launch_fused64_t<T, false, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+        }
+        return;
+    }
     if constexpr (sizeof(T) == 4) {
         if (bf16x3) {
             HPUP_REQUIRE(!grav, HPUP_ERR_UNSUPPORTED, "3xBF16 drift has no density-form gravity");

@@ -185,7 +185,7 @@ def _bench_plan_derivative(precision):
     return cfg, Ffull, dFfull
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5), ("bf16x3", 2e-5)])
 def test_benchmark_workload_derivative(precision, tol):
     """B = 15, 30^3 omega grid over [-5, 5]^3, the benchmark's seeded
     matrix-Fisher x Gaussian coefficients: the plan's derivative against the
