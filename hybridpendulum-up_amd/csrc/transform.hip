@@ -661,51 +661,13 @@ template <class T, int DIR> __device__ __forceinline__ void dft4(cT<T>* x) {
     x[3] = csub<T>(b, d);
 }
 
-// 60 = 4 * 3 * 5, Good-Thomas with the input map n = 15 n1 + 20 n2 + 12 n3
-// (mod 60) and the output map k = 45 k1 + 40 k2 + 36 k3 (mod 60), the CRT
-// idempotents (45 = 1 mod 4, 40 = 1 mod 3, 36 = 1 mod 5, each 0 modulo the
-// other factors): then n k = 15 n1 k1 + 20 n2 k2 + 12 n3 k3 (mod 60), a
-// twiddle-free 4 x 3 x 5 transform.
-template <class T, int DIR> __device__ __forceinline__ void fft60(cT<T>* x) {
-    cT<T> a[4][3][5];
-#pragma unroll
-    for (int n1 = 0; n1 < 4; n1++)
-#pragma unroll
-        for (int n2 = 0; n2 < 3; n2++)
-#pragma unroll
-            for (int n3 = 0; n3 < 5; n3++) a[n1][n2][n3] = x[(15 * n1 + 20 * n2 + 12 * n3) % 60];
-#pragma unroll
-    for (int n2 = 0; n2 < 3; n2++)
-#pragma unroll
-        for (int n3 = 0; n3 < 5; n3++) {
-            cT<T> v[4] = {a[0][n2][n3], a[1][n2][n3], a[2][n2][n3], a[3][n2][n3]};
-            dft4<T, DIR>(v);
-#pragma unroll
-            for (int k1 = 0; k1 < 4; k1++) a[k1][n2][n3] = v[k1];
-        }
-#pragma unroll
-    for (int n1 = 0; n1 < 4; n1++)
-#pragma unroll
-        for (int n3 = 0; n3 < 5; n3++) dft3<T, DIR>(a[n1][0][n3], a[n1][1][n3], a[n1][2][n3]);
-#pragma unroll
-    for (int n1 = 0; n1 < 4; n1++)
-#pragma unroll
-        for (int n2 = 0; n2 < 3; n2++) dft5<T, DIR>(a[n1][n2]);
-#pragma unroll
-    for (int k1 = 0; k1 < 4; k1++)
-#pragma unroll
-        for (int k2 = 0; k2 < 3; k2++)
-#pragma unroll
-            for (int k3 = 0; k3 < 5; k3++) x[(45 * k1 + 40 * k2 + 36 * k3) % 60] = a[k1][k2][k3];
-}
-
-// the length-2B transform of the register-row FFT kernels (2B = 30 or 60)
+// the length-2B transform of the register-row FFT kernels (2B = 30)
 template <class T, int DIR, int NF> __device__ __forceinline__ void fftn(cT<T>* x) {
-    if constexpr (NF == 30) fft30<T, DIR>(x);
-    else fft60<T, DIR>(x);
+    static_assert(NF == 30, "register-row FFTs are built for 2B = 30");
+    fft30<T, DIR>(x);
 }
 
-// Inverse (alpha, gamma) stage at 2B = 30 or 60: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}
+// Inverse (alpha, gamma) stage at 2B = 30: H[m][g] = sum_n G[m][n] e^{-i n gamma_g}
 // (item (m, xl): its G row, frequency n at index n mod 30), then
 // f[a][g] = Re sum_m H[m][g] e^{-i m alpha_a} (item (g, xl): its H column).
 // Row stride (elements) of the FFT kernels' H tile [m][g][xl]: padded so that
@@ -760,7 +722,7 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft(XformDev<T> X, const cT<T>* _
     }
 }
 
-// Forward (alpha, gamma) stage at 2B = 30 or 60: H[m][g] = sum_a f[a][g] e^{+i m alpha_a},
+// Forward (alpha, gamma) stage at 2B = 30: H[m][g] = sum_a f[a][g] e^{+i m alpha_a},
 // m >= 0 (item (g, xl): its f row), then G[m][n] = sum_g H[m][g] e^{+i n gamma_g}
 // and G[-m][-n] = conj (item (m, xl): its H row).
 template <class T, int B, int XC, int NT>
@@ -819,6 +781,227 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_fft(XformDev<T> X, const T* __res
     }
 }
 
+// ------------------------------------------- (alpha, gamma) stages, 2B = 60 --
+//
+// A whole 60-point vector per thread needs more than 120 VGPRs (one wave per
+// SIMD), so each length-60 DFT is spread over the four lanes of a quad: lane
+// q holds x[4t + q], t < 15, transforms it with a 15-point Good-Thomas FFT
+// (15 = 3 x 5: input n = 5 n1 + 3 n2, output k = 10 k1 + 6 k2, mod 15),
+// multiplies by W^(q k), W = e^{DIR 2 pi i / 60}, and a radix-4 DFT across
+// the quad in two lane exchanges finishes the transform:
+//   X[k + 15 p] = sum_q W4^(q p) W^(q k) Y_q[k],  W4 = W^15 = DIR i.
+// With p = 2 p1 + p0 and q = 2 q1 + q0 the exchange with lane q ^ 2 forms
+// sum_q1 (-1)^(q1 p0) y_(2 q1 + q0), lane q keeping p0 = q1; lane 3 then
+// multiplies by W4 (q0 = p0 = 1); the exchange with lane q ^ 1 forms the sum
+// over q0 with p1 = q0.  Lane L ends with X[k + 15 rev(L)], k < 15, rev the
+// 2-bit reversal.
+
+template <class T, int DIR> __device__ __forceinline__ void fft15(cT<T>* x) {
+    cT<T> a[3][5];
+#pragma unroll
+    for (int n1 = 0; n1 < 3; n1++)
+#pragma unroll
+        for (int n2 = 0; n2 < 5; n2++) a[n1][n2] = x[(5 * n1 + 3 * n2) % 15];
+#pragma unroll
+    for (int n2 = 0; n2 < 5; n2++) dft3<T, DIR>(a[0][n2], a[1][n2], a[2][n2]);
+#pragma unroll
+    for (int n1 = 0; n1 < 3; n1++) dft5<T, DIR>(a[n1]);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; k1++)
+#pragma unroll
+        for (int k2 = 0; k2 < 5; k2++) x[(10 * k1 + 6 * k2) % 15] = a[k1][k2];
+}
+
+// the value of lane (l ^ 1) or (l ^ 2) of the quad (DPP quad_perm [1,0,3,2] =
+// 0xB1, [2,3,0,1] = 0x4E); every lane of the quad must be active
+template <int CTRL> __device__ __forceinline__ float qperm(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> __device__ __forceinline__ double qperm(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL, class T> __device__ __forceinline__ cT<T> qperm_c(cT<T> v) {
+    return cmk<T>(qperm<CTRL>(v.x), qperm<CTRL>(v.y));
+}
+__device__ __forceinline__ int rev2(int q) { return ((q & 1) << 1) | (q >> 1); }
+
+// tw[t] = e^{+2 pi i t / 60}, t < 60 (LDS)
+template <class T, int DIR> __device__ __forceinline__ void fft60_quad(cT<T>* y, int q, const cT<T>* tw) {
+    fft15<T, DIR>(y);
+#pragma unroll
+    for (int k = 1; k < 15; k++) {
+        cT<T> w = tw[q * k];
+        if (DIR < 0) w.y = -w.y;
+        y[k] = cmul<T>(y[k], w);
+    }
+    const bool hi = q & 2, lo = q & 1;
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+        const cT<T> p = qperm_c<0x4E, T>(y[k]);
+        y[k] = hi ? csub<T>(p, y[k]) : cadd<T>(y[k], p);
+        if (q == 3) y[k] = irot<T, DIR>(y[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+        const cT<T> p = qperm_c<0xB1, T>(y[k]);
+        y[k] = lo ? csub<T>(p, y[k]) : cadd<T>(y[k], p);
+    }
+}
+
+// Forward (alpha, gamma) stage at 2B = 60, block (beta_j, XC omega points),
+// one quad lane per thread in each stage:
+//  A. rows g and g + 30 of the f tile as one complex vector z = f_g + i f_(g+30),
+//     transformed over alpha (DIR +1); the packed spectrum Z_g goes to LDS;
+//  B. per (xl, m < 30): H[m][g] = (Z_g[m] + conj Z_g[-m]) / 2 for g < 30 and
+//     (Z_(g-30)[m] - conj Z_(g-30)[-m]) / (2i) for g >= 30 (the spectra of
+//     the two real rows), transformed over gamma (DIR +1): G[m][n] for
+//     n = -29..29, and G[-m][-n] = conj G[m][n].
+template <class T, int XC, int NT>
+__global__ __launch_bounds__(NT) void k_fwd_ag_q60(XformDev<T> X, const T* __restrict__ f, const T* __restrict__ wR,
+                                                   int wstride, cT<T>* __restrict__ G, int nxc) {
+    constexpr int B = 30, n2 = 60, lmax = 29, M = 59, rp = n2 + 1, ZS = n2;
+    constexpr int TQ = XC * 30 * 4;
+    static_assert(TQ <= NT, "one quad lane per thread");
+    extern __shared__ char smem_raw[];
+    cT<T>* tw = reinterpret_cast<cT<T>*>(smem_raw);
+    T* sf = reinterpret_cast<T*>(smem_raw + 64 * sizeof(cT<T>));       // [xl][g][a], row stride rp
+    cT<T>* sZ = reinterpret_cast<cT<T>*>(smem_raw + 64 * sizeof(cT<T>));  // [xl][g < 30][K], aliases sf
+    const int w = xcd_major(blockIdx.x, gridDim.x);
+    const int j = w / nxc;
+    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    const int64_t nx = X.nx;
+    const int tid = threadIdx.x;
+    const int64_t n2c = (int64_t)n2 * n2 * n2;
+    if (tid < n2) tw[tid] = X.twB[tid];
+    for (int i = tid; i < XC * n2 * n2; i += NT) {
+        const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
+        const int64_t x = x0 + xl;
+        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
+        T v = x < nx ? f[iR + n2c * x] : T(0);
+        if (wR) v *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
+        sf[(xl * n2 + g) * rp + a] = v;
+    }
+    __syncthreads();
+    // quads beyond TQ run on clamped in-range data and store nothing
+    const int q = tid & 3, P = rev2(q);
+    const bool act = tid < TQ;
+    const int pr = act ? tid >> 2 : 0;
+    cT<T> y[15];
+    {
+        const int xl = pr / 30, g = pr % 30;
+        const T* r0 = sf + (xl * n2 + g) * rp + q;
+        const T* r1 = r0 + 30 * rp;
+#pragma unroll
+        for (int t = 0; t < 15; t++) y[t] = cmk<T>(r0[4 * t], r1[4 * t]);
+    }
+    fft60_quad<T, 1>(y, q, tw);
+    __syncthreads();  // sZ overwrites sf
+    if (act) {
+        cT<T>* zr = sZ + pr * ZS + 15 * P;
+#pragma unroll
+        for (int k = 0; k < 15; k++) zr[k] = y[k];
+    }
+    __syncthreads();
+    const int xl = pr / 30, m = pr % 30, mneg = m == 0 ? 0 : n2 - m;
+#pragma unroll
+    for (int t = 0; t < 15; t++) {
+        const int g = 4 * t + q, gr = g < 30 ? g : g - 30;
+        const cT<T>* zr = sZ + (xl * 30 + gr) * ZS;
+        const cT<T> a = zr[m], b = cconj<T>(zr[mneg]);
+        const cT<T> s = g < 30 ? cadd<T>(a, b) : csub<T>(a, b);
+        // (a + b) / 2, or (a - b) / (2i) = -i (a - b) / 2
+        y[t] = g < 30 ? cmk<T>(T(0.5) * s.x, T(0.5) * s.y) : cmk<T>(T(0.5) * s.y, T(-0.5) * s.x);
+    }
+    fft60_quad<T, 1>(y, q, tw);
+    const int64_t x = x0 + xl;
+    if (act && x < nx) {
+        cT<T>* Gx = G + g_off(B, 0, j, x);
+#pragma unroll
+        for (int k = 0; k < 15; k++) {
+            const int K = k + 15 * P;  // frequency n = K or K - 60; the Nyquist bin K = 30 is dropped
+            if (K == 30) continue;
+            const int ni = K <= lmax ? K + lmax : K - 31;
+            Gx[((int64_t)(m + lmax) + M * ni) * n2 * kGX] = y[k];
+            if (m > 0) Gx[((int64_t)(lmax - m) + M * (M - 1 - ni)) * n2 * kGX] = cconj<T>(y[k]);
+        }
+    }
+}
+
+// Inverse (alpha, gamma) stage at 2B = 60, block (beta_j, XC omega points):
+//  1. per (xl, mi < 59): v[K] = G[mi][ni] at K = ni - 29 (mod 60), v[30] = 0,
+//     transformed over n (DIR -1): H[mi][g] to LDS;
+//  2. per (xl, g): v[K] = H[mi][g] at K = mi - 29 (mod 60), v[30] = 0,
+//     transformed over m (DIR -1); f[a][g] = Re v[a], staged in LDS and
+//     written row by row (60 consecutive reals per (g, xl)).
+template <class T, int XC, int NT>
+__global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
+                                                   int nxc) {
+    constexpr int B = 30, n2 = 60, M = 59, HS = n2;
+    constexpr int T1 = XC * M * 4, T2 = XC * n2 * 4;
+    static_assert(T1 <= NT && T2 <= NT, "one quad lane per thread");
+    extern __shared__ char smem_raw[];
+    cT<T>* tw = reinterpret_cast<cT<T>*>(smem_raw);
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw + 64 * sizeof(cT<T>));  // [xl][mi][g]
+    T* so = reinterpret_cast<T*>(smem_raw + 64 * sizeof(cT<T>));          // [xl][g][a], aliases sH
+    const int w = xcd_major(blockIdx.x, gridDim.x);
+    const int j = w / nxc;
+    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    const int64_t nx = X.nx;
+    const int tid = threadIdx.x;
+    if (tid < n2) tw[tid] = X.twB[tid];
+    const int q = tid & 3, P = rev2(q);
+    cT<T> y[15];
+    const bool act1 = tid < T1;
+    {
+        const int pr = act1 ? tid >> 2 : 0;
+        const int xl = pr % XC, mi = pr / XC;
+        const int64_t x = x0 + xl;
+        const bool ld = x < nx;
+        // clamped loads; a column beyond nx is zeroed after the load
+        const cT<T>* Gp = G + g_off(B, mi, j, min<int64_t>(x, nx - 1));
+#pragma unroll
+        for (int t = 0; t < 15; t++) {
+            const int ni = (4 * t + q + 29) % n2;  // 59 at the Nyquist bin K = 30
+            const cT<T> g = Gp[(int64_t)M * (ni < M ? ni : 0) * n2 * kGX];
+            y[t] = (ld && ni < M) ? g : czero<T>();
+        }
+    }
+    __syncthreads();  // twiddles
+    fft60_quad<T, -1>(y, q, tw);
+    if (act1) {
+        const int pr = tid >> 2, xl = pr % XC, mi = pr / XC;
+        cT<T>* hr = sH + (xl * M + mi) * HS + 15 * P;
+#pragma unroll
+        for (int k = 0; k < 15; k++) hr[k] = y[k];
+    }
+    __syncthreads();
+    const bool act2 = tid < T2;
+    const int pr2 = act2 ? tid >> 2 : 0;
+    const int xl2 = pr2 % XC, g2 = pr2 / XC;
+#pragma unroll
+    for (int t = 0; t < 15; t++) {
+        const int mi = (4 * t + q + 29) % n2;
+        const cT<T> h = sH[(xl2 * M + (mi < M ? mi : 0)) * HS + g2];
+        y[t] = mi < M ? h : czero<T>();
+    }
+    fft60_quad<T, -1>(y, q, tw);
+    __syncthreads();  // so overwrites sH
+    if (act2) {
+        T* orow = so + (xl2 * n2 + g2) * n2 + 15 * P;
+#pragma unroll
+        for (int k = 0; k < 15; k++) orow[k] = y[k].x;
+    }
+    __syncthreads();
+    for (int e = tid; e < XC * n2 * n2; e += NT) {
+        const int xl = e / (n2 * n2), g = (e / n2) % n2, a = e % n2;
+        const int64_t x = x0 + xl;
+        if (x < nx) f[a + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x))] = so[(xl * n2 + g) * n2 + a];
+    }
+}
+
 // ---------------------------------------------------------- launchers -----
 
 static int pick_xc(size_t per_x, size_t budget, int64_t nx) {
@@ -868,6 +1051,28 @@ static void inv_ag_fft(const XformDev<T>& X, const void* G, void* f, hipStream_t
                        (const cT<T>*)G, (T*)f, nxc);
 }
 
+template <class T, int XC, int NT>
+static void fwd_ag_q60(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
+    constexpr int n2 = 60;
+    const int nxc = (int)((X.nx + XC - 1) / XC);
+    HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+    const size_t sh = 64 * sizeof(cT<T>) + std::max(sizeof(T) * XC * n2 * (n2 + 1), sizeof(cT<T>) * XC * 30 * n2);
+    allow_lds((const void*)k_fwd_ag_q60<T, XC, NT>, sh);
+    hipLaunchKernelGGL((k_fwd_ag_q60<T, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X, (const T*)f,
+                       (const T*)wR, wstride, (cT<T>*)G, nxc);
+}
+
+template <class T, int XC, int NT>
+static void inv_ag_q60(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
+    constexpr int n2 = 60, M = 59;
+    const int nxc = (int)((X.nx + XC - 1) / XC);
+    HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
+    const size_t sh = 64 * sizeof(cT<T>) + std::max(sizeof(cT<T>) * XC * M * n2, sizeof(T) * XC * n2 * n2);
+    allow_lds((const void*)k_inv_ag_q60<T, XC, NT>, sh);
+    hipLaunchKernelGGL((k_inv_ag_q60<T, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X,
+                       (const cT<T>*)G, (T*)f, nxc);
+}
+
 template <class T>
 void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
     const int n2 = 2 * X.B;
@@ -875,9 +1080,9 @@ void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstr
         fwd_ag_fft<T, 15, kFftFwdXC, kFftNT>(X, f, wR, wstride, G, st);
         return;
     }
-    if (n2 == 60) {  // 2B = 60: two (FP64) or four (FP32) omega points per block
-        if constexpr (sizeof(T) == 8) fwd_ag_fft<T, 30, 2, 128>(X, f, wR, wstride, G, st);
-        else fwd_ag_fft<T, 30, 4, 256>(X, f, wR, wstride, G, st);
+    if (n2 == 60) {  // quad FFTs: two (FP64) or four (FP32) omega points per block
+        if constexpr (sizeof(T) == 8) fwd_ag_q60<T, 2, 256>(X, f, wR, wstride, G, st);
+        else fwd_ag_q60<T, 4, 512>(X, f, wR, wstride, G, st);
         return;
     }
     if (n2 <= kAgMax) {
@@ -943,9 +1148,9 @@ void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st)
         inv_ag_fft<T, 15, kFftInvXC * (sizeof(T) == 8 ? 1 : 2), kFftNT>(X, G, f, st);
         return;
     }
-    if (n2 == 60) {
-        if constexpr (sizeof(T) == 8) inv_ag_fft<T, 30, 2, 128>(X, G, f, st);
-        else inv_ag_fft<T, 30, 4, 256>(X, G, f, st);
+    if (n2 == 60) {  // quad FFTs: one (FP64) or two (FP32) omega points per block
+        if constexpr (sizeof(T) == 8) inv_ag_q60<T, 1, 256>(X, G, f, st);
+        else inv_ag_q60<T, 2, 512>(X, G, f, st);
         return;
     }
     if (n2 <= kAgMax) {
