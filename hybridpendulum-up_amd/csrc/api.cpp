@@ -319,8 +319,7 @@ template <class T> struct PlanT : PlanBase {
         const int M = 2 * model.B - 1;
         ensure(G, sizeof(cT<T>) * (size_t)g_elems(model.B, nx));
         ensure(tmp, cbytes());
-        HPUP_HIP(hipMemcpyAsync(tmp.p, F, cbytes(), hipMemcpyDeviceToDevice, st));
-        launch_omega_dft<T>(xf.view, tmp.p, nullptr, true, st);
+        launch_omega_dft_to<T>(xf.view, F, tmp.p, true, st);
         launch_beta_inv<T>(xf.view, tmp.p, G.p, st);
         launch_inv_ag<T>(xf.view, G.p, f, st);
         HPUP_HIP(hipGetLastError());

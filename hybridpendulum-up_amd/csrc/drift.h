@@ -144,6 +144,10 @@ void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st)
 // in-place omega DFT of the compact rows (forward sign -1, inverse +1 with 1/nx)
 template <class T>
 void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hipStream_t st);
+// the same from in to out (out = in allowed): one pass where the row kernel
+// serves the grid, else a copy and the in-place transform
+template <class T>
+void launch_omega_dft_to(const XformDev<T>& X, const void* in, void* out, bool inverse, hipStream_t st);
 
 // layout conversion full (M,M,B,nx) <-> compact [r][x]
 template <class T>

@@ -104,20 +104,28 @@ template <class T, bool RAMP = false, int TILE = 32> struct GemmF {
     const CircTab<T, TILE>* c1;
     const CircTab<T, TILE>* c2;
     int k2t, k1t, lr, lq, b1, b2;
+    // 32-point tiles: K = 4 steps per re/im half that reach the data, ks =
+    // ceil(n / 4) for n data points per axis (the rest of the tile is zero),
+    // and T2's lane stride in its reduction order
+    int ks;
     static __device__ __forceinline__ int orow(int q, int v) { return MF::orow(q, v); }
-    // T2's reduction index of step u (< KS) in each re/im half
+    // T2's reduction index of step u (< KS) in each re/im half (64-point tiles)
     static __device__ __forceinline__ int k2of(int u) { return (u & 7) + 32 * (u >> 3); }
     __device__ __forceinline__ void init(const CircTab<T, TILE>* tab1, const CircTab<T, TILE>* tab2, int k2t_,
-                                         int k1t_, int lr_, int lq_) {
+                                         int k1t_, int lr_, int lq_, int n = TILE) {
         c1 = tab1; c2 = tab2;
         k2t = k2t_; k1t = k1t_; lr = lr_; lq = lq_;
-        // T1 reduces over k = 4 u + lq, T2 over k = 8 lq + k2of(u) (in each
-        // re/im half): the order of a GEMM's reduction is free, and these
-        // orders make the data reads conflict-free (TP = 2 mod 32).
-        // d = o - k + TILE.
+        ks = TILE == 32 ? (n + 3) / 4 : KS;
+        // T1 reduces over k = 4 u + lq; T2 over k = ks lq + u (32-point tiles)
+        // or 8 lq + k2of(u) (64-point tiles), in each re/im half.  The order
+        // of a GEMM's reduction is free; with TP = 2 mod 32 these orders keep
+        // the data reads conflict-free at ks = 8 (a few 2-way conflicts for
+        // smaller ks).  d = o - k + TILE.
         b1 = 16 * k1t + lr + TILE - lq;
-        b2 = 16 * k2t + lr + TILE - 8 * lq;
+        b2 = 16 * k2t + lr + TILE - (TILE == 32 ? ks : 8) * lq;
     }
+    // T2's reduction index (relative to its lane's start) of step u
+    __device__ __forceinline__ int k2u(int u) const { return TILE == 32 ? u : k2of(u); }
     // Y = X C1^T along k1 (slab rows as the A operand); c0: Re kappa_1 (RAMP)
     __device__ __forceinline__ void t1(const T* pr, const T* pi, cT<T>* out, T c0 = T(0)) const {
         acc_t are = MF::zero(), aim = MF::zero();
@@ -125,7 +133,7 @@ template <class T, bool RAMP = false, int TILE = 32> struct GemmF {
         if constexpr (RAMP) {
             T sr = T(0), si = T(0);
 #pragma unroll 1
-            for (int s = 0; s < KS; s++) {
+            for (int s = 0; s < ks; s++) {
                 const int col = 4 * s + lq;
                 const T xr = pr[row + col], xi = pi[row + col];
                 const cT<T> f = c1->rim[b1 - 4 * s];
@@ -146,10 +154,16 @@ template <class T, bool RAMP = false, int TILE = 32> struct GemmF {
             return;
         }
 #pragma unroll 4
-        for (int s = 0; s < 2 * KS; s++) {
-            const int u = s & (KS - 1);
-            const T av = (s < KS ? pr : pi)[row + 4 * u + lq];
-            const cT<T> f = (s < KS ? c1->re_in : c1->im_in)[b1 - 4 * u];
+        for (int u = 0; u < ks; u++) {  // real half
+            const T av = pr[row + 4 * u + lq];
+            const cT<T> f = c1->re_in[b1 - 4 * u];
+            are = MF::fma(av, f.x, are);
+            aim = MF::fma(av, f.y, aim);
+        }
+#pragma unroll 4
+        for (int u = 0; u < ks; u++) {  // imaginary half
+            const T av = pi[row + 4 * u + lq];
+            const cT<T> f = c1->im_in[b1 - 4 * u];
             are = MF::fma(av, f.x, are);
             aim = MF::fma(av, f.y, aim);
         }
@@ -162,11 +176,12 @@ template <class T, bool RAMP = false, int TILE = 32> struct GemmF {
         const int col = 16 * k1t + lr;
         if constexpr (RAMP) {
             T sr = T(0), si = T(0);
+            const int r0 = (TILE == 32 ? ks : 8) * lq;
 #pragma unroll 1
-            for (int s = 0; s < KS; s++) {
-                const int rw = (8 * lq + k2of(s)) * TP + col;
+            for (int s = 0; s < ks; s++) {
+                const int rw = (r0 + k2u(s)) * TP + col;
                 const T xr = pr[rw], xi = pi[rw];
-                const cT<T> f = c2->rim[b2 - k2of(s)];
+                const cT<T> f = c2->rim[b2 - k2u(s)];
                 are = MF::fma(f.x, xi, are);
                 aim = MF::fma(f.y, xr, aim);
                 sr += xr;
@@ -179,11 +194,18 @@ template <class T, bool RAMP = false, int TILE = 32> struct GemmF {
             for (int v = 0; v < 4; v++) out[v] = cmk<T>(are[v] + c0 * sr, aim[v] + c0 * si);
             return;
         }
+        const int r0 = (TILE == 32 ? ks : 8) * lq;
 #pragma unroll 4
-        for (int s = 0; s < 2 * KS; s++) {
-            const int u = s & (KS - 1);
-            const T bv = (s < KS ? pr : pi)[(8 * lq + k2of(u)) * TP + col];
-            const cT<T> f = (s < KS ? c2->re_in : c2->im_in)[b2 - k2of(u)];
+        for (int u = 0; u < ks; u++) {  // real half
+            const T bv = pr[(r0 + k2u(u)) * TP + col];
+            const cT<T> f = c2->re_in[b2 - k2u(u)];
+            are = MF::fma(f.x, bv, are);
+            aim = MF::fma(f.y, bv, aim);
+        }
+#pragma unroll 4
+        for (int u = 0; u < ks; u++) {  // imaginary half
+            const T bv = pi[(r0 + k2u(u)) * TP + col];
+            const cT<T> f = c2->im_in[b2 - k2u(u)];
             are = MF::fma(f.x, bv, are);
             aim = MF::fma(f.y, bv, aim);
         }
@@ -233,7 +255,7 @@ struct GemmBf16x3 {
     int k2t, k1t, lr, lq;
     static __device__ __forceinline__ int orow(int q, int v) { return 4 * q + v; }
     __device__ __forceinline__ void init(const CircTab<float>* tab1, const CircTab<float>* tab2, int k2t_, int k1t_,
-                                         int lr_, int lq_) {
+                                         int lr_, int lq_, int = 32) {
         k2t = k2t_; k1t = k1t_; lr = lr_; lq = lq_;
         // entry (o, k) of the real-ified circulant, as GemmF reads it
         auto ent = [](const CircTab<float>* t, int o, int k, bool im_out) {
@@ -380,7 +402,7 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
     tabs[1].fill(D.kap + N, N, tid);
     __syncthreads();
     G gm;
-    gm.init(&tabs[0], &tabs[1], k2t, k1t, lr, lq);
+    gm.init(&tabs[0], &tabs[1], k2t, k1t, lr, lq, N);
 
     // this lane's four elements (k2, k1) of every slab: k1 = 16 k1t + lr and
     // k2 = 16 k2t + orow(lq, v); their omega factors stay in registers, their

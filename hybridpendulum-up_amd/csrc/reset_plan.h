@@ -13,7 +13,7 @@ constexpr int kResetMaxN = 64;
 struct ResetConsts {
     int N;
     double xg[kResetMaxN];  // omega grid values along an axis
-    double x0, hx;          // 3-D: grid origin and spacing
+    double x0, hx, ihx;     // 3-D: grid origin, spacing and its reciprocal
     double xlo, xhi;        // 2-D: getOmega's box of valid pull-backs
     double Lg, dx2;         // 2-D: getIndRule's box length and cell area
     double kfac;            // (1 + eps) / eps

@@ -27,8 +27,11 @@
 // Layouts: the plan's grid (nR, nx) with the rotation index fastest; the
 // region's rows k (rotations with theta >= theta0 - theta_t, ascending) and a
 // compact df (numR, nx) with k fastest.  The geometry is FP64 in every
-// precision (IEEE operations in the oracle's order: contraction off), so the
-// masks and interpolation weights do not depend on the plan's precision.
+// precision, so the masks and interpolation weights do not depend on the
+// plan's precision; dim 2 keeps the oracle's operation order (contraction
+// off), dim 3 locates cells and weights by multiplying with 1/h (the
+// interpolant is continuous across cell faces, so a face assigned to either
+// neighbouring cell gives the same value to rounding).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -132,10 +135,10 @@ __global__ __launch_bounds__(256) void k_reset_df(const T* __restrict__ f, const
         bool support = true;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const double cf = floor((Oo[a] - c.x0) / c.hx);
+            const double cf = floor((Oo[a] - c.x0) * c.ihx);
             if (!(cf >= -1.0 && cf <= (double)(N - 1))) support = false;
             cl[a] = support ? (int64_t)cf : 0;
-            s[a] = (Oo[a] - (c.x0 + c.hx * (double)cl[a])) / c.hx;
+            s[a] = (Oo[a] - (c.x0 + c.hx * (double)cl[a])) * c.ihx;
         }
         if (support && vq < 0.0) {
             double fv[8], wt[8];
@@ -169,23 +172,67 @@ __global__ __launch_bounds__(256) void k_reset_apply(const T* __restrict__ dfc, 
     out[i] = base ? (T)((double)base[i] + d) : (T)d;
 }
 
-// The jump term's pieces at one (region row, omega point), in FP64: the
-// outflow mask, the pulled-back point with its interpolation corners and
-// weights (inflow mask false: no corners), and d(pull-back)/d(eps).
+// A region row's geometry: R (column-major), t, PC, lambda and (dim 3) the
+// contact factor sigma = (PC x e_w) . t.
+struct RowGeo {
+    double R[9], t[3], P[3], lam, sigma;
+};
+
+__device__ __forceinline__ RowGeo row_geo(const double* g, int stride, int k, const ResetConsts& c) {
+    RowGeo r;
+#pragma unroll
+    for (int a = 0; a < 9; a++) r.R[a] = g[(int64_t)(G_R + a) * stride + k];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        r.t[a] = g[(int64_t)(G_T + a) * stride + k];
+        r.P[a] = g[(int64_t)(G_PC + a) * stride + k];
+    }
+    r.lam = g[(int64_t)G_LAM * stride + k];
+    r.sigma = contact_sigma(r.P, c.ew, r.t);
+    return r;
+}
+
+// omega point x -> grid indices (i0, i1, i2), and their advance by a fixed
+// step decomposed the same way (no division in the loops)
+struct OmegaIdx {
+    int i0, i1, i2;
+    __device__ __forceinline__ void set(int x, int N) {
+        i0 = x % N;
+        const int q = x / N;
+        i1 = q % N;
+        i2 = q / N;
+    }
+    __device__ __forceinline__ void advance(const OmegaIdx& d, int N) {
+        i0 += d.i0;
+        const int c0 = i0 >= N ? 1 : 0;
+        i0 -= c0 * N;
+        i1 += d.i1 + c0;
+        const int c1 = i1 >= N ? 1 : 0;
+        i1 -= c1 * N;
+        i2 += d.i2 + c1;
+    }
+};
+
+// The jump term's pieces at one (region row, omega point (i0, i1, i2)), in
+// FP64: the outflow mask, the pulled-back point with its interpolation
+// corners (omega point indices) and weights (inflow mask false: no corners),
+// and d(pull-back)/d(eps).
 template <int DIM> struct Jump {
     static constexpr int NQ = DIM == 2 ? 4 : 8;
     bool out_on, in_on;
-    int64_t q[NQ];
+    int q[NQ];
     double w[NQ];
     double dw[NQ];  // d w_q / d eps through the pulled-back point
 };
 
 template <int DIM>
-__device__ __forceinline__ void jump_at(const double* Rr, const double* t, const double* P, const ResetConsts& c,
-                                        int64_t ix, double deps_scale, Jump<DIM>& J) {
+__device__ __forceinline__ void jump_at(const RowGeo& G, const ResetConsts& c, int i0, int i1, int i2,
+                                        double deps_scale, Jump<DIM>& J) {
 #pragma clang fp contract(off)
     const int N = c.N;
-    const int64_t i0 = ix % N, i1 = (ix / N) % N;
+    const double* Rr = G.R;
+    const double* t = G.t;
+    const double* P = G.P;
     J.in_on = false;
     if constexpr (DIM == 2) {
         const double x1 = c.xg[i0], x2 = c.xg[i1];
@@ -211,7 +258,7 @@ __device__ __forceinline__ void jump_at(const double* Rr, const double* t, const
         J.w[1] = a * e / c.dx2;
         J.w[2] = b * cc / c.dx2;
         J.w[3] = a * cc / c.dx2;
-        const int64_t q0 = j1 + (int64_t)j2 * N;
+        const int q0 = j1 + j2 * N;
         J.q[0] = q0;
         J.q[1] = q0 + 1;
         J.q[2] = q0 + N;
@@ -224,32 +271,31 @@ __device__ __forceinline__ void jump_at(const double* Rr, const double* t, const
         J.dw[2] = (-cc * da0 + b * da1) / c.dx2;
         J.dw[3] = (cc * da0 + a * da1) / c.dx2;
     } else {
-        const int64_t i2 = ix / ((int64_t)N * N);
         const double Om[3] = {c.x0 + c.hx * (double)i0, c.x0 + c.hx * (double)i1, c.x0 + c.hx * (double)i2};
         double w[3];
         for (int a = 0; a < 3; a++) w[a] = Rr[a] * Om[0] + Rr[a + 3] * Om[1] + Rr[a + 6] * Om[2];
         const double dot = w[0] * t[0] + w[1] * t[1] + w[2] * t[2];
-        const double vq = contact_sigma(P, c.ew, t) * dot;
+        const double vq = G.sigma * dot;
         J.out_on = vq > 0.0;
+        if (!(vq < 0.0)) return;
         const double fac = c.kfac * dot;
         for (int a = 0; a < 3; a++) w[a] -= fac * t[a];
         double Oo[3];
         for (int a = 0; a < 3; a++) Oo[a] = Rr[3 * a] * w[0] + Rr[3 * a + 1] * w[1] + Rr[3 * a + 2] * w[2];
-        if (!(vq < 0.0)) return;
-        int64_t cl[3];
+        int cl[3];
         double s[3], ds[3];
         for (int a = 0; a < 3; a++) {
-            const double cf = floor((Oo[a] - c.x0) / c.hx);
+            const double cf = floor((Oo[a] - c.x0) * c.ihx);
             if (!(cf >= -1.0 && cf <= (double)(N - 1))) return;  // beyond the zero-extended support
-            cl[a] = (int64_t)cf;
-            s[a] = (Oo[a] - (c.x0 + c.hx * (double)cl[a])) / c.hx;
+            cl[a] = (int)cf;
+            s[a] = (Oo[a] - (c.x0 + c.hx * (double)cl[a])) * c.ihx;
             // d Oo_a / d eps = deps_scale (w . t) (R^T t)_a
-            ds[a] = deps_scale * dot * (Rr[3 * a] * t[0] + Rr[3 * a + 1] * t[1] + Rr[3 * a + 2] * t[2]) / c.hx;
+            ds[a] = deps_scale * dot * (Rr[3 * a] * t[0] + Rr[3 * a + 1] * t[1] + Rr[3 * a + 2] * t[2]) * c.ihx;
         }
         J.in_on = true;
         for (int q = 0; q < 8; q++) {
             const int b0 = q & 1, b1 = (q >> 1) & 1, b2 = (q >> 2) & 1;
-            const int64_t j0 = cl[0] + b0, j1 = cl[1] + b1, j2 = cl[2] + b2;
+            const int j0 = cl[0] + b0, j1 = cl[1] + b1, j2 = cl[2] + b2;
             const bool in = j0 >= 0 && j0 < N && j1 >= 0 && j1 < N && j2 >= 0 && j2 < N;
             const double u0 = b0 ? s[0] : 1 - s[0], u1 = b1 ? s[1] : 1 - s[1], u2 = b2 ? s[2] : 1 - s[2];
             // corners outside the grid carry the zero extension: no weight, no derivative
@@ -257,21 +303,9 @@ __device__ __forceinline__ void jump_at(const double* Rr, const double* t, const
             J.dw[q] = in ? (b0 ? ds[0] : -ds[0]) * u1 * u2 + u0 * (b1 ? ds[1] : -ds[1]) * u2 +
                                u0 * u1 * (b2 ? ds[2] : -ds[2])
                          : 0.0;
-            J.q[q] = in ? j0 + N * (j1 + (int64_t)N * j2) : 0;
+            J.q[q] = in ? j0 + N * (j1 + N * j2) : 0;
         }
     }
-}
-
-__device__ __forceinline__ void load_geo(const double* geo, int numR, int k, double* Rr, double* t, double* P,
-                                         double& lam) {
-#pragma unroll
-    for (int a = 0; a < 9; a++) Rr[a] = geo[(int64_t)(G_R + a) * numR + k];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        t[a] = geo[(int64_t)(G_T + a) * numR + k];
-        P[a] = geo[(int64_t)(G_PC + a) * numR + k];
-    }
-    lam = geo[(int64_t)G_LAM * numR + k];
 }
 
 // Transpose of the jump term, scattered into acc (numR, nx), zeroed first:
@@ -283,17 +317,18 @@ __global__ __launch_bounds__(256) void k_reset_adj(const T* __restrict__ g, cons
                                                    const int32_t* __restrict__ indR, int numR, int64_t nR,
                                                    ResetConsts c, T* acc) {
     const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int64_t ix = blockIdx.y;
+    const int ix = (int)blockIdx.y;
     if (k >= numR) return;
-    double Rr[9], t[3], P[3], lam;
-    load_geo(geo, numR, k, Rr, t, P, lam);
+    const RowGeo G = row_geo(geo, numR, k, c);
+    OmegaIdx o;
+    o.set(ix, c.N);
     Jump<DIM> J;
-    jump_at<DIM>(Rr, t, P, c, ix, 0.0, J);
+    jump_at<DIM>(G, c, o.i0, o.i1, o.i2, 0.0, J);
     const int64_t iR = indR[k];
     const double gv = (double)g[iR + nR * ix];
-    if (J.out_on) atomicAdd(acc + k + (int64_t)numR * ix, (T)(-lam * gv));
+    if (J.out_on) atomicAdd(acc + k + (int64_t)numR * ix, (T)(-G.lam * gv));
     if (J.in_on) {
-        const double a = c.J * lam * gv;
+        const double a = c.J * G.lam * gv;
 #pragma unroll
         for (int q = 0; q < Jump<DIM>::NQ; q++) atomicAdd(acc + k + (int64_t)numR * J.q[q], (T)(a * J.w[q]));
     }
@@ -309,13 +344,15 @@ __global__ __launch_bounds__(256) void k_reset_deps(const T* __restrict__ f, con
                                                     double* __restrict__ part) {
     __shared__ double red[4];
     const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int64_t ix = blockIdx.y;
+    const int ix = (int)blockIdx.y;
     double v = 0.0;
     if (k < numR) {
-        double Rr[9], t[3], P[3], lam;
-        load_geo(geo, numR, k, Rr, t, P, lam);
+        const RowGeo G = row_geo(geo, numR, k, c);
+        const double lam = G.lam;
+        OmegaIdx o;
+        o.set(ix, c.N);
         Jump<DIM> J;
-        jump_at<DIM>(Rr, t, P, c, ix, -dk, J);
+        jump_at<DIM>(G, c, o.i0, o.i1, o.i2, -dk, J);
         if (J.in_on) {
             const int64_t iR = indR[k];
             double I = 0.0, dI = 0.0;
@@ -367,51 +404,40 @@ __device__ __forceinline__ int xcd_major_r(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
+// The lanes' split of a group: thread tid works on row r = tid % RG (RG a
+// power of two dividing the block) and the omega points x = tid / RG + k S,
+// S = blockDim.x / RG; neighbouring lanes hold neighbouring rows (the grid's
+// fastest index) at one point.
+struct RowLanes {
+    int r, x0, S;
+    OmegaIdx p, d;  // the first point's indices and the step's
+    __device__ __forceinline__ RowLanes(int RG, int N) {
+        r = (int)threadIdx.x & (RG - 1);
+        x0 = (int)threadIdx.x / RG;
+        S = (int)blockDim.x / RG;
+        p.set(x0, N);
+        d.set(S, N);
+    }
+};
+
 template <class T>
 __device__ __forceinline__ void stage_rows(const T* __restrict__ src, const int32_t* __restrict__ indR, int k0, int nr,
-                                           int64_t nR, int64_t nx, T* rows) {
-    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
-        const int r = (int)(e % nr);
-        const int64_t x = e / nr;
-        rows[(int64_t)r * nx + x] = src[indR[k0 + r] + nR * x];
-    }
+                                           int RG, int64_t nR, int nx, T* rows) {
+    const int r = (int)threadIdx.x & (RG - 1), S = (int)blockDim.x / RG;
+    if (r >= nr) return;
+    const T* s = src + indR[k0 + r];
+    T* row = rows + (int64_t)r * nx;
+    for (int x = (int)threadIdx.x / RG; x < nx; x += S) row[x] = s[nR * x];
 }
 
 __device__ __forceinline__ void stage_geo(const double* __restrict__ geo, int numR, int k0, int nr, double* sg) {
     for (int e = threadIdx.x; e < G_N * nr; e += blockDim.x) sg[e] = geo[(int64_t)(e / nr) * numR + k0 + e % nr];
 }
 
-// jump term at (row r of the group, omega point ix) from the staged rows
-template <class T, int DIM>
-__device__ __forceinline__ double jump_lds(const T* rows, const double* sg, int r, int nr, int64_t nx, int64_t ix,
-                                           const ResetConsts& c) {
-#pragma clang fp contract(off)
-    double Rr[9], t[3], P[3];
-#pragma unroll
-    for (int a = 0; a < 9; a++) Rr[a] = sg[(G_R + a) * nr + r];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        t[a] = sg[(G_T + a) * nr + r];
-        P[a] = sg[(G_PC + a) * nr + r];
-    }
-    const double lam = sg[G_LAM * nr + r];
-    const T* row = rows + (int64_t)r * nx;
-    Jump<DIM> J;
-    jump_at<DIM>(Rr, t, P, c, ix, 0.0, J);
-    double out = J.out_on ? -(double)row[ix] * lam : 0.0;
-    if (J.in_on) {
-        double v = 0.0;
-#pragma unroll
-        for (int q = 0; q < Jump<DIM>::NQ; q++) v += (double)row[J.q[q]] * J.w[q];
-        out = DIM == 2 ? v * lam + out : out + c.J * lam * v;
-    }
-    return out;
-}
-
 // out rows = (accumulate ? staged + scale df : scale df) on the group's rows
 template <class T, int DIM>
 __global__ __launch_bounds__(1024) void k_reset_rows(const T* f_in, T* f_out, const double* __restrict__ geo,
-                                                     const int32_t* __restrict__ indR, int numR, int64_t nR, int64_t nx,
+                                                     const int32_t* __restrict__ indR, int numR, int64_t nR, int nx,
                                                      int RG, ResetConsts c, double scale, int accumulate) {
     extern __shared__ double smem_rr[];
     double* sg = smem_rr;                       // G_N x RG
@@ -419,14 +445,25 @@ __global__ __launch_bounds__(1024) void k_reset_rows(const T* f_in, T* f_out, co
     const int k0 = xcd_major_r(blockIdx.x, gridDim.x) * RG;
     const int nr = min(RG, numR - k0);
     stage_geo(geo, numR, k0, nr, sg);
-    stage_rows<T>(f_in, indR, k0, nr, nR, nx, rows);
+    stage_rows<T>(f_in, indR, k0, nr, RG, nR, nx, rows);
     __syncthreads();
-    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
-        const int r = (int)(e % nr);
-        const int64_t x = e / nr;
-        const double d = jump_lds<T, DIM>(rows, sg, r, nr, nx, x, c);
-        const double base = accumulate ? (double)rows[(int64_t)r * nx + x] : 0.0;
-        f_out[indR[k0 + r] + nR * x] = (T)(base + scale * d);
+    RowLanes L(RG, c.N);
+    if (L.r >= nr) return;
+    const RowGeo G = row_geo(sg, nr, L.r, c);
+    const T* row = rows + (int64_t)L.r * nx;
+    T* out = f_out + indR[k0 + L.r];
+    for (int x = L.x0; x < nx; x += L.S, L.p.advance(L.d, c.N)) {
+        Jump<DIM> J;
+        jump_at<DIM>(G, c, L.p.i0, L.p.i1, L.p.i2, 0.0, J);
+        double d = J.out_on ? -(double)row[x] * G.lam : 0.0;
+        if (J.in_on) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < Jump<DIM>::NQ; q++) v += (double)row[J.q[q]] * J.w[q];
+            d = DIM == 2 ? v * G.lam + d : d + c.J * G.lam * v;
+        }
+        const double base = accumulate ? (double)row[x] : 0.0;
+        out[nR * x] = (T)(base + scale * d);
     }
 }
 
@@ -435,7 +472,7 @@ __global__ __launch_bounds__(1024) void k_reset_rows(const T* f_in, T* f_out, co
 template <class T, int DIM>
 __global__ __launch_bounds__(1024) void k_reset_adj_rows(const T* g_in, T* g_out, const double* __restrict__ geo,
                                                          const int32_t* __restrict__ indR, int numR, int64_t nR,
-                                                         int64_t nx, int RG, ResetConsts c, double scale) {
+                                                         int nx, int RG, ResetConsts c, double scale) {
     extern __shared__ double smem_ra[];
     double* sg = smem_ra;
     T* rows = (T*)(smem_ra + G_N * RG);
@@ -443,37 +480,34 @@ __global__ __launch_bounds__(1024) void k_reset_adj_rows(const T* g_in, T* g_out
     const int k0 = xcd_major_r(blockIdx.x, gridDim.x) * RG;
     const int nr = min(RG, numR - k0);
     stage_geo(geo, numR, k0, nr, sg);
-    stage_rows<T>(g_in, indR, k0, nr, nR, nx, rows);
-    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) acc[e] = T(0);
+    stage_rows<T>(g_in, indR, k0, nr, RG, nR, nx, rows);
+    for (int e = threadIdx.x; e < nr * nx; e += blockDim.x) acc[e] = T(0);
     __syncthreads();
-    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
-        const int r = (int)(e % nr);
-        const int64_t x = e / nr;
-        double Rr[9], t[3], P[3];
-        for (int a = 0; a < 9; a++) Rr[a] = sg[(G_R + a) * nr + r];
-        for (int a = 0; a < 3; a++) {
-            t[a] = sg[(G_T + a) * nr + r];
-            P[a] = sg[(G_PC + a) * nr + r];
-        }
-        const double lam = sg[G_LAM * nr + r];
-        Jump<DIM> J;
-        jump_at<DIM>(Rr, t, P, c, x, 0.0, J);
-        const double gv = (double)rows[(int64_t)r * nx + x];
-        T* ar = acc + (int64_t)r * nx;
-        if (J.out_on) atomicAdd(ar + x, (T)(-lam * gv));
-        if (J.in_on) {
-            const double a = c.J * lam * gv;
+    RowLanes L(RG, c.N);
+    const bool act = L.r < nr;
+    if (act) {
+        const RowGeo G = row_geo(sg, nr, L.r, c);
+        const T* row = rows + (int64_t)L.r * nx;
+        T* ar = acc + (int64_t)L.r * nx;
+        for (int x = L.x0; x < nx; x += L.S, L.p.advance(L.d, c.N)) {
+            Jump<DIM> J;
+            jump_at<DIM>(G, c, L.p.i0, L.p.i1, L.p.i2, 0.0, J);
+            const double gv = (double)row[x];
+            if (J.out_on) atomicAdd(ar + x, (T)(-G.lam * gv));
+            if (J.in_on) {
+                const double a = c.J * G.lam * gv;
 #pragma unroll
-            for (int q = 0; q < Jump<DIM>::NQ; q++)
-                if (J.w[q] != 0.0) atomicAdd(ar + J.q[q], (T)(a * J.w[q]));
+                for (int q = 0; q < Jump<DIM>::NQ; q++)
+                    if (J.w[q] != 0.0) atomicAdd(ar + J.q[q], (T)(a * J.w[q]));
+            }
         }
     }
     __syncthreads();
-    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
-        const int r = (int)(e % nr);
-        const int64_t x = e / nr;
-        g_out[indR[k0 + r] + nR * x] = (T)((double)rows[(int64_t)r * nx + x] + scale * (double)acc[(int64_t)r * nx + x]);
-    }
+    if (!act) return;
+    const T* row = rows + (int64_t)L.r * nx;
+    const T* ar = acc + (int64_t)L.r * nx;
+    T* out = g_out + indR[k0 + L.r];
+    for (int x = L.x0; x < nx; x += L.S) out[nR * x] = (T)((double)row[x] + scale * (double)ar[x]);
 }
 
 // <g, (dR/d eps) f> over the group's rows, f staged, g read in place; the
@@ -482,7 +516,7 @@ template <class T, int DIM>
 __global__ __launch_bounds__(1024) void k_reset_deps_rows(const T* __restrict__ f, const T* __restrict__ g,
                                                           const double* __restrict__ geo,
                                                           const int32_t* __restrict__ indR, int numR, int64_t nR,
-                                                          int64_t nx, int RG, ResetConsts c, double dJ, double dk,
+                                                          int nx, int RG, ResetConsts c, double dJ, double dk,
                                                           double* __restrict__ part) {
     extern __shared__ double smem_rd[];
     __shared__ double red[16];
@@ -492,31 +526,27 @@ __global__ __launch_bounds__(1024) void k_reset_deps_rows(const T* __restrict__ 
     const int k0 = grp * RG;
     const int nr = min(RG, numR - k0);
     stage_geo(geo, numR, k0, nr, sg);
-    stage_rows<T>(f, indR, k0, nr, nR, nx, rows);
+    stage_rows<T>(f, indR, k0, nr, RG, nR, nx, rows);
     __syncthreads();
+    RowLanes L(RG, c.N);
     double v = 0.0;
-    for (int64_t e = threadIdx.x; e < (int64_t)nr * nx; e += blockDim.x) {
-        const int r = (int)(e % nr);
-        const int64_t x = e / nr;
-        double Rr[9], t[3], P[3];
-        for (int a = 0; a < 9; a++) Rr[a] = sg[(G_R + a) * nr + r];
-        for (int a = 0; a < 3; a++) {
-            t[a] = sg[(G_T + a) * nr + r];
-            P[a] = sg[(G_PC + a) * nr + r];
-        }
-        const double lam = sg[G_LAM * nr + r];
-        Jump<DIM> J;
-        jump_at<DIM>(Rr, t, P, c, x, -dk, J);
-        if (J.in_on) {
-            const T* row = rows + (int64_t)r * nx;
-            double I = 0.0, dI = 0.0;
+    if (L.r < nr) {
+        const RowGeo G = row_geo(sg, nr, L.r, c);
+        const T* row = rows + (int64_t)L.r * nx;
+        const T* gr = g + indR[k0 + L.r];
+        for (int x = L.x0; x < nx; x += L.S, L.p.advance(L.d, c.N)) {
+            Jump<DIM> J;
+            jump_at<DIM>(G, c, L.p.i0, L.p.i1, L.p.i2, -dk, J);
+            if (J.in_on) {
+                double I = 0.0, dI = 0.0;
 #pragma unroll
-            for (int q = 0; q < Jump<DIM>::NQ; q++) {
-                const double fq = (double)row[J.q[q]];
-                I += J.w[q] * fq;
-                dI += J.dw[q] * fq;
+                for (int q = 0; q < Jump<DIM>::NQ; q++) {
+                    const double fq = (double)row[J.q[q]];
+                    I += J.w[q] * fq;
+                    dI += J.dw[q] * fq;
+                }
+                v += (double)gr[nR * x] * G.lam * (dJ * I + c.J * dI);
             }
-            v += (double)g[indR[k0 + r] + nR * x] * lam * (dJ * I + c.J * dI);
         }
     }
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -558,6 +588,7 @@ void ResetGeom::build(int B, int dim_, int N_, double L, const hpup_reset_params
     for (int i = 0; i < kResetMaxN; i++) c.xg[i] = i < N ? -L / 2 + L / N * (double)i : 0.0;
     c.x0 = -L / 2;
     c.hx = L / N;
+    c.ihx = 1.0 / c.hx;
     if (dim == 2) {
         // getOmega's box (the gateway's xlo/xhi) and getIndRule's L and dx^2
         const double dx = c.xg[1] - c.xg[0];
@@ -628,8 +659,8 @@ void ResetGeom::build(int B, int dim_, int N_, double L, const hpup_reset_params
 // per omega point into 128 KiB, at most 8; 0 if not even one row fits
 static int rows_per_group(int64_t nx, size_t per) {
     const int64_t budget = 128 * 1024 - G_N * 8 * 8;
-    const int64_t rg = budget / (nx * (int64_t)per);
-    return (int)std::min<int64_t>(rg, 8);
+    const int64_t rg = std::min<int64_t>(budget / (nx * (int64_t)per), 8);
+    return rg >= 8 ? 8 : rg >= 4 ? 4 : rg >= 2 ? 2 : (int)rg;  // a power of two (the kernels' lane split), or 0
 }
 
 bool ResetGeom::rows_fit(size_t per) const { return rows_per_group(nx, per) >= 1; }
@@ -652,11 +683,11 @@ bool ResetGeom::rows_pass(const void* fin, void* fout, double scale, bool accumu
     if (dim == 2) {
         allow_lds((const void*)k_reset_rows<T, 2>, sh);
         hipLaunchKernelGGL((k_reset_rows<T, 2>), dim3(ng), dim3(1024), sh, st, (const T*)fin, (T*)fout,
-                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale, accumulate ? 1 : 0);
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, (int)nx, RG, consts, scale, accumulate ? 1 : 0);
     } else {
         allow_lds((const void*)k_reset_rows<T, 3>, sh);
         hipLaunchKernelGGL((k_reset_rows<T, 3>), dim3(ng), dim3(1024), sh, st, (const T*)fin, (T*)fout,
-                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale, accumulate ? 1 : 0);
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, (int)nx, RG, consts, scale, accumulate ? 1 : 0);
     }
     HPUP_HIP(hipGetLastError());
     return true;
@@ -672,11 +703,11 @@ bool ResetGeom::adjoint_rows(const void* g, void* out, double scale, hipStream_t
     if (dim == 2) {
         allow_lds((const void*)k_reset_adj_rows<T, 2>, sh);
         hipLaunchKernelGGL((k_reset_adj_rows<T, 2>), dim3(ng), dim3(1024), sh, st, (const T*)g, (T*)out,
-                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale);
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, (int)nx, RG, consts, scale);
     } else {
         allow_lds((const void*)k_reset_adj_rows<T, 3>, sh);
         hipLaunchKernelGGL((k_reset_adj_rows<T, 3>), dim3(ng), dim3(1024), sh, st, (const T*)g, (T*)out,
-                           geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, scale);
+                           geo.as<double>(), indR.as<int32_t>(), numR, nR, (int)nx, RG, consts, scale);
     }
     HPUP_HIP(hipGetLastError());
     return true;
@@ -731,11 +762,11 @@ void ResetGeom::eps_grad(const void* f, const void* g, double* part, double scal
         if (dim == 2) {
             allow_lds((const void*)k_reset_deps_rows<T, 2>, sh);
             hipLaunchKernelGGL((k_reset_deps_rows<T, 2>), dim3(ng), dim3(1024), sh, st, (const T*)f, (const T*)g,
-                               geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, dJ, dk, part);
+                               geo.as<double>(), indR.as<int32_t>(), numR, nR, (int)nx, RG, consts, dJ, dk, part);
         } else {
             allow_lds((const void*)k_reset_deps_rows<T, 3>, sh);
             hipLaunchKernelGGL((k_reset_deps_rows<T, 3>), dim3(ng), dim3(1024), sh, st, (const T*)f, (const T*)g,
-                               geo.as<double>(), indR.as<int32_t>(), numR, nR, nx, RG, consts, dJ, dk, part);
+                               geo.as<double>(), indR.as<int32_t>(), numR, nR, (int)nx, RG, consts, dJ, dk, part);
         }
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, (const double*)part, (int64_t)ng, scale, out);
         HPUP_HIP(hipGetLastError());

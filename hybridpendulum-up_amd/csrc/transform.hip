@@ -29,6 +29,26 @@ __device__ __forceinline__ int64_t g_off(int B, int mn, int j, int64_t x) {
     return ((x / kGX) * (M * M * n2) + (int64_t)mn * n2 + j) * kGX + (x % kGX);
 }
 
+// The forward (alpha, gamma) stages fill the half of G that the beta stage
+// reads: the rows mn >= (M^2 - 1)/2, i.e. n > 0, or n = 0 and m >= 0.  The
+// grid is real, so G[-m][-n] = conj G[m][n]: a value computed at (m >= 0, n)
+// goes to (m, n) for n >= 0 and, conjugated, to (-m, -n) for n < 0 and
+// m > 0 (at m = 0 the n < 0 half is the conjugate of the n > 0 half).
+// Gx: G at (mn = 0, j, x); ni = n + lmax.
+template <class T>
+__device__ __forceinline__ void put_g_half(cT<T>* Gx, int B, int m, int ni, cT<T> z) {
+    const int lmax = B - 1, M = 2 * B - 1;
+    const int64_t rs = (int64_t)2 * B * kGX;
+    if (ni >= lmax) Gx[((int64_t)(m + lmax) + M * ni) * rs] = z;
+    else if (m > 0) Gx[((int64_t)(lmax - m) + M * (M - 1 - ni)) * rs] = cconj<T>(z);
+}
+
+// (-1)^d conj z: the forward coefficient at (-m, -n) from the one at (m, n),
+// d = m - n (before the omega DFT, at the same omega point)
+template <class T> __device__ __forceinline__ cT<T> mirror_coef(cT<T> z, int d) {
+    return (d & 1) ? cmk<T>(-z.x, z.y) : cmk<T>(z.x, -z.y);
+}
+
 // ---------------------------------------------------------- forward 2-D --
 
 template <class T>
@@ -83,8 +103,7 @@ __global__ __launch_bounds__(256) void k_fwd_ag(XformDev<T> X, const T* __restri
             t += step;
             if (t >= n2) t -= n2;
         }
-        G[g_off(B, (m + lmax) + M * (n + lmax), j, x)] = acc;
-        if (m > 0) G[g_off(B, (-m + lmax) + M * (-n + lmax), j, x)] = cconj<T>(acc);
+        put_g_half<T>(G + g_off(B, 0, j, x), B, m, n + lmax, acc);
     }
 }
 
@@ -93,7 +112,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
     extern __shared__ char smem_raw[];
     T* sW = reinterpret_cast<T*>(smem_raw);
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
-    const int mn = blockIdx.y;
+    const int mn = (M * M - 1) / 2 + blockIdx.y;  // the half of G the forward stages fill
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
     for (int i = threadIdx.x; i < B * n2; i += blockDim.x) sW[i] = X.Wf[(size_t)mn * B * n2 + i];
@@ -119,6 +138,8 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
         for (int t = 0; t < 8; t++) {
             if (l0 + t > lmax) break;
             F[d_ridx(B, l0 + t, m, n) * X.nx + x] = acc[t];
+            // F^l_{-m,-n} = (-1)^(m-n) conj F^l_mn (a real grid; d^l_{-m,-n} = (-1)^(m-n) d^l_mn)
+            if (mn != (M * M - 1) / 2) F[d_ridx(B, l0 + t, -m, -n) * X.nx + x] = mirror_coef<T>(acc[t], m - n);
         }
     }
 }
@@ -136,7 +157,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
-    const int mn = blockIdx.y;
+    const int mn = (M * M - 1) / 2 + blockIdx.y;  // the half of G the forward stages fill
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
@@ -173,7 +194,11 @@ __global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int l = 16 * rt + MF::orow(lq, v);
-            if (l >= lmin && l < B) F[d_ridx(B, l, m, n) * nx + x] = cmk<T>(re[rt][v], im[rt][v]);
+            if (l >= lmin && l < B) {
+                const cT<T> z = cmk<T>(re[rt][v], im[rt][v]);
+                F[d_ridx(B, l, m, n) * nx + x] = z;
+                if (mn != (M * M - 1) / 2) F[d_ridx(B, l, -m, -n) * nx + x] = mirror_coef<T>(z, m - n);
+            }
         }
 }
 
@@ -442,6 +467,27 @@ __device__ __forceinline__ int xcd_major(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
+// linear omega index e -> (i0, i1, i2) of an N^3 grid, and its advance by a
+// fixed step decomposed the same way (no division in the loops)
+struct OmegaStep3 {
+    int i0, i1, i2;
+    __device__ __forceinline__ void set(int e, int N) {
+        i0 = e % N;
+        const int q = e / N;
+        i1 = q % N;
+        i2 = q / N;
+    }
+    __device__ __forceinline__ void advance(const OmegaStep3& d, int N) {
+        i0 += d.i0;
+        const int c0 = i0 >= N ? 1 : 0;
+        i0 -= c0 * N;
+        i1 += d.i1 + c0;
+        const int c1 = i1 >= N ? 1 : 0;
+        i1 -= c1 * N;
+        i2 += d.i2 + c1;
+    }
+};
+
 // tasks of a stage -> parts of the block: <= 64 -> 4, <= 128 -> 2, else 1
 __device__ __forceinline__ int stage_parts(int T) { return T <= 64 ? 4 : (T <= 128 ? 2 : 1); }
 
@@ -509,10 +555,7 @@ __global__ __launch_bounds__(256) void k_fwd_ag_r(XformDev<T> X, const T* __rest
             cT<T> acc = czero<T>();
 #pragma unroll
             for (int g = 0; g < kAgMax; g++) cfma<T>(acc, h[g], etw<T>(e, g));
-            if (st) {
-                Gx[((int64_t)(m + lmax) + M * ni) * n2 * kGX] = acc;
-                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * kGX] = cconj<T>(acc);
-            }
+            if (st) put_g_half<T>(Gx, B, m, ni, acc);
         }
     }
 }
@@ -772,11 +815,7 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_fft(XformDev<T> X, const T* __res
         if (x < nx) {
             cT<T>* Gx = G + g_off(B, 0, j, x);
 #pragma unroll
-            for (int ni = 0; ni < M; ni++) {
-                const cT<T> z = v[(ni - lmax + n2) % n2];
-                Gx[((int64_t)(m + lmax) + M * ni) * n2 * kGX] = z;
-                if (m > 0) Gx[((int64_t)(-m + lmax) + M * (M - 1 - ni)) * n2 * kGX] = cconj<T>(z);
-            }
+            for (int ni = 0; ni < M; ni++) put_g_half<T>(Gx, B, m, ni, v[(ni - lmax + n2) % n2]);
         }
     }
 }
@@ -923,9 +962,7 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_q60(XformDev<T> X, const T* __res
         for (int k = 0; k < 15; k++) {
             const int K = k + 15 * P;  // frequency n = K or K - 60; the Nyquist bin K = 30 is dropped
             if (K == 30) continue;
-            const int ni = K <= lmax ? K + lmax : K - 31;
-            Gx[((int64_t)(m + lmax) + M * ni) * n2 * kGX] = y[k];
-            if (m > 0) Gx[((int64_t)(lmax - m) + M * (M - 1 - ni)) * n2 * kGX] = cconj<T>(y[k]);
+            put_g_half<T>(Gx, B, m, K <= lmax ? K + lmax : K - 31, y[k]);
         }
     }
 }
@@ -1051,6 +1088,23 @@ static void inv_ag_fft(const XformDev<T>& X, const void* G, void* f, hipStream_t
                        (const cT<T>*)G, (T*)f, nxc);
 }
 
+// omega points per block of the quad-FFT kernels (their blocks hold one
+// quad lane per thread: 120 XC forward, 240 XC inverse)
+#ifndef HPUP_Q60_FWD_XC32
+#define HPUP_Q60_FWD_XC32 4
+#endif
+#ifndef HPUP_Q60_FWD_XC64
+#define HPUP_Q60_FWD_XC64 2
+#endif
+#ifndef HPUP_Q60_INV_XC32
+#define HPUP_Q60_INV_XC32 2
+#endif
+#ifndef HPUP_Q60_INV_XC64
+#define HPUP_Q60_INV_XC64 1
+#endif
+constexpr int kQ60FwdXC32 = HPUP_Q60_FWD_XC32, kQ60FwdXC64 = HPUP_Q60_FWD_XC64, kQ60InvXC32 = HPUP_Q60_INV_XC32,
+              kQ60InvXC64 = HPUP_Q60_INV_XC64;
+
 template <class T, int XC, int NT>
 static void fwd_ag_q60(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st) {
     constexpr int n2 = 60;
@@ -1080,9 +1134,11 @@ void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstr
         fwd_ag_fft<T, 15, kFftFwdXC, kFftNT>(X, f, wR, wstride, G, st);
         return;
     }
-    if (n2 == 60) {  // quad FFTs: two (FP64) or four (FP32) omega points per block
-        if constexpr (sizeof(T) == 8) fwd_ag_q60<T, 2, 256>(X, f, wR, wstride, G, st);
-        else fwd_ag_q60<T, 4, 512>(X, f, wR, wstride, G, st);
+    if (n2 == 60) {  // quad FFTs, kQ60* omega points per block
+        if constexpr (sizeof(T) == 8)
+            fwd_ag_q60<T, kQ60FwdXC64, (120 * kQ60FwdXC64 + 63) / 64 * 64>(X, f, wR, wstride, G, st);
+        else
+            fwd_ag_q60<T, kQ60FwdXC32, (120 * kQ60FwdXC32 + 63) / 64 * 64>(X, f, wR, wstride, G, st);
         return;
     }
     if (n2 <= kAgMax) {
@@ -1108,7 +1164,7 @@ void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstr
 template <class T, int RT, int KS>
 static void beta_fwd_mfma(const XformDev<T>& X, const void* G, void* F, hipStream_t st) {
     const int M = 2 * X.B - 1;
-    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)(M * M));
+    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)((M * M + 1) / 2));  // half of (m, n), mirrored
     hipLaunchKernelGGL((k_beta_fwd_mfma<T, RT, KS>), grid, dim3(256), 0, st, X, (const cT<T>*)G, (cT<T>*)F);
 }
 template <class T, int RT, int KS>
@@ -1125,7 +1181,7 @@ void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t s
     if (X.B <= 16) return beta_fwd_mfma<T, 1, 8>(X, G, F, st);    // l < 16, j < 32
     if (X.B <= 32) return beta_fwd_mfma<T, 2, 16>(X, G, F, st);   // l < 32, j < 64
     const int M = 2 * X.B - 1;
-    dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)(M * M));
+    dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)((M * M + 1) / 2));  // half of (m, n), mirrored
     hipLaunchKernelGGL(k_beta_fwd<T>, grid, dim3(256), sizeof(T) * X.B * 2 * X.B, st, X, (const cT<T>*)G,
                        (cT<T>*)F);
 }
@@ -1148,9 +1204,9 @@ void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st)
         inv_ag_fft<T, 15, kFftInvXC * (sizeof(T) == 8 ? 1 : 2), kFftNT>(X, G, f, st);
         return;
     }
-    if (n2 == 60) {  // quad FFTs: one (FP64) or two (FP32) omega points per block
-        if constexpr (sizeof(T) == 8) inv_ag_q60<T, 1, 256>(X, G, f, st);
-        else inv_ag_q60<T, 2, 512>(X, G, f, st);
+    if (n2 == 60) {  // quad FFTs, kQ60* omega points per block
+        if constexpr (sizeof(T) == 8) inv_ag_q60<T, kQ60InvXC64, (240 * kQ60InvXC64 + 63) / 64 * 64>(X, G, f, st);
+        else inv_ag_q60<T, kQ60InvXC32, (240 * kQ60InvXC32 + 63) / 64 * 64>(X, G, f, st);
         return;
     }
     if (n2 <= kAgMax) {
@@ -1175,12 +1231,104 @@ void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st)
     hipLaunchKernelGGL((k_inv_ag<T, GC>), grid, dim3(256), sh, st, X, (const cT<T>*)G, (T*)f, XC);
 }
 
+// Omega DFT of whole compact rows, 3-D grids of at most kRowDftMaxN points
+// per axis: one workgroup per rotation row loads the row's N^3 coefficients
+// into LDS (layout k1 + (N + 1) k2 + N (N + 1) k3: the padding spreads the
+// axis-1 lines of neighbouring threads over the banks), transforms the three
+// axes in turn with direct length-N DFTs (one line per thread, held in
+// registers, written back in place), and stores the row times `scale` -- one
+// pass over the row instead of the slab and axis passes.  tw[t] =
+// e^{sign 2 pi i t / N}; in and out may alias.
+constexpr int kRowDftMaxN = 22;
+template <class T>
+__global__ __launch_bounds__(512) void k_omega_dft_row(const cT<T>* in, cT<T>* out, const cT<T>* __restrict__ tw,
+                                                       int N, T scale) {
+    extern __shared__ char smem_od[];
+    cT<T>* tws = reinterpret_cast<cT<T>*>(smem_od);
+    cT<T>* row = tws + kRowDftMaxN;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int S2 = N + 1, S3 = N * (N + 1), NN = N * N, nx = NN * N;
+    const int64_t r = blockIdx.x;
+    const cT<T>* src = in + r * nx;
+    cT<T>* dst = out + r * nx;
+    if (tid < N) tws[tid] = tw[tid];
+    // element e = k1 + N k2 + NN k3 -> its padded LDS slot; e advances by nt
+    OmegaStep3 st0, stp;
+    st0.set(tid, N);
+    stp.set(nt, N);
+    {
+        OmegaStep3 p = st0;
+        for (int e = tid; e < nx; e += nt, p.advance(stp, N)) row[p.i0 + S2 * p.i1 + S3 * p.i2] = src[e];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int axis = 0; axis < 3; axis++) {
+        const int stride = axis == 0 ? 1 : (axis == 1 ? S2 : S3);
+        for (int l = tid; l < NN; l += nt) {
+            const int i = l % N, j = l / N;  // the other two indices, fastest first
+            const int base = axis == 0 ? S2 * i + S3 * j : (axis == 1 ? i + S3 * j : i + S2 * j);
+            cT<T> x[kRowDftMaxN];
+#pragma unroll
+            for (int a = 0; a < kRowDftMaxN; a++) x[a] = a < N ? row[base + stride * a] : czero<T>();
+#pragma unroll 1
+            for (int k = 0; k < N; k++) {
+                cT<T> acc = czero<T>();
+                int idx = 0;  // (a k) mod N
+#pragma unroll
+                for (int a = 0; a < kRowDftMaxN; a++) {
+                    if (a < N) {
+                        cfma<T>(acc, x[a], tws[idx]);
+                        idx += k;
+                        idx -= idx >= N ? N : 0;
+                    }
+                }
+                row[base + stride * k] = acc;
+            }
+        }
+        __syncthreads();
+    }
+    OmegaStep3 p = st0;
+    for (int e = tid; e < nx; e += nt, p.advance(stp, N)) dst[e] = cscale<T>(row[p.i0 + S2 * p.i1 + S3 * p.i2], scale);
+}
+
+template <class T> static size_t row_dft_lds(int N) {
+    return sizeof(cT<T>) * ((size_t)kRowDftMaxN + (size_t)(N + 1) * N * N);
+}
+// whether the one-pass row kernel serves this grid (3-D, N <= 22, the row in LDS)
+template <class T> static bool row_dft_fits(const XformDev<T>& X) {
+    return X.dim == 3 && X.N <= kRowDftMaxN && row_dft_lds<T>(X.N) <= lds_limit() && X.nrc < (1ll << 31);
+}
+template <class T> static void launch_omega_dft_row(const XformDev<T>& X, const void* in, void* out, bool inverse,
+                                                    hipStream_t st) {
+    const int N = X.N, NN = N * N;
+    const size_t sh = row_dft_lds<T>(N);
+    allow_lds((const void*)k_omega_dft_row<T>, sh);
+    const int nt = std::min(512, (NN + 63) / 64 * 64);
+    hipLaunchKernelGGL(k_omega_dft_row<T>, dim3((unsigned)X.nrc), dim3(nt), sh, st, (const cT<T>*)in, (cT<T>*)out,
+                       inverse ? X.twNi : X.twNf, N, inverse ? T(1) / T(NN * N) : T(1));
+}
+
+template <class T>
+void launch_omega_dft_to(const XformDev<T>& X, const void* in, void* out, bool inverse, hipStream_t st) {
+    if (row_dft_fits<T>(X)) {
+        launch_omega_dft_row<T>(X, in, out, inverse, st);
+        return;
+    }
+    if (in != out)
+        HPUP_HIP(hipMemcpyAsync(out, in, sizeof(cT<T>) * (size_t)(X.nrc * X.nx), hipMemcpyDeviceToDevice, st));
+    launch_omega_dft<T>(X, out, nullptr, inverse, st);
+}
+
 template <class T>
 void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hipStream_t st) {
     (void)tmp;
     if (X.dim == 0) return;
     const int N = X.N, NN = N * N;
     HPUP_REQUIRE(NN <= 4096, HPUP_ERR_UNSUPPORTED, "omega grid larger than 64 per axis");
+    if (row_dft_fits<T>(X)) {
+        launch_omega_dft_row<T>(X, F, F, inverse, st);
+        return;
+    }
     if (N <= 32 && X.bsF) {  // chirp-Toeplitz products on MFMA
         const cT<T>* bs = inverse ? X.bsI : X.bsF;
         const int64_t nslab = X.nrc * (X.dim == 3 ? N : 1);
@@ -1224,6 +1372,7 @@ void launch_compact_to_full(int B, int64_t nx, const void* compact, void* full, 
     template void launch_beta_inv<T>(const XformDev<T>&, const void*, void*, hipStream_t);           \
     template void launch_inv_ag<T>(const XformDev<T>&, const void*, void*, hipStream_t);             \
     template void launch_omega_dft<T>(const XformDev<T>&, void*, void*, bool, hipStream_t);          \
+    template void launch_omega_dft_to<T>(const XformDev<T>&, const void*, void*, bool, hipStream_t); \
     template void launch_full_to_compact<T>(int, int64_t, const void*, void*, hipStream_t);          \
     template void launch_compact_to_full<T>(int, int64_t, const void*, void*, hipStream_t);
 HPUP_INST(double)
