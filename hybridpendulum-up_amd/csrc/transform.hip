@@ -49,6 +49,18 @@ template <class T> __device__ __forceinline__ cT<T> mirror_coef(cT<T> z, int d) 
     return (d & 1) ? cmk<T>(-z.x, z.y) : cmk<T>(z.x, -z.y);
 }
 
+// The inverse transform's real output depends on G only through its
+// Hermitian part, H[m][n] = (G[m][n] + conj G[-m][-n]) / 2 at each (beta_j,
+// omega point): Re sum_mn G[m][n] e^{-i(m a + n g)} = sum_mn H[m][n] e^{-i(m a + n g)}.
+// The inverse beta stage therefore writes H on the same half as the forward
+// (rows mn >= (M^2 - 1)/2), and the (alpha, gamma) stages read the other half
+// as conjugates.  Value at (mi, ni); Gx: G at (mn = 0, j, x).
+template <class T> __device__ __forceinline__ cT<T> get_g_half(const cT<T>* Gx, int B, int mi, int ni) {
+    const int M = 2 * B - 1, mn = mi + M * ni;
+    const int64_t rs = (int64_t)2 * B * kGX;
+    return mn >= (M * M - 1) / 2 ? Gx[mn * rs] : cconj<T>(Gx[(int64_t)(M * M - 1 - mn) * rs]);
+}
+
 // ---------------------------------------------------------- forward 2-D --
 
 template <class T>
@@ -210,7 +222,7 @@ __global__ __launch_bounds__(256) void k_beta_inv_mfma(XformDev<T> X, const cT<T
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
-    const int mn = blockIdx.y;
+    const int mn = (M * M - 1) / 2 + blockIdx.y;  // the stored half of H
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
@@ -218,13 +230,16 @@ __global__ __launch_bounds__(256) void k_beta_inv_mfma(XformDev<T> X, const cT<T
     const int64_t x = (int64_t)blockIdx.x * kGX + 16 * w + lr;
     const bool xok = x < nx;
     const int64_t xc = xok ? x : nx - 1;
+    // (F^l_mn + (-1)^(m-n) conj F^l_{-m,-n}) / 2: with Wi[-m,-n] = (-1)^(m-n) Wi[m,n]
+    // its contraction is H[m][n]
     cT<T> f[KS];
 #pragma unroll
     for (int s = 0; s < KS; s++) {
         const int l = 4 * s + lq;
         const bool lok = l >= lmin && l < B;
         const cT<T> v = F[d_ridx(B, lok ? l : lmax, m, n) * nx + xc];
-        f[s] = (lok && xok) ? v : czero<T>();
+        const cT<T> u = mirror_coef<T>(F[d_ridx(B, lok ? l : lmax, -m, -n) * nx + xc], m - n);
+        f[s] = (lok && xok) ? cmk<T>(T(0.5) * (v.x + u.x), T(0.5) * (v.y + u.y)) : czero<T>();
     }
     const T* W = X.Wi + (size_t)mn * n2 * B;
     acc_t re[RT], im[RT];
@@ -259,7 +274,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
     extern __shared__ char smem_raw[];
     T* sW = reinterpret_cast<T*>(smem_raw);
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
-    const int mn = blockIdx.y;
+    const int mn = (M * M - 1) / 2 + blockIdx.y;  // the stored half of H (see get_g_half)
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
     for (int i = threadIdx.x; i < B * n2; i += blockDim.x) sW[i] = X.Wi[(size_t)mn * B * n2 + i];
@@ -272,7 +287,9 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
 #pragma unroll
         for (int t = 0; t < 8; t++) acc[t] = czero<T>();
         for (int l = lmin; l <= lmax; l++) {
-            const cT<T> fv = F[d_ridx(B, l, m, n) * X.nx + x];
+            const cT<T> v = F[d_ridx(B, l, m, n) * X.nx + x];
+            const cT<T> u = mirror_coef<T>(F[d_ridx(B, l, -m, -n) * X.nx + x], m - n);
+            const cT<T> fv = cmk<T>(T(0.5) * (v.x + u.x), T(0.5) * (v.y + u.y));
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 if (j0 + t >= n2) break;
@@ -315,9 +332,9 @@ __global__ __launch_bounds__(256) void k_inv_ag(XformDev<T> X, const cT<T>* __re
 #pragma unroll
         for (int k = 0; k < GC; k++) acc[k] = czero<T>();
         if (x < nx) {
-            const cT<T>* Gp = G + g_off(B, mi, j, x);
+            const cT<T>* Gx = G + g_off(B, 0, j, x);
             for (int ni = 0; ni < M; ni++) {
-                const cT<T> gv = Gp[(int64_t)M * ni * n2 * kGX];
+                const cT<T> gv = get_g_half<T>(Gx, B, mi, ni);
                 const int n = ni - lmax;
                 const int dn = ((n % n2) + n2) % n2;  // (n g) mod n2 advances by dn per gamma
                 int t = (int)(((int64_t)dn * g0) % n2);
@@ -581,11 +598,11 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
         const int64_t x = x0 + xl;
         const bool ld = act && x < nx;
         // clamped, branch-free loads; invalid entries are zeroed after the load
-        const cT<T>* Gp = G + g_off(B, min(mi, M - 1), j, min<int64_t>(x, nx - 1));
+        const cT<T>* Gx = G + g_off(B, 0, j, min<int64_t>(x, nx - 1));
         cT<T> gr[kAgMax];
 #pragma unroll
         for (int ni = 0; ni < kAgMax; ni++) {
-            const cT<T> v = Gp[(int64_t)M * min(ni, M - 1) * n2 * kGX];
+            const cT<T> v = get_g_half<T>(Gx, B, min(mi, M - 1), min(ni, M - 1));
             gr[ni] = (ld && ni < M) ? v : czero<T>();
         }
         for (int g = part; g < n2; g += S1) {
@@ -725,28 +742,30 @@ template <class T, int B, int XC, int NT>
 __global__ __launch_bounds__(NT) void k_inv_ag_fft(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
                                                    int nxc) {
     extern __shared__ char smem_raw[];
-    constexpr int n2 = 2 * B, M = 2 * B - 1, SH = fft_sh<T>(XC, n2);
-    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl], row stride SH
+    constexpr int n2 = 2 * B, M = 2 * B - 1, lmax = B - 1, SH = fft_sh<T>(XC, n2);
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m >= 0][g][xl], row stride SH
     const int w = xcd_major(blockIdx.x, gridDim.x);
     const int j = w / nxc;
     const int64_t x0 = (int64_t)(w % nxc) * XC;
     const int64_t nx = X.nx;
-    for (int task = threadIdx.x; task < M * XC; task += NT) {
-        const int mi = task / XC, xl = task % XC;
+    // the Hermitian part of G (get_g_half): H[-m][g] = conj H[m][g], so only
+    // the rows m >= 0 are transformed over n
+    for (int task = threadIdx.x; task < B * XC; task += NT) {
+        const int m = task / XC, xl = task % XC;
         const int64_t x = x0 + xl;
         const bool ld = x < nx;
         // clamped, branch-free loads; a column beyond nx is zeroed after the load
-        const cT<T>* Gp = G + g_off(B, mi, j, min<int64_t>(x, nx - 1));
+        const cT<T>* Gx = G + g_off(B, 0, j, min<int64_t>(x, nx - 1));
         cT<T> v[n2];
 #pragma unroll
         for (int ni = 0; ni < M; ni++) {
-            const cT<T> g = Gp[(int64_t)M * ni * n2 * kGX];
-            v[(ni - (B - 1) + n2) % n2] = ld ? g : czero<T>();
+            const cT<T> g = get_g_half<T>(Gx, B, lmax + m, ni);
+            v[(ni - lmax + n2) % n2] = ld ? g : czero<T>();
         }
         v[B] = czero<T>();
         fftn<T, -1, n2>(v);
 #pragma unroll
-        for (int g = 0; g < n2; g++) sH[mi * SH + g * XC + xl] = v[g];
+        for (int g = 0; g < n2; g++) sH[m * SH + g * XC + xl] = v[g];
     }
     __syncthreads();
     for (int task = threadIdx.x; task < n2 * XC; task += NT) {
@@ -754,7 +773,11 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft(XformDev<T> X, const cT<T>* _
         const int64_t x = x0 + xl;
         cT<T> v[n2];
 #pragma unroll
-        for (int mi = 0; mi < M; mi++) v[(mi - (B - 1) + n2) % n2] = sH[mi * SH + g * XC + xl];
+        for (int mi = 0; mi < M; mi++) {
+            const int m = mi - lmax;
+            const cT<T> h = sH[(m >= 0 ? m : -m) * SH + g * XC + xl];
+            v[(mi - lmax + n2) % n2] = m >= 0 ? h : cconj<T>(h);
+        }
         v[B] = czero<T>();
         fftn<T, -1, n2>(v);
         if (x < nx) {
@@ -976,12 +999,12 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_q60(XformDev<T> X, const T* __res
 template <class T, int XC, int NT>
 __global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* __restrict__ G, T* __restrict__ f,
                                                    int nxc) {
-    constexpr int B = 30, n2 = 60, M = 59, HS = n2;
-    constexpr int T1 = XC * M * 4, T2 = XC * n2 * 4;
+    constexpr int B = 30, n2 = 60, M = 59, lmax = 29, HS = n2;
+    constexpr int T1 = XC * B * 4, T2 = XC * n2 * 4;
     static_assert(T1 <= NT && T2 <= NT, "one quad lane per thread");
     extern __shared__ char smem_raw[];
     cT<T>* tw = reinterpret_cast<cT<T>*>(smem_raw);
-    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw + 64 * sizeof(cT<T>));  // [xl][mi][g]
+    cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw + 64 * sizeof(cT<T>));  // [xl][m >= 0][g]
     T* so = reinterpret_cast<T*>(smem_raw + 64 * sizeof(cT<T>));          // [xl][g][a], aliases sH
     const int w = xcd_major(blockIdx.x, gridDim.x);
     const int j = w / nxc;
@@ -991,26 +1014,28 @@ __global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* _
     if (tid < n2) tw[tid] = X.twB[tid];
     const int q = tid & 3, P = rev2(q);
     cT<T> y[15];
+    // stage 1 on the rows m >= 0 of the Hermitian part of G (get_g_half); the
+    // rows m < 0 of H are the conjugates
     const bool act1 = tid < T1;
     {
         const int pr = act1 ? tid >> 2 : 0;
-        const int xl = pr % XC, mi = pr / XC;
+        const int xl = pr % XC, m = pr / XC;
         const int64_t x = x0 + xl;
         const bool ld = x < nx;
         // clamped loads; a column beyond nx is zeroed after the load
-        const cT<T>* Gp = G + g_off(B, mi, j, min<int64_t>(x, nx - 1));
+        const cT<T>* Gx = G + g_off(B, 0, j, min<int64_t>(x, nx - 1));
 #pragma unroll
         for (int t = 0; t < 15; t++) {
             const int ni = (4 * t + q + 29) % n2;  // 59 at the Nyquist bin K = 30
-            const cT<T> g = Gp[(int64_t)M * (ni < M ? ni : 0) * n2 * kGX];
+            const cT<T> g = get_g_half<T>(Gx, B, lmax + m, ni < M ? ni : 0);
             y[t] = (ld && ni < M) ? g : czero<T>();
         }
     }
     __syncthreads();  // twiddles
     fft60_quad<T, -1>(y, q, tw);
     if (act1) {
-        const int pr = tid >> 2, xl = pr % XC, mi = pr / XC;
-        cT<T>* hr = sH + (xl * M + mi) * HS + 15 * P;
+        const int pr = tid >> 2, xl = pr % XC, m = pr / XC;
+        cT<T>* hr = sH + (xl * B + m) * HS + 15 * P;
 #pragma unroll
         for (int k = 0; k < 15; k++) hr[k] = y[k];
     }
@@ -1020,9 +1045,10 @@ __global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* _
     const int xl2 = pr2 % XC, g2 = pr2 / XC;
 #pragma unroll
     for (int t = 0; t < 15; t++) {
-        const int mi = (4 * t + q + 29) % n2;
-        const cT<T> h = sH[(xl2 * M + (mi < M ? mi : 0)) * HS + g2];
-        y[t] = mi < M ? h : czero<T>();
+        const int mi = (4 * t + q + 29) % n2;  // 59 at the Nyquist bin
+        const int m = mi - lmax;
+        const cT<T> h = sH[(xl2 * B + (m >= 0 ? (mi < M ? m : 0) : -m)) * HS + g2];
+        y[t] = mi < M ? (m >= 0 ? h : cconj<T>(h)) : czero<T>();
     }
     fft60_quad<T, -1>(y, q, tw);
     __syncthreads();  // so overwrites sH
@@ -1082,7 +1108,7 @@ static void inv_ag_fft(const XformDev<T>& X, const void* G, void* f, hipStream_t
     constexpr int n2 = 2 * B, M = 2 * B - 1;
     const int nxc = (int)((X.nx + XC - 1) / XC);
     HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
-    const size_t sh = sizeof(cT<T>) * M * fft_sh<T>(XC, n2);
+    const size_t sh = sizeof(cT<T>) * B * fft_sh<T>(XC, n2);  // rows m >= 0
     allow_lds((const void*)k_inv_ag_fft<T, B, XC, NT>, sh);
     hipLaunchKernelGGL((k_inv_ag_fft<T, B, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X,
                        (const cT<T>*)G, (T*)f, nxc);
@@ -1118,10 +1144,10 @@ static void fwd_ag_q60(const XformDev<T>& X, const void* f, const void* wR, int 
 
 template <class T, int XC, int NT>
 static void inv_ag_q60(const XformDev<T>& X, const void* G, void* f, hipStream_t st) {
-    constexpr int n2 = 60, M = 59;
+    constexpr int n2 = 60, B = 30;
     const int nxc = (int)((X.nx + XC - 1) / XC);
     HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
-    const size_t sh = 64 * sizeof(cT<T>) + std::max(sizeof(cT<T>) * XC * M * n2, sizeof(T) * XC * n2 * n2);
+    const size_t sh = 64 * sizeof(cT<T>) + std::max(sizeof(cT<T>) * XC * B * n2, sizeof(T) * XC * n2 * n2);
     allow_lds((const void*)k_inv_ag_q60<T, XC, NT>, sh);
     hipLaunchKernelGGL((k_inv_ag_q60<T, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X,
                        (const cT<T>*)G, (T*)f, nxc);
@@ -1170,7 +1196,7 @@ static void beta_fwd_mfma(const XformDev<T>& X, const void* G, void* F, hipStrea
 template <class T, int RT, int KS>
 static void beta_inv_mfma(const XformDev<T>& X, const void* F, void* G, hipStream_t st) {
     const int M = 2 * X.B - 1;
-    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)(M * M));
+    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)((M * M + 1) / 2));  // the stored half of H
     hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS>), grid, dim3(256), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
 }
 
@@ -1192,7 +1218,7 @@ void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t s
     if (X.B <= 16) return beta_inv_mfma<T, 2, 4>(X, F, G, st);    // j < 32, l < 16
     if (X.B <= 32) return beta_inv_mfma<T, 4, 8>(X, F, G, st);    // j < 64, l < 32
     const int M = 2 * X.B - 1;
-    dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)(M * M));
+    dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)((M * M + 1) / 2));  // the stored half of H
     hipLaunchKernelGGL(k_beta_inv<T>, grid, dim3(256), sizeof(T) * X.B * 2 * X.B, st, X, (const cT<T>*)F,
                        (cT<T>*)G);
 }
