@@ -24,9 +24,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "hybridpendulum-up_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# MI355X dense MFMA peaks, TFLOP/s.  A 3xBF16 product costs three bf16 MFMAs,
-# so its FP32-equivalent peak is a third of the bf16 one.
-MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0 / 3}
+# MI355X dense MFMA peaks, TFLOP/s.  A 3xBF16 product (three-piece splits,
+# the six products of order <= 2) costs six bf16 MFMAs, so its FP32-equivalent
+# peak is a sixth of the bf16 one.
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0 / 6}
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (scaled by the copy-probe factors) for the kernels of this workload; the

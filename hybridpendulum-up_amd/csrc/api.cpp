@@ -119,6 +119,7 @@ template <class T> struct DriftTables {
     DevBuf cg_src, cg_coef, u_src, u_coef, e3, kap, kc, kd, M3, mix_src, mix_coef;
     DriftDev<T> view{};
     MixDev<T> mix{};
+    bool smix = false;
     // Merges the Clebsch-Gordan and u_1/u_2 term lists of each row by source row.
     void build_mix(const DriftSpec& s) {
         const int64_t nrc = nrc_of(s.B);
@@ -166,6 +167,26 @@ template <class T> struct DriftTables {
                 src[r * TS + t] = rows[r][t].first | (fl << 24);
                 std::copy(c.begin(), c.end(), coef.begin() + (r * TS + t) * NC * 2);
             }
+        // Structured mixing: in the pendulum models -i CG_1 is real, -i CG_2
+        // imaginary, CG_3 zero (a diagonal inertia), u_1 imaginary and u_2 real
+        // (to rounding of the transforms that produce them); the kernels then
+        // multiply by the one nonzero component
+        {
+            double mx[5] = {0, 0, 0, 0, 0}, off[5] = {0, 0, 0, 0, 0};
+            const int part[5] = {1, 0, -1, 0, 1};  // the component that must vanish (-1: both)
+            for (size_t e = 0; e < coef.size() / 2; e++) {
+                const int k = (int)(e % NC), cls = k < s.dim ? k : 3 + (k - s.dim);
+                const double re = coef[2 * e], im = coef[2 * e + 1];
+                mx[cls] = std::max(mx[cls], std::hypot(re, im));
+                const double z = part[cls] == 0 ? std::abs(re) : part[cls] == 1 ? std::abs(im) : std::hypot(re, im);
+                off[cls] = std::max(off[cls], z);
+            }
+            double gmax = 0.0;
+            for (int c = 0; c < 5; c++) gmax = std::max(gmax, mx[c]);
+            smix = true;
+            for (int c = 0; c < 5; c++)
+                if (off[c] > 1e-13 * gmax) smix = false;
+        }
         mix_src = upload(src);
         mix_coef = upload(cast_vec<T>(coef));
         mix.TS = TS;
@@ -212,6 +233,7 @@ template <class T> struct DriftTables {
         for (int i = 0; i < 9; i++) v.G[i] = (T)s.G[i];
         for (int i = 0; i < 3; i++) { v.b[i] = (T)s.b[i]; v.gyro[i] = (T)s.gyro[i]; v.c0[i] = (T)s.c0[i]; }
         v.ramp = s.ramp ? 1 : 0;
+        v.smix = smix ? 1 : 0;
         v.adj = s.adj ? 1 : 0;
         v.has_gyro = s.has_gyro ? 1 : 0;
         v.grav[0] = v.grav[1] = v.grav[2] = nullptr;

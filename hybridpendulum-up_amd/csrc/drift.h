@@ -36,6 +36,9 @@ template <class T> struct DriftDev {
     // every kap_p = c0[p] + i (real circulant): the real-circulant path
     int ramp;
     T c0[3];
+    // structured mixing coefficients: -i CG_1 real, -i CG_2 imaginary,
+    // CG_3 zero, u_1 imaginary, u_2 real (the pendulum models)
+    int smix;
     // the adjoint operator (transposed tables, damping through the V planes)
     int adj;
     // 3-D adjoint: T3^H [(e3 - i b3 kc(k3)) F] from the scaled axis-3 pass
@@ -89,7 +92,8 @@ void launch_axis3_mfma(const DriftDev<T>& D, const void* in, void* out, bool bf1
 // Toeplitz kernel c_d at [d + 32] for d in [-32, 32) and then the chirp p_k
 // (zero for k >= N); see k_slab_dft_mfma.
 template <class T>
-void launch_slab_dft_mfma(void* F, int64_t nslab, int N, const void* bs, T scale, hipStream_t st);
+void launch_slab_dft_mfma(const void* in, void* out, int64_t nslab, int N, const void* bs, T scale,
+                          hipStream_t st);
 template <class T>
 void launch_axis_dft_mfma(const void* in, void* out, int N, int64_t inner, int64_t nlines, const void* bs, T scale,
                           hipStream_t st);
@@ -144,8 +148,7 @@ void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st)
 // in-place omega DFT of the compact rows (forward sign -1, inverse +1 with 1/nx)
 template <class T>
 void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hipStream_t st);
-// the same from in to out (out = in allowed): one pass where the row kernel
-// serves the grid, else a copy and the in-place transform
+// the same from in to out (out = in allowed)
 template <class T>
 void launch_omega_dft_to(const XformDev<T>& X, const void* in, void* out, bool inverse, hipStream_t st);
 

@@ -363,6 +363,39 @@ template <class T> __device__ __forceinline__ cT<T> kc_cplx(kc_ptr<T> p, int64_t
     return cmk<T>(p[2 * i], p[2 * i + 1]);
 }
 
+// One mixing source with structured coefficients: W += (a + i b_v) F with
+// a = Re(-i CG_1) kc1 and b_v = Im(-i CG_2) kc2 (CG_3 = 0), V1 += i Im(u_1) F,
+// V2 += Re(u_2) F -- nine FP operations per element instead of sixteen.
+template <class T, int DIM>
+__device__ __forceinline__ void mix_structured(kc_ptr<T> coefT, int64_t cb, int fl, T kc1, const T* kc2v,
+                                               const cT<T>* f, cT<T>* Wv, cT<T>* V1, cT<T>* V2) {
+    if (fl & 1) {
+        const T a = coefT[2 * cb] * kc1, g1 = coefT[2 * (cb + 1) + 1];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const T b = g1 * kc2v[v];
+            Wv[v].x = fma(a, f[v].x, fma(-b, f[v].y, Wv[v].x));
+            Wv[v].y = fma(a, f[v].y, fma(b, f[v].x, Wv[v].y));
+        }
+    }
+    if (fl & 2) {
+        const T ui = coefT[2 * (cb + DIM) + 1];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            V1[v].x = fma(-ui, f[v].y, V1[v].x);
+            V1[v].y = fma(ui, f[v].x, V1[v].y);
+        }
+    }
+    if (fl & 4) {
+        const T ur = coefT[2 * (cb + DIM + 1)];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            V2[v].x = fma(ur, f[v].x, V2[v].x);
+            V2[v].y = fma(ur, f[v].y, V2[v].y);
+        }
+    }
+}
+
 // Waves per SIMD the fused kernel is compiled for (the register budget):
 // 4 for FP32 (at most 128 VGPRs), 2 for FP64 and 3xBF16.
 template <class T, class G> struct FusedWaves { static constexpr int n = 2; };
@@ -381,7 +414,9 @@ template <bool R> struct FusedWaves<float, GemmF<float, R>> { static constexpr i
 // Re<T_p^H(-i K_p ybar), y>, formed in the GEMM pair the adjoint leaves free
 // (planes A and D hold -i K_1 ybar and -i K_2 ybar), the diffusion ones
 // elementwise; each workgroup writes its five sums to D.gpart.
-template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false>
+// SMIX (forward only): structured mixing coefficients (DriftDev::smix), each
+// multiplied by its one nonzero component.
+template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false, bool SMIX = false>
 __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
                                                      const cT<T>* __restrict__ Z3, const cT<T>* F0, cT<T>* Fout,
                                                      cT<T>* ACC, StageCoef<T> sc, int64_t nslab) {
@@ -528,6 +563,10 @@ __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<
                 if (src[u] < 0) continue;
                 const int fl = src[u] >> 24;
                 const int64_t cb = (int64_t)(t0 + u) * NC;
+                if constexpr (SMIX) {
+                    mix_structured<T, DIM>(coefT, cb, fl, kc1, kc2v, fv[u], Wv, V1, V2);
+                    continue;
+                }
                 if (fl & 1) {
                     const cT<T> g0 = kc_cplx<T>(coefT, cb), g1 = kc_cplx<T>(coefT, cb + 1),
                                 g2 = DIM == 3 ? kc_cplx<T>(coefT, cb + 2) : czero<T>();
@@ -753,7 +792,7 @@ template <class T> struct Fused64 {
     static constexpr size_t lds = 2 * sizeof(CircTab<T, 64>) + (size_t)2 * pairs * plane * sizeof(T);
 };
 
-template <class T, bool RAMP, bool GRAV>
+template <class T, bool RAMP, bool GRAV, bool SMIX>
 __global__ __launch_bounds__(kFused64Threads, 1) void k_drift_fused64(DriftDev<T> D, MixDev<T> MX,
                                                                       const cT<T>* Fin, const cT<T>* F0,
                                                                       cT<T>* Fout, cT<T>* ACC, StageCoef<T> sc,
@@ -864,6 +903,10 @@ __global__ __launch_bounds__(kFused64Threads, 1) void k_drift_fused64(DriftDev<T
                 if (src[u] < 0) continue;
                 const int fl = src[u] >> 24;
                 const int64_t cb = (int64_t)(t0 + u) * NC;
+                if constexpr (SMIX) {
+                    mix_structured<T, 2>(coefT, cb, fl, kc1, kc2v, fv[u], Wv, V1, V2);
+                    continue;
+                }
                 if (fl & 1) {
                     const cT<T> g0 = kc_cplx<T>(coefT, cb), g1 = kc_cplx<T>(coefT, cb + 1);
                     const cT<T> e = cscale<T>(g0, kc1);
@@ -1186,10 +1229,10 @@ template <class K> static unsigned persistent_grid(K kernel, int64_t nslab, int 
     return (unsigned)g;
 }
 
-template <class T, bool RAMP, bool GRAV>
+template <class T, bool RAMP, bool GRAV, bool SMIX>
 static void launch_fused64_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* F0, void* Fout,
                              void* ACC, const StageCoef<T>& sc, int64_t nslab, hipStream_t st) {
-    auto kern = k_drift_fused64<T, RAMP, GRAV>;
+    auto kern = k_drift_fused64<T, RAMP, GRAV, SMIX>;
     const size_t sh = Fused64<T>::lds;
     allow_lds((const void*)kern, sh);
     const unsigned grid = persistent_grid(kern, nslab, kFused64Threads, sh);
@@ -1197,17 +1240,27 @@ static void launch_fused64_t(const DriftDev<T>& D, const MixDev<T>& MX, const vo
                        (cT<T>*)Fout, (cT<T>*)ACC, sc, (int)nslab);
 }
 
-template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false>
+template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false, bool SMIX = false>
 static unsigned launch_fused_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3,
                                const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab,
                                hipStream_t st) {
-    auto kern = k_drift_fused<T, G, DIM, GRAV, ADJ, GRAD>;
+    auto kern = k_drift_fused<T, G, DIM, GRAV, ADJ, GRAD, SMIX>;
     const unsigned grid = persistent_grid(kern, nslab);
     // the gradient variant writes five partials per workgroup
     HPUP_REQUIRE(!GRAD || grid <= (unsigned)kGradMaxWG, HPUP_ERR_UNSUPPORTED, "gradient stage: partial buffer too small");
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, D, MX, (const cT<T>*)Fin, (const cT<T>*)Z3,
                        (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
     return grid;
+}
+
+// forward operator: the structured-mixing instantiation where the tables allow it
+template <class T, class G, int DIM, bool GRAV>
+static void launch_fwd_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
+                         void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab, hipStream_t st) {
+    if (D.smix)
+        launch_fused_t<T, G, DIM, GRAV, false, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+    else
+        launch_fused_t<T, G, DIM, GRAV>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
 }
 
 template <class T>
@@ -1222,12 +1275,15 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     if (D.N > 32) {
         HPUP_REQUIRE(!bf16x3, HPUP_ERR_UNSUPPORTED, "3xBF16 drift needs N <= 32");
         HPUP_REQUIRE(!D.adj, HPUP_ERR_UNSUPPORTED, "the adjoint operator needs N <= 32");
-        if (D.ramp) {
-            if (grav) launch_fused64_t<T, true, true>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
-            else launch_fused64_t<T, true, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+        if (D.ramp && D.smix) {
+            if (grav) launch_fused64_t<T, true, true, true>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+            else launch_fused64_t<T, true, false, true>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+        } else if (D.ramp) {
+            if (grav) launch_fused64_t<T, true, true, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+            else launch_fused64_t<T, true, false, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
         } else {
-            if (grav) launch_fused64_t<T, false, true>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
-            else launch_fused64_t<T, false, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+            if (grav) launch_fused64_t<T, false, true, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
+            else launch_fused64_t<T, false, false, false>(D, MX, Fin, F0, Fout, ACC, sc, nslab, st);
         }
         return;
     }
@@ -1236,9 +1292,9 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
             HPUP_REQUIRE(!grav, HPUP_ERR_UNSUPPORTED, "3xBF16 drift has no density-form gravity");
             HPUP_REQUIRE(!D.adj, HPUP_ERR_UNSUPPORTED, "the adjoint operator has no 3xBF16 form");
             if (D.dim == 3)
-                launch_fused_t<float, GemmBf16x3, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+                launch_fwd_t<float, GemmBf16x3, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
             else
-                launch_fused_t<float, GemmBf16x3, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+                launch_fwd_t<float, GemmBf16x3, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
             return;
         }
     }
@@ -1267,11 +1323,11 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
     }
     if (D.ramp) {
         if (D.dim == 3)
-            launch_fused_t<T, GemmF<T, true>, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+            launch_fwd_t<T, GemmF<T, true>, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
         else if (grav)
-            launch_fused_t<T, GemmF<T, true>, 2, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+            launch_fwd_t<T, GemmF<T, true>, 2, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
         else
-            launch_fused_t<T, GemmF<T, true>, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+            launch_fwd_t<T, GemmF<T, true>, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
         return;
     }
     if (D.dim == 3)
@@ -1327,10 +1383,11 @@ __device__ __forceinline__ void fill_toeplitz(CircTab<T>& t, const cT<T>* bs, in
     }
 }
 
-// 2-D DFT over (k1, k2) of every N x N slab, in place, times `scale`.
+// 2-D DFT over (k1, k2) of every N x N slab of in, times `scale`, into out
+// (in place when they are equal: a slab is read whole before it is written).
 template <class T>
-__global__ __launch_bounds__(256) void k_slab_dft_mfma(cT<T>* F, const cT<T>* __restrict__ bs, int N, T scale,
-                                                       int nslab) {
+__global__ __launch_bounds__(256) void k_slab_dft_mfma(const cT<T>* in, cT<T>* out, const cT<T>* __restrict__ bs,
+                                                       int N, T scale, int nslab) {
     __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad];
     __shared__ CircTab<T> tab;
     __shared__ cT<T> pk[32];
@@ -1345,14 +1402,15 @@ __global__ __launch_bounds__(256) void k_slab_dft_mfma(cT<T>* F, const cT<T>* __
     const int k1 = 16 * k1t + lr;
     const cT<T> p1 = pk[k1];
     for (int sl = blockIdx.x; sl < nslab; sl += gridDim.x) {
-        cT<T>* Fb = F + (int64_t)sl * NN;
+        const cT<T>* Fb = in + (int64_t)sl * NN;
+        cT<T>* Fo = out + (int64_t)sl * NN;
         cT<T> x[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {  // clamped in-range loads, zero outside
             const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-            const bool in = a2 < N && a1 < N;
-            const cT<T> v = Fb[in ? a2 * N + a1 : 0];
-            x[i] = in ? cmul<T>(v, pk[a1]) : czero<T>();
+            const bool ok = a2 < N && a1 < N;
+            const cT<T> v = Fb[ok ? a2 * N + a1 : 0];
+            x[i] = ok ? cmul<T>(v, pk[a1]) : czero<T>();
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -1375,7 +1433,7 @@ __global__ __launch_bounds__(256) void k_slab_dft_mfma(cT<T>* F, const cT<T>* __
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int k2 = 16 * k2t + GemmF<T, false>::orow(lq, v);
-            if (k1 < N && k2 < N) Fb[k2 * N + k1] = cscale<T>(cmul<T>(R[v], pk[k2]), scale);
+            if (k1 < N && k2 < N) Fo[k2 * N + k1] = cscale<T>(cmul<T>(R[v], pk[k2]), scale);
         }
     }
 }
@@ -1445,11 +1503,13 @@ __global__ __launch_bounds__(256) void k_axis_dft_mfma(const cT<T>* in, cT<T>* o
 }
 
 template <class T>
-void launch_slab_dft_mfma(void* F, int64_t nslab, int N, const void* bs, T scale, hipStream_t st) {
+void launch_slab_dft_mfma(const void* in, void* out, int64_t nslab, int N, const void* bs, T scale,
+                          hipStream_t st) {
     HPUP_REQUIRE(N <= 32, HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT needs N <= 32");
     HPUP_REQUIRE(nslab < ((int64_t)1 << 31), HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT: too many slabs");
     const unsigned grid = (unsigned)std::min<int64_t>(nslab, 4096);
-    hipLaunchKernelGGL(k_slab_dft_mfma<T>, dim3(grid), dim3(256), 0, st, (cT<T>*)F, (const cT<T>*)bs, N, scale,
+    hipLaunchKernelGGL(k_slab_dft_mfma<T>, dim3(grid), dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out,
+                       (const cT<T>*)bs, N, scale,
                        (int)nslab);
 }
 
@@ -1462,8 +1522,8 @@ void launch_axis_dft_mfma(const void* in, void* out, int N, int64_t inner, int64
                        (const cT<T>*)bs, N, inner, scale);
 }
 
-template void launch_slab_dft_mfma<float>(void*, int64_t, int, const void*, float, hipStream_t);
-template void launch_slab_dft_mfma<double>(void*, int64_t, int, const void*, double, hipStream_t);
+template void launch_slab_dft_mfma<float>(const void*, void*, int64_t, int, const void*, float, hipStream_t);
+template void launch_slab_dft_mfma<double>(const void*, void*, int64_t, int, const void*, double, hipStream_t);
 template void launch_axis_dft_mfma<float>(const void*, void*, int, int64_t, int64_t, const void*, float, hipStream_t);
 template void launch_axis_dft_mfma<double>(const void*, void*, int, int64_t, int64_t, const void*, double,
                                            hipStream_t);

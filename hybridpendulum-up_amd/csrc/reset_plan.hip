@@ -420,14 +420,30 @@ struct RowLanes {
     }
 };
 
+// Rows of the group into LDS: a thread's loads are issued in batches of 8
+// before their LDS stores, so the staging costs one memory round trip per
+// batch instead of one per element.
 template <class T>
 __device__ __forceinline__ void stage_rows(const T* __restrict__ src, const int32_t* __restrict__ indR, int k0, int nr,
                                            int RG, int64_t nR, int nx, T* rows) {
+    constexpr int BT = 8;
     const int r = (int)threadIdx.x & (RG - 1), S = (int)blockDim.x / RG;
     if (r >= nr) return;
     const T* s = src + indR[k0 + r];
     T* row = rows + (int64_t)r * nx;
-    for (int x = (int)threadIdx.x / RG; x < nx; x += S) row[x] = s[nR * x];
+    for (int x0 = (int)threadIdx.x / RG; x0 < nx; x0 += BT * S) {
+        T v[BT];
+#pragma unroll
+        for (int u = 0; u < BT; u++) {
+            const int x = x0 + u * S;
+            v[u] = s[nR * (x < nx ? x : x0)];
+        }
+#pragma unroll
+        for (int u = 0; u < BT; u++) {
+            const int x = x0 + u * S;
+            if (x < nx) row[x] = v[u];
+        }
+    }
 }
 
 __device__ __forceinline__ void stage_geo(const double* __restrict__ geo, int numR, int k0, int nr, double* sg) {
@@ -534,18 +550,30 @@ __global__ __launch_bounds__(1024) void k_reset_deps_rows(const T* __restrict__ 
         const RowGeo G = row_geo(sg, nr, L.r, c);
         const T* row = rows + (int64_t)L.r * nx;
         const T* gr = g + indR[k0 + L.r];
-        for (int x = L.x0; x < nx; x += L.S, L.p.advance(L.d, c.N)) {
-            Jump<DIM> J;
-            jump_at<DIM>(G, c, L.p.i0, L.p.i1, L.p.i2, -dk, J);
-            if (J.in_on) {
-                double I = 0.0, dI = 0.0;
+        constexpr int BT = 4;  // g values of a batch of points loaded together
+        for (int xb = L.x0; xb < nx; xb += BT * L.S) {
+            T gv[BT];
 #pragma unroll
-                for (int q = 0; q < Jump<DIM>::NQ; q++) {
-                    const double fq = (double)row[J.q[q]];
-                    I += J.w[q] * fq;
-                    dI += J.dw[q] * fq;
+            for (int u = 0; u < BT; u++) {
+                const int x = xb + u * L.S;
+                gv[u] = gr[nR * (x < nx ? x : xb)];
+            }
+#pragma unroll
+            for (int u = 0; u < BT; u++) {
+                if (xb + u * L.S >= nx) break;
+                Jump<DIM> J;
+                jump_at<DIM>(G, c, L.p.i0, L.p.i1, L.p.i2, -dk, J);
+                if (J.in_on) {
+                    double I = 0.0, dI = 0.0;
+#pragma unroll
+                    for (int q = 0; q < Jump<DIM>::NQ; q++) {
+                        const double fq = (double)row[J.q[q]];
+                        I += J.w[q] * fq;
+                        dI += J.dw[q] * fq;
+                    }
+                    v += (double)gv[u] * G.lam * (dJ * I + c.J * dI);
                 }
-                v += (double)gr[nR * x] * G.lam * (dJ * I + c.J * dI);
+                L.p.advance(L.d, c.N);
             }
         }
     }

@@ -484,27 +484,6 @@ __device__ __forceinline__ int xcd_major(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-// linear omega index e -> (i0, i1, i2) of an N^3 grid, and its advance by a
-// fixed step decomposed the same way (no division in the loops)
-struct OmegaStep3 {
-    int i0, i1, i2;
-    __device__ __forceinline__ void set(int e, int N) {
-        i0 = e % N;
-        const int q = e / N;
-        i1 = q % N;
-        i2 = q / N;
-    }
-    __device__ __forceinline__ void advance(const OmegaStep3& d, int N) {
-        i0 += d.i0;
-        const int c0 = i0 >= N ? 1 : 0;
-        i0 -= c0 * N;
-        i1 += d.i1 + c0;
-        const int c1 = i1 >= N ? 1 : 0;
-        i1 -= c1 * N;
-        i2 += d.i2 + c1;
-    }
-};
-
 // tasks of a stage -> parts of the block: <= 64 -> 4, <= 128 -> 2, else 1
 __device__ __forceinline__ int stage_parts(int T) { return T <= 64 ? 4 : (T <= 128 ? 2 : 1); }
 
@@ -635,6 +614,39 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
                 acc = fma(hr[mi].x, tw.x, fma(hr[mi].y, tw.y, acc));
             }
             if (st) fo[a] = acc;
+        }
+    }
+}
+
+// The f tile of a forward (alpha, gamma) block, [xl][g][a] at row stride rp:
+// f(alpha_a, beta_j, gamma_g, x0 + xl) (times the grid weight wR when given),
+// zero beyond nx.  Each thread issues all of its loads before the first LDS
+// store, so the tile costs one memory round trip instead of one per element
+// the thread stages.
+template <class T, int XC, int NT, int n2>
+__device__ __forceinline__ void stage_f_tile(const T* __restrict__ f, const T* __restrict__ wR, int wstride, int j,
+                                             int64_t x0, int64_t nx, T* sf, int rp) {
+    constexpr int TOT = XC * n2 * n2, NL = (TOT + NT - 1) / NT;
+    const int64_t n2c = (int64_t)n2 * n2 * n2;
+    const int tid = threadIdx.x;
+    T v[NL];
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+        const int i = tid + u * NT;
+        const int ic = i < TOT ? i : TOT - 1;
+        const int xl = ic / (n2 * n2), g = (ic / n2) % n2, a = ic % n2;
+        const int64_t x = x0 + xl;
+        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
+        const T fv = f[iR + n2c * (x < nx ? x : nx - 1)];
+        v[u] = (i < TOT && x < nx) ? fv : T(0);
+        if (wR) v[u] *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
+    }
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+        const int i = tid + u * NT;
+        if (i < TOT) {
+            const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
+            sf[(xl * n2 + g) * rp + a] = v[u];
         }
     }
 }
@@ -803,15 +815,7 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_fft(XformDev<T> X, const T* __res
     const int64_t x0 = (int64_t)(w % nxc) * XC;
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
-    const int64_t n2c = (int64_t)n2 * n2 * n2;
-    for (int i = tid; i < XC * n2 * n2; i += NT) {
-        const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
-        const int64_t x = x0 + xl;
-        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
-        T v = x < nx ? f[iR + n2c * x] : T(0);
-        if (wR) v *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
-        sf[(xl * n2 + g) * rp + a] = v;
-    }
+    stage_f_tile<T, XC, NT, n2>(f, wR, wstride, j, x0, nx, sf, rp);
     __syncthreads();
     constexpr int TA = n2 * XC;
     static_assert(TA <= NT, "one alpha row per thread");
@@ -936,16 +940,8 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_q60(XformDev<T> X, const T* __res
     const int64_t x0 = (int64_t)(w % nxc) * XC;
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
-    const int64_t n2c = (int64_t)n2 * n2 * n2;
     if (tid < n2) tw[tid] = X.twB[tid];
-    for (int i = tid; i < XC * n2 * n2; i += NT) {
-        const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
-        const int64_t x = x0 + xl;
-        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
-        T v = x < nx ? f[iR + n2c * x] : T(0);
-        if (wR) v *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
-        sf[(xl * n2 + g) * rp + a] = v;
-    }
+    stage_f_tile<T, XC, NT, n2>(f, wR, wstride, j, x0, nx, sf, rp);
     __syncthreads();
     // quads beyond TQ run on clamped in-range data and store nothing
     const int q = tid & 3, P = rev2(q);
@@ -1257,87 +1253,22 @@ void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st)
     hipLaunchKernelGGL((k_inv_ag<T, GC>), grid, dim3(256), sh, st, X, (const cT<T>*)G, (T*)f, XC);
 }
 
-// Omega DFT of whole compact rows, 3-D grids of at most kRowDftMaxN points
-// per axis: one workgroup per rotation row loads the row's N^3 coefficients
-// into LDS (layout k1 + (N + 1) k2 + N (N + 1) k3: the padding spreads the
-// axis-1 lines of neighbouring threads over the banks), transforms the three
-// axes in turn with direct length-N DFTs (one line per thread, held in
-// registers, written back in place), and stores the row times `scale` -- one
-// pass over the row instead of the slab and axis passes.  tw[t] =
-// e^{sign 2 pi i t / N}; in and out may alias.
-constexpr int kRowDftMaxN = 22;
-template <class T>
-__global__ __launch_bounds__(512) void k_omega_dft_row(const cT<T>* in, cT<T>* out, const cT<T>* __restrict__ tw,
-                                                       int N, T scale) {
-    extern __shared__ char smem_od[];
-    cT<T>* tws = reinterpret_cast<cT<T>*>(smem_od);
-    cT<T>* row = tws + kRowDftMaxN;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int S2 = N + 1, S3 = N * (N + 1), NN = N * N, nx = NN * N;
-    const int64_t r = blockIdx.x;
-    const cT<T>* src = in + r * nx;
-    cT<T>* dst = out + r * nx;
-    if (tid < N) tws[tid] = tw[tid];
-    // element e = k1 + N k2 + NN k3 -> its padded LDS slot; e advances by nt
-    OmegaStep3 st0, stp;
-    st0.set(tid, N);
-    stp.set(nt, N);
-    {
-        OmegaStep3 p = st0;
-        for (int e = tid; e < nx; e += nt, p.advance(stp, N)) row[p.i0 + S2 * p.i1 + S3 * p.i2] = src[e];
-    }
-    __syncthreads();
-#pragma unroll 1
-    for (int axis = 0; axis < 3; axis++) {
-        const int stride = axis == 0 ? 1 : (axis == 1 ? S2 : S3);
-        for (int l = tid; l < NN; l += nt) {
-            const int i = l % N, j = l / N;  // the other two indices, fastest first
-            const int base = axis == 0 ? S2 * i + S3 * j : (axis == 1 ? i + S3 * j : i + S2 * j);
-            cT<T> x[kRowDftMaxN];
-#pragma unroll
-            for (int a = 0; a < kRowDftMaxN; a++) x[a] = a < N ? row[base + stride * a] : czero<T>();
-#pragma unroll 1
-            for (int k = 0; k < N; k++) {
-                cT<T> acc = czero<T>();
-                int idx = 0;  // (a k) mod N
-#pragma unroll
-                for (int a = 0; a < kRowDftMaxN; a++) {
-                    if (a < N) {
-                        cfma<T>(acc, x[a], tws[idx]);
-                        idx += k;
-                        idx -= idx >= N ? N : 0;
-                    }
-                }
-                row[base + stride * k] = acc;
-            }
-        }
-        __syncthreads();
-    }
-    OmegaStep3 p = st0;
-    for (int e = tid; e < nx; e += nt, p.advance(stp, N)) dst[e] = cscale<T>(row[p.i0 + S2 * p.i1 + S3 * p.i2], scale);
-}
-
-template <class T> static size_t row_dft_lds(int N) {
-    return sizeof(cT<T>) * ((size_t)kRowDftMaxN + (size_t)(N + 1) * N * N);
-}
-// whether the one-pass row kernel serves this grid (3-D, N <= 22, the row in LDS)
-template <class T> static bool row_dft_fits(const XformDev<T>& X) {
-    return X.dim == 3 && X.N <= kRowDftMaxN && row_dft_lds<T>(X.N) <= lds_limit() && X.nrc < (1ll << 31);
-}
-template <class T> static void launch_omega_dft_row(const XformDev<T>& X, const void* in, void* out, bool inverse,
-                                                    hipStream_t st) {
-    const int N = X.N, NN = N * N;
-    const size_t sh = row_dft_lds<T>(N);
-    allow_lds((const void*)k_omega_dft_row<T>, sh);
-    const int nt = std::min(512, (NN + 63) / 64 * 64);
-    hipLaunchKernelGGL(k_omega_dft_row<T>, dim3((unsigned)X.nrc), dim3(nt), sh, st, (const cT<T>*)in, (cT<T>*)out,
-                       inverse ? X.twNi : X.twNf, N, inverse ? T(1) / T(NN * N) : T(1));
-}
-
+// out-of-place omega DFT: on the chirp path the first pass reads in and
+// writes out (the k3 pass of a 3-D grid, then the (k1, k2) pass in place on
+// out: DFTs along different axes commute); otherwise a copy and the in-place
+// transform
 template <class T>
 void launch_omega_dft_to(const XformDev<T>& X, const void* in, void* out, bool inverse, hipStream_t st) {
-    if (row_dft_fits<T>(X)) {
-        launch_omega_dft_row<T>(X, in, out, inverse, st);
+    const int N = X.N, NN = N * N;
+    if ((X.dim == 2 || X.dim == 3) && N <= 32 && X.bsF && in != out) {
+        const cT<T>* bs = inverse ? X.bsI : X.bsF;
+        const T s2 = inverse ? T(1) / T(NN) : T(1);
+        if (X.dim == 3) {
+            launch_axis_dft_mfma<T>(in, out, N, (int64_t)NN, X.nrc, bs, inverse ? T(1) / T(N) : T(1), st);
+            launch_slab_dft_mfma<T>(out, out, X.nrc * N, N, bs, s2, st);
+        } else {
+            launch_slab_dft_mfma<T>(in, out, X.nrc, N, bs, s2, st);
+        }
         return;
     }
     if (in != out)
@@ -1351,14 +1282,10 @@ void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hi
     if (X.dim == 0) return;
     const int N = X.N, NN = N * N;
     HPUP_REQUIRE(NN <= 4096, HPUP_ERR_UNSUPPORTED, "omega grid larger than 64 per axis");
-    if (row_dft_fits<T>(X)) {
-        launch_omega_dft_row<T>(X, F, F, inverse, st);
-        return;
-    }
     if (N <= 32 && X.bsF) {  // chirp-Toeplitz products on MFMA
         const cT<T>* bs = inverse ? X.bsI : X.bsF;
         const int64_t nslab = X.nrc * (X.dim == 3 ? N : 1);
-        launch_slab_dft_mfma<T>(F, nslab, N, bs, inverse ? T(1) / T(NN) : T(1), st);
+        launch_slab_dft_mfma<T>(F, F, nslab, N, bs, inverse ? T(1) / T(NN) : T(1), st);
         if (X.dim == 3) launch_axis_dft_mfma<T>(F, F, N, (int64_t)NN, X.nrc, bs, inverse ? T(1) / T(N) : T(1), st);
         return;
     }
