@@ -33,8 +33,8 @@ MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0 / 6}
 # (scaled by the copy-probe factors) for the kernels of this workload; the
 # summaries are under profiles/.  None where not measured.
 MEASURED_TRAFFIC = {
-    ("k_drift_fused", "fp32", "RK4"): 5.061e9,   # profiles/k_drift_fused_B15_N30_RK4.md
-    ("k_drift_fused", "fp64", "RK4"): 9.435e9,
+    ("k_drift_fused", "fp32", "RK4"): 5.085e9,   # profiles/k_drift_fused_B15_N30_RK4.md
+    ("k_drift_fused", "fp64", "RK4"): 9.451e9,
 }
 
 

@@ -414,8 +414,10 @@ template <bool R> struct FusedWaves<float, GemmF<float, R>> { static constexpr i
 // Re<T_p^H(-i K_p ybar), y>, formed in the GEMM pair the adjoint leaves free
 // (planes A and D hold -i K_1 ybar and -i K_2 ybar), the diffusion ones
 // elementwise; each workgroup writes its five sums to D.gpart.
-// SMIX (forward only): structured mixing coefficients (DriftDev::smix), each
-// multiplied by its one nonzero component.
+// SMIX: structured mixing coefficients (DriftDev::smix), each multiplied by
+// its one nonzero component (the adjoint's transposed tables keep the
+// classes: conjugation leaves a real coefficient real, an imaginary one
+// imaginary).
 template <class T, class G, int DIM, bool GRAV, bool ADJ = false, bool GRAD = false, bool SMIX = false>
 __global__ __launch_bounds__(256, (GRAD ? (sizeof(T) == 4 ? 3 : 2) : FusedWaves<T, G>::n)) void k_drift_fused(DriftDev<T> D, MixDev<T> MX, const cT<T>* Fin,
                                                      const cT<T>* __restrict__ Z3, const cT<T>* F0, cT<T>* Fout,
@@ -1307,18 +1309,23 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
         if (D.dim == 3) {
             HPUP_REQUIRE(!D.gpart, HPUP_ERR_UNSUPPORTED, "parameter gradients: reduced model (dim 2) only");
             HPUP_REQUIRE(D.s3, HPUP_ERR_ARG, "3-D adjoint stage needs its axis-3 term");
-            launch_fused_t<T, GemmF<T, true>, 3, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+            if (D.smix) launch_fused_t<T, GemmF<T, true>, 3, false, true, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+            else launch_fused_t<T, GemmF<T, true>, 3, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
             return;
         }
         if (D.gpart) {
             HPUP_REQUIRE(D.gy && sc.deriv_only, HPUP_ERR_ARG, "gradient stage needs the forward stage value");
-            const unsigned nwg = launch_fused_t<T, GemmF<T, true>, 2, false, true, true>(D, MX, Fin, Z3, F0, Fout, ACC,
-                                                                                       sc, nslab, st);
+            const unsigned nwg =
+                D.smix ? launch_fused_t<T, GemmF<T, true>, 2, false, true, true, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc,
+                                                                                      nslab, st)
+                       : launch_fused_t<T, GemmF<T, true>, 2, false, true, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc,
+                                                                                nslab, st);
             hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(kGradRedThreads), 0, st, (const double*)D.gpart, (int)nwg,
                                (double)D.gscale, D.gout);
             return;
         }
-        launch_fused_t<T, GemmF<T, true>, 2, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        if (D.smix) launch_fused_t<T, GemmF<T, true>, 2, false, true, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        else launch_fused_t<T, GemmF<T, true>, 2, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
         return;
     }
     if (D.ramp) {
