@@ -963,7 +963,9 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_q60(XformDev<T> X, const T* __res
         for (int k = 0; k < 15; k++) zr[k] = y[k];
     }
     __syncthreads();
-    const int xl = pr / 30, m = pr % 30, mneg = m == 0 ? 0 : n2 - m;
+    // quads of neighbouring omega points take neighbouring lanes, so each G
+    // store instruction writes runs of XC consecutive points
+    const int xl = pr % XC, m = pr / XC, mneg = m == 0 ? 0 : n2 - m;
 #pragma unroll
     for (int t = 0; t < 15; t++) {
         const int g = 4 * t + q, gr = g < 30 ? g : g - 30;

@@ -148,6 +148,23 @@ def test_propagate_reduced_den(B, N, method):
     assert rel_l2(out, ref) < 1e-10
 
 
+def test_mixing_without_structure():
+    """u_1 given a real part: the mixing coefficients lose the models'
+    structure (u_1 imaginary, u_2 real, -i CG_1 real, -i CG_2 imaginary), the
+    host detects it and the general complex mixing runs, against the oracle
+    with the same u, at the 32-point and the 64-point tile."""
+    for N in (8, 40):
+        _print("case unstructured mixing", N)
+        F, X, MR, bt, G, u, cells, L = _reduced_case(B=3, N=N)
+        u2 = u.copy()
+        u2[..., 0] = u2[..., 0] + 0.3 * np.abs(u2[..., 0])
+        B, dt = F.shape[2], 0.01
+        ref = O.integrate(B, 2, N, F, X, None, MR, bt, G, dt, L, u2, cells, "RK2")
+        out = hpup.pendulum_propagate_reduced(F, X, MR, bt, G, dt, L, u2, cells, "RK2")
+        assert rel_l2(out, ref) < 1e-10
+        assert rel_l2(out, O.integrate(B, 2, N, F, X, None, MR, bt, G, dt, L, u, cells, "RK2")) > 1e-6
+
+
 def test_reduced_gateway_large_omega_grid():
     """N = 34 on the reduced gateway: the 64-point MFMA drift kernel."""
     _print("case reduced N=34")
