@@ -645,10 +645,11 @@ template <class T> struct PlanT : PlanBase {
         return nk;
     }
     const char* profile_name(int i) const override {
-        static const char* fz[2] = {"k_axis3_mfma", "k_drift_fused"};
-        static const char* fz64[2] = {"k_axis3_mfma", "k_drift_fused64"};
         static const char* names[3] = {"k_rotmix", "k_axis_outer", "k_drift_slab"};
-        if (fused()) return (i >= 0 && i < 2) ? (model.N > 32 ? fz64 : fz)[i] : "";
+        if (fused()) {
+            if (i == 0) return "k_axis3_mfma";
+            return i == 1 ? drift_fused_name<T>(dr.view, den, bf16x3) : "";
+        }
         return (i >= 0 && i < 3) ? names[i] : "";
     }
     // Algorithmic HBM bytes per launch of kernel i, averaged over the stages

@@ -82,6 +82,8 @@ template <class T> struct MixDev {
 template <class T>
 void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
                         void* Fout, void* ACC, const StageCoef<T>& sc, bool bf16x3, hipStream_t st);
+// name of the kernel launch_drift_fused runs for D (grav: density-form gravity)
+template <class T> const char* drift_fused_name(const DriftDev<T>& D, bool grav, bool bf16x3);
 // out = T3 in; scaled: out = T3 [(e3[r] - i b3 kc(k3)) in] (the 3-D adjoint's
 // axis-3 term, on an adjoint plan's tables)
 template <class T>

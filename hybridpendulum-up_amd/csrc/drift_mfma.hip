@@ -1014,6 +1014,279 @@ __global__ __launch_bounds__(kFused64Threads, 1) void k_drift_fused64(DriftDev<T
     }
 }
 
+// ------------------------------------------ fused kernel, compact layout ---
+//
+// The same operator and epilogue as k_drift_fused (forward, real circulants,
+// structured mixing) with the elementwise phases on a compact layout: lane
+// tid of the workgroup holds the slab elements e = tid + 256 i, i < EPL
+// (k2 = e / N, k1 = e % N), so no lane carries padding of the 32 x 32 MFMA
+// tile, and the slab, T3 F, mixing-source and epilogue accesses of a wave are
+// contiguous.  Only the circulant GEMMs use the tile layout.  The two layouts
+// meet in LDS, three barriers per slab:
+//   compact: load F, T3 F; F -> A; mixing; V1 (without the gyroscopic P2
+//            term) -> B, V2 -> C; S' = W + (e3 + i b3 c3) T3 F in registers
+//   sync 1
+//   tile:    P1 = T1 A, P2 = T2 A; B -= i kc3 gyro3 P2 (own elements);
+//            D = i (b1 kc1 P1 + b2 kc2 P2)
+//   sync 2
+//   tile:    Q = T1 B + T2 C -> A (valid elements only: the padding of A,
+//            B and C stays zero from the start)
+//   sync 3
+//   compact: dF = A + D + S'; epilogue.
+// A compact lane writes and reads only its own elements, so the next slab's
+// F -> A needs no further barrier; the tile lanes read A, B, C and write D
+// only between barriers.
+template <class T> struct CompactWaves { static constexpr int n = sizeof(T) == 4 ? 4 : 2; };
+
+template <class T, int DIM, int EPL>
+__global__ __launch_bounds__(256, CompactWaves<T>::n) void k_drift_fused_c(DriftDev<T> D, MixDev<T> MX,
+                                                                          const cT<T>* Fin, const cT<T>* __restrict__ Z3,
+                                                                          const cT<T>* F0, cT<T>* Fout, cT<T>* ACC,
+                                                                          StageCoef<T> sc, int64_t nslab) {
+    using G = GemmF<T, true, 32>;
+    constexpr int PL = 32 * kPad;
+    __shared__ T pa_re[PL], pa_im[PL], pb_re[PL], pb_im[PL], pc_re[PL], pc_im[PL], pd_re[PL], pd_im[PL];
+    __shared__ CircTab<T> tabs[2];
+    constexpr int NC = DIM + 2;
+    constexpr int MB = MixBatch<T>::n * (EPL <= 2 ? 2 : 1);  // source rows loaded together
+
+    const int N = D.N, NN = N * N;
+    const int nz = DIM == 3 ? N : 1;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int k2t = w >> 1, k1t = w & 1, lr = lane & 15, lq = lane >> 4;
+    tabs[0].fill(D.kap, N, tid);
+    tabs[1].fill(D.kap + N, N, tid);
+    for (int i = tid; i < PL; i += 256) {  // the padding stays zero for the whole launch
+        pa_re[i] = pa_im[i] = pb_re[i] = pb_im[i] = pc_re[i] = pc_im[i] = T(0);
+    }
+    __syncthreads();
+    G gm;
+    gm.init(&tabs[0], &tabs[1], k2t, k1t, lr, lq, N);
+
+    // tile layout: lane's four elements (16 k2t + orow(lq, v), 16 k1t + lr)
+    const int k1m = 16 * k1t + lr;
+    const T kc1m = k1m < N ? D.kc[k1m] : T(0);
+    T kc2m[4];
+    bool okm[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int k2 = 16 * k2t + G::orow(lq, v);
+        okm[v] = k1m < N && k2 < N;
+        kc2m[v] = okm[v] ? D.kc[k2] : T(0);
+    }
+    // compact layout: elements e = tid + 256 i
+    T kc1c[EPL], kc2c[EPL], ddc[EPL];
+    int pos[EPL];  // LDS index k2 kPad + k1 (0 for e >= NN)
+    unsigned offc[EPL];
+    bool okc[EPL];
+    const T g00 = D.G[0], g11 = D.G[DIM + 1], g01 = D.G[1] + D.G[DIM];
+    const T g22 = DIM == 3 ? D.G[8] : T(0), g02 = DIM == 3 ? D.G[2] + D.G[6] : T(0),
+            g12 = DIM == 3 ? D.G[5] + D.G[7] : T(0);
+#pragma unroll
+    for (int i = 0; i < EPL; i++) {
+        const int e = tid + 256 * i;
+        okc[i] = e < NN;
+        const int ec = okc[i] ? e : NN - 1;
+        const int k2 = ec / N, k1 = ec % N;
+        pos[i] = k2 * kPad + k1;
+        offc[i] = (unsigned)ec * (unsigned)sizeof(cT<T>);
+        kc1c[i] = D.kc[k1];
+        kc2c[i] = D.kc[k2];
+        ddc[i] = g00 * D.kd[k1] + g11 * D.kd[k2] - g01 * kc1c[i] * kc2c[i];
+    }
+
+    const int nrc = (int)D.nrc;
+    const int nxcd = gridDim.x >= 8 ? 8 : 1;
+    const int per = (int)gridDim.x / nxcd;
+    const int xcd = (int)blockIdx.x % nxcd;
+    const int hi = (int)(nslab * (xcd + 1) / nxcd);
+    int sl = (int)(nslab * xcd / nxcd) + (int)blockIdx.x / nxcd;
+    const int dr = per % nrc;
+    const int dk = per / nrc;
+    int r = sl % nrc;
+    int k3 = sl / nrc;
+    while (sl < hi) {
+        const int64_t base = ((int64_t)r * nz + k3) * NN;
+        const cT<T>* Fb = Fin + base;
+        const T kc3 = DIM == 3 ? as_const<T>(D.kc)[k3] : T(0), kd3 = DIM == 3 ? as_const<T>(D.kd)[k3] : T(0);
+
+        // ---- compact: slab, T3 F, plane A, mixing
+        cT<T> fo[EPL], Zv[EPL];
+#pragma unroll
+        for (int i = 0; i < EPL; i++) {
+            fo[i] = *at_b(Fb, offc[i]);
+            Zv[i] = DIM == 3 ? ld_once<T>(at_b(Z3 + base, offc[i])) : czero<T>();
+        }
+#pragma unroll
+        for (int i = 0; i < EPL; i++)
+            if (okc[i]) {
+                pa_re[pos[i]] = fo[i].x;
+                pa_im[pos[i]] = fo[i].y;
+            }
+        cT<T> V1[EPL], V2[EPL], Wv[EPL];
+#pragma unroll
+        for (int i = 0; i < EPL; i++) {
+            T dd = ddc[i];
+            if (DIM == 3) dd += g22 * kd3 - (g02 * kc1c[i] + g12 * kc2c[i]) * kc3;
+            Wv[i] = cscale<T>(fo[i], dd);
+            V1[i] = czero<T>();
+            V2[i] = czero<T>();
+        }
+        const int TS = MX.TS;
+        const kc_ptr<int32_t> srcT = as_const<int32_t>(MX.src) + (int64_t)r * TS;
+        const kc_ptr<T> coefT = as_const<T>((const T*)MX.coef) + 2 * (int64_t)r * TS * NC;
+        for (int t0 = 0; t0 < TS; t0 += MB) {
+            int src[MB];
+            cT<T> fv[MB][EPL];
+#pragma unroll
+            for (int u = 0; u < MB; u++) src[u] = t0 + u < TS ? srcT[t0 + u] : -1;
+#pragma unroll
+            for (int u = 0; u < MB; u++) {
+                const cT<T>* Fs = Fin + ((int64_t)(src[u] < 0 ? 0 : (src[u] & 0xffffff)) * nz + k3) * NN;
+#pragma unroll
+                for (int i = 0; i < EPL; i++) fv[u][i] = *at_b(Fs, offc[i]);
+            }
+#pragma unroll
+            for (int u = 0; u < MB; u++) {
+                if (src[u] < 0) continue;
+                const int fl = src[u] >> 24;
+                const int64_t cb = (int64_t)(t0 + u) * NC;
+                if (fl & 1) {  // W += (a + i b) F, a = Re(-i CG_1) kc1, b = Im(-i CG_2) kc2
+                    const T g0 = coefT[2 * cb], g1 = coefT[2 * (cb + 1) + 1];
+#pragma unroll
+                    for (int i = 0; i < EPL; i++) {
+                        const T a = g0 * kc1c[i], b = g1 * kc2c[i];
+                        Wv[i].x = fma(a, fv[u][i].x, fma(-b, fv[u][i].y, Wv[i].x));
+                        Wv[i].y = fma(a, fv[u][i].y, fma(b, fv[u][i].x, Wv[i].y));
+                    }
+                }
+                if (fl & 2) {
+                    const T ui = coefT[2 * (cb + DIM) + 1];
+#pragma unroll
+                    for (int i = 0; i < EPL; i++) {
+                        V1[i].x = fma(-ui, fv[u][i].y, V1[i].x);
+                        V1[i].y = fma(ui, fv[u][i].x, V1[i].y);
+                    }
+                }
+                if (fl & 4) {
+                    const T ur = coefT[2 * (cb + DIM + 1)];
+#pragma unroll
+                    for (int i = 0; i < EPL; i++) {
+                        V2[i].x = fma(ur, fv[u][i].x, V2[i].x);
+                        V2[i].y = fma(ur, fv[u][i].y, V2[i].y);
+                    }
+                }
+            }
+        }
+        // gyroscopic terms from T3 F; S' = W + (e3 + i b3 kc3) T3 F
+        cT<T> Sp[EPL];
+        const cT<T> e3 = DIM == 3 ? kc_cplx<T>(as_const<T>((const T*)D.e3), r) : czero<T>();
+#pragma unroll
+        for (int i = 0; i < EPL; i++) {
+            if (DIM == 3 && D.has_gyro) {
+                V1[i] = csub<T>(V1[i], cimul<T>(Zv[i], kc2c[i] * D.gyro[1]));
+                V2[i] = csub<T>(V2[i], cimul<T>(Zv[i], kc1c[i] * D.gyro[0]));
+            }
+            Sp[i] = Wv[i];
+            if (DIM == 3) {
+                Sp[i] = cadd<T>(Sp[i], cmul<T>(Zv[i], e3));
+                Sp[i] = cadd<T>(Sp[i], cimul<T>(Zv[i], D.b[2] * kc3));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < EPL; i++)
+            if (okc[i]) {
+                pb_re[pos[i]] = V1[i].x;
+                pb_im[pos[i]] = V1[i].y;
+                pc_re[pos[i]] = V2[i].x;
+                pc_im[pos[i]] = V2[i].y;
+            }
+        __syncthreads();  // 1: A, B, C complete
+
+        // ---- tile: P GEMMs, the gyroscopic P2 term into B, the damping terms into D
+        {
+            cT<T> P1[4], P2[4];
+            gm.t1(pa_re, pa_im, P1, D.c0[0]);
+            gm.t2(pa_re, pa_im, P2, D.c0[1]);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (!okm[v]) continue;
+                const int p = (16 * k2t + G::orow(lq, v)) * kPad + k1m;
+                if (DIM == 3 && D.has_gyro) {
+                    const cT<T> g = cimul<T>(P2[v], kc3 * D.gyro[2]);
+                    pb_re[p] -= g.x;
+                    pb_im[p] -= g.y;
+                }
+                const cT<T> s = cadd<T>(cimul<T>(P1[v], D.b[0] * kc1m), cimul<T>(P2[v], D.b[1] * kc2m[v]));
+                pd_re[p] = s.x;
+                pd_im[p] = s.y;
+            }
+        }
+        __syncthreads();  // 2: B and D complete; A consumed
+        {
+            cT<T> Q1[4], Q2[4];
+            gm.t1(pb_re, pb_im, Q1, D.c0[0]);
+            gm.t2(pc_re, pc_im, Q2, D.c0[1]);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (!okm[v]) continue;
+                const int p = (16 * k2t + G::orow(lq, v)) * kPad + k1m;
+                pa_re[p] = Q1[v].x + Q2[v].x;
+                pa_im[p] = Q1[v].y + Q2[v].y;
+            }
+        }
+        __syncthreads();  // 3: Q in A complete
+
+        // ---- compact: dF = Q + D + S', epilogue (as k_drift_fused)
+        cT<T> q[EPL], f0v[EPL], acv[EPL];
+#pragma unroll
+        for (int i = 0; i < EPL; i++)
+            q[i] = cadd<T>(cadd<T>(cmk<T>(pa_re[pos[i]], pa_im[pos[i]]), cmk<T>(pd_re[pos[i]], pd_im[pos[i]])), Sp[i]);
+        if (sc.deriv_only) {
+#pragma unroll
+            for (int i = 0; i < EPL; i++)
+                if (okc[i]) st_once<T>(at_b(Fout + base, offc[i]), cscale<T>(q[i], sc.dt));
+        } else {
+            if (sc.read_f0) {
+#pragma unroll
+                for (int i = 0; i < EPL; i++) f0v[i] = ld_once<T>(at_b(F0 + base, offc[i]));
+            } else {
+#pragma unroll
+                for (int i = 0; i < EPL; i++) f0v[i] = fo[i];
+            }
+            if (sc.read_acc) {
+#pragma unroll
+                for (int i = 0; i < EPL; i++) acv[i] = ld_once<T>(at_b(ACC + base, offc[i]));
+            } else {
+#pragma unroll
+                for (int i = 0; i < EPL; i++) acv[i] = czero<T>();
+            }
+            cT<T> fn[EPL], an[EPL];
+#pragma unroll
+            for (int i = 0; i < EPL; i++) {
+                const cT<T> upd = cadd<T>(cscale<T>(acv[i], sc.ca), cscale<T>(q[i], sc.cd));
+                fn[i] = cadd<T>(f0v[i], cscale<T>(upd, sc.dt));
+                an[i] = cadd<T>(cscale<T>(acv[i], sc.ea), cscale<T>(q[i], sc.ed));
+                asm volatile("" : "+v"(fn[i].x), "+v"(fn[i].y), "+v"(an[i].x), "+v"(an[i].y));
+            }
+#pragma unroll
+            for (int i = 0; i < EPL; i++) {
+                if (okc[i]) {
+                    st_once<T>(at_b(Fout + base, offc[i]), fn[i]);
+                    if (sc.write_acc) st_once<T>(at_b(ACC + base, offc[i]), an[i]);
+                }
+            }
+        }
+        sl += per;
+        r += dr;
+        k3 += dk;
+        if (r >= nrc) {
+            r -= nrc;
+            k3++;
+        }
+    }
+}
+
 // out[i] += scale * sum over the workgroups' partials, one workgroup of
 // kGradRedThreads: thread t sums partials t, t + kGradRedThreads, ... in that
 // order, then a fixed pairwise tree combines the threads, so the result does
@@ -1231,6 +1504,33 @@ template <class K> static unsigned persistent_grid(K kernel, int64_t nslab, int 
     return (unsigned)g;
 }
 
+// the compact-layout kernel serves omega grids up to kCompactMaxN points per
+// axis (at most two elements per lane: the padding of the MFMA tile is then
+// at least half of it)
+#ifndef HPUP_COMPACT_MAXN
+#define HPUP_COMPACT_MAXN 22
+#endif
+constexpr int kCompactMaxN = HPUP_COMPACT_MAXN;
+
+template <class T, int DIM, int EPL>
+static void launch_compact_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3,
+                             const void* F0, void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab,
+                             hipStream_t st) {
+    auto kern = k_drift_fused_c<T, DIM, EPL>;
+    const unsigned grid = persistent_grid(kern, nslab);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, D, MX, (const cT<T>*)Fin, (const cT<T>*)Z3,
+                       (const cT<T>*)F0, (cT<T>*)Fout, (cT<T>*)ACC, sc, nslab);
+}
+template <class T, int DIM>
+static void launch_compact(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* Z3, const void* F0,
+                           void* Fout, void* ACC, const StageCoef<T>& sc, int64_t nslab, hipStream_t st) {
+    const int NN = D.N * D.N;
+    if (NN <= 256) launch_compact_t<T, DIM, 1>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+    else if (NN <= 512) launch_compact_t<T, DIM, 2>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+    else if (NN <= 768) launch_compact_t<T, DIM, 3>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+    else launch_compact_t<T, DIM, 4>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+}
+
 template <class T, bool RAMP, bool GRAV, bool SMIX>
 static void launch_fused64_t(const DriftDev<T>& D, const MixDev<T>& MX, const void* Fin, const void* F0, void* Fout,
                              void* ACC, const StageCoef<T>& sc, int64_t nslab, hipStream_t st) {
@@ -1328,6 +1628,11 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
         else launch_fused_t<T, GemmF<T, true>, 2, false, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
         return;
     }
+    if (D.ramp && D.smix && !grav && D.N <= kCompactMaxN) {
+        if (D.dim == 3) launch_compact<T, 3>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        else launch_compact<T, 2>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+        return;
+    }
     if (D.ramp) {
         if (D.dim == 3)
             launch_fwd_t<T, GemmF<T, true>, 3, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
@@ -1343,6 +1648,12 @@ void launch_drift_fused(const DriftDev<T>& D, const MixDev<T>& MX, const void* F
         launch_fused_t<T, GemmF<T>, 2, true>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
     else
         launch_fused_t<T, GemmF<T>, 2, false>(D, MX, Fin, Z3, F0, Fout, ACC, sc, nslab, st);
+}
+
+template <class T> const char* drift_fused_name(const DriftDev<T>& D, bool grav, bool bf16x3) {
+    if (D.N > 32) return "k_drift_fused64";
+    if (!bf16x3 && !D.adj && D.ramp && D.smix && !grav && D.N <= kCompactMaxN) return "k_drift_fused_c";
+    return "k_drift_fused";
 }
 
 template <class T>
@@ -1539,6 +1850,8 @@ template void launch_drift_fused<float>(const DriftDev<float>&, const MixDev<flo
                                         const void*, void*, void*, const StageCoef<float>&, bool, hipStream_t);
 template void launch_drift_fused<double>(const DriftDev<double>&, const MixDev<double>&, const void*, const void*,
                                          const void*, void*, void*, const StageCoef<double>&, bool, hipStream_t);
+template const char* drift_fused_name<float>(const DriftDev<float>&, bool, bool);
+template const char* drift_fused_name<double>(const DriftDev<double>&, bool, bool);
 template void launch_axis3_mfma<float>(const DriftDev<float>&, const void*, void*, bool, hipStream_t, bool);
 template void launch_axis3_mfma<double>(const DriftDev<double>&, const void*, void*, bool, hipStream_t, bool);
 
