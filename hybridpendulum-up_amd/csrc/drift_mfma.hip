@@ -1714,8 +1714,10 @@ __global__ __launch_bounds__(256) void k_slab_dft_mfma(const cT<T>* in, cT<T>* o
     fill_toeplitz<T>(tab, bs, tid);
     if (tid < 32) pk[tid] = bs[64 + tid];
     __syncthreads();
+    // reductions trimmed to the N data points (plane A is zero beyond N along
+    // k1; plane B's rows k2 >= N are zero through p_k2 = 0)
     GemmF<T, false> gm;
-    gm.init(&tab, &tab, k2t, k1t, lr, lq);
+    gm.init(&tab, &tab, k2t, k1t, lr, lq, N);
     const int NN = N * N;
     const int k1 = 16 * k1t + lr;
     const cT<T> p1 = pk[k1];
@@ -1796,16 +1798,23 @@ __global__ __launch_bounds__(256) void k_axis_dft_mfma(const cT<T>* in, cT<T>* o
     acc_t acc[4];
 #pragma unroll
     for (int it = 0; it < 4; it++) acc[it] = MF::zero();
-#pragma unroll 4
-    for (int s = 0; s < 16; s++) {
-        const int k = 4 * s + lq;
-        const T bv = (s < 8 ? sre : sim)[(k & 31) * CP + 16 * w + lr];
-        const cT<T>* t = s < 8 ? tab.re_in : tab.im_in;
+    // K = 4 steps over the rows that hold data (rows a >= N are zero), for the
+    // real and then the imaginary half
+    const int ks = (N + 3) / 4;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const cT<T> f = t[16 * h + lr + 32 - (k & 31)];
-            acc[h] = MF::fma(f.x, bv, acc[h]);
-            acc[h + 2] = MF::fma(f.y, bv, acc[h + 2]);
+    for (int half = 0; half < 2; half++) {
+        const T* sp = half == 0 ? sre : sim;
+        const cT<T>* t = half == 0 ? tab.re_in : tab.im_in;
+#pragma unroll 4
+        for (int s = 0; s < ks; s++) {
+            const int k = 4 * s + lq;
+            const T bv = sp[k * CP + 16 * w + lr];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const cT<T> f = t[16 * h + lr + 32 - k];
+                acc[h] = MF::fma(f.x, bv, acc[h]);
+                acc[h + 2] = MF::fma(f.y, bv, acc[h + 2]);
+            }
         }
     }
     const int64_t j = j0 + 16 * w + lr;

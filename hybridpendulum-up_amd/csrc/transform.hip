@@ -484,6 +484,30 @@ __device__ __forceinline__ int xcd_major(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
+// Work of block blockIdx.x of an (alpha, gamma) kernel: the slice beta_j and
+// the omega points x0 .. x0 + XC - 1.  The blocks take groups of XT x-tiles
+// (the points of one 128-byte line of G) for every j in turn, so that the
+// blocks an XCD runs side by side share the G lines of neighbouring x-tiles
+// and also the f lines that straddle the slices j and j + 1 of a (gamma, x) row
+// (with j slowest, every such line was fetched twice: 1.47x the f bytes).
+template <class T, int XC>
+__device__ __forceinline__ void ag_block(int nxc, int n2, int& j, int64_t& x0) {
+    constexpr int XT = XC * (int)sizeof(cT<T>) >= 128 ? 1 : 128 / (XC * (int)sizeof(cT<T>));
+    const int w = xcd_major(blockIdx.x, gridDim.x);
+    const int full = nxc / XT, head = full * XT * n2;
+    int xt;
+    if (w < head) {
+        const int r = w % (XT * n2);
+        j = r / XT;
+        xt = (w / (XT * n2)) * XT + r % XT;
+    } else {  // the last nxc % XT x-tiles
+        const int tr = nxc - full * XT, r = w - head;
+        j = r / tr;
+        xt = full * XT + r % tr;
+    }
+    x0 = (int64_t)xt * XC;
+}
+
 // tasks of a stage -> parts of the block: <= 64 -> 4, <= 128 -> 2, else 1
 __device__ __forceinline__ int stage_parts(int T) { return T <= 64 ? 4 : (T <= 128 ? 2 : 1); }
 
@@ -496,9 +520,9 @@ __global__ __launch_bounds__(256) void k_fwd_ag_r(XformDev<T> X, const T* __rest
     T* sf = reinterpret_cast<T*>(smem_raw);                  // [xl][g][a]  XC*n2*rp
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);          // [m>=0][g][xl], aliases sf
     const cTab<T> E = (cTab<T>)(const T*)X.twMB;
-    const int w = xcd_major(blockIdx.x, gridDim.x);
-    const int j = w / nxc;
-    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    int j;
+    int64_t x0;
+    ag_block<T, XC>(nxc, n2, j, x0);
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
     const int64_t n2c = (int64_t)n2 * n2 * n2;
@@ -563,9 +587,9 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
     const int B = X.B, n2 = 2 * B, M = 2 * B - 1;
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m][g][xl]
     const cTab<T> ET = (cTab<T>)(const T*)X.twMB + 2 * kAgMax * kAgMax;  // [t][r]
-    const int w = xcd_major(blockIdx.x, gridDim.x);
-    const int j = w / nxc;
-    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    int j;
+    int64_t x0;
+    ag_block<T, XC>(nxc, n2, j, x0);
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
     // H[m][g] = sum_n G[m][n] e^{-i n gamma_g}; item (m, xl) holds its G row
@@ -756,9 +780,9 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft(XformDev<T> X, const cT<T>* _
     extern __shared__ char smem_raw[];
     constexpr int n2 = 2 * B, M = 2 * B - 1, lmax = B - 1, SH = fft_sh<T>(XC, n2);
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m >= 0][g][xl], row stride SH
-    const int w = xcd_major(blockIdx.x, gridDim.x);
-    const int j = w / nxc;
-    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    int j;
+    int64_t x0;
+    ag_block<T, XC>(nxc, n2, j, x0);
     const int64_t nx = X.nx;
     // the Hermitian part of G (get_g_half): H[-m][g] = conj H[m][g], so only
     // the rows m >= 0 are transformed over n
@@ -810,9 +834,9 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_fft(XformDev<T> X, const T* __res
     constexpr int n2 = 2 * B, lmax = B - 1, M = 2 * B - 1, rp = n2 + 1, SH = fft_sh<T>(XC, n2);
     T* sf = reinterpret_cast<T*>(smem_raw);          // [xl][g][a]  XC*n2*rp
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw);  // [m>=0][g][xl], row stride SH, aliases sf
-    const int w = xcd_major(blockIdx.x, gridDim.x);
-    const int j = w / nxc;
-    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    int j;
+    int64_t x0;
+    ag_block<T, XC>(nxc, n2, j, x0);
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
     stage_f_tile<T, XC, NT, n2>(f, wR, wstride, j, x0, nx, sf, rp);
@@ -935,9 +959,9 @@ __global__ __launch_bounds__(NT) void k_fwd_ag_q60(XformDev<T> X, const T* __res
     cT<T>* tw = reinterpret_cast<cT<T>*>(smem_raw);
     T* sf = reinterpret_cast<T*>(smem_raw + 64 * sizeof(cT<T>));       // [xl][g][a], row stride rp
     cT<T>* sZ = reinterpret_cast<cT<T>*>(smem_raw + 64 * sizeof(cT<T>));  // [xl][g < 30][K], aliases sf
-    const int w = xcd_major(blockIdx.x, gridDim.x);
-    const int j = w / nxc;
-    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    int j;
+    int64_t x0;
+    ag_block<T, XC>(nxc, n2, j, x0);
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
     if (tid < n2) tw[tid] = X.twB[tid];
@@ -1004,9 +1028,9 @@ __global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* _
     cT<T>* tw = reinterpret_cast<cT<T>*>(smem_raw);
     cT<T>* sH = reinterpret_cast<cT<T>*>(smem_raw + 64 * sizeof(cT<T>));  // [xl][m >= 0][g]
     T* so = reinterpret_cast<T*>(smem_raw + 64 * sizeof(cT<T>));          // [xl][g][a], aliases sH
-    const int w = xcd_major(blockIdx.x, gridDim.x);
-    const int j = w / nxc;
-    const int64_t x0 = (int64_t)(w % nxc) * XC;
+    int j;
+    int64_t x0;
+    ag_block<T, XC>(nxc, n2, j, x0);
     const int64_t nx = X.nx;
     const int tid = threadIdx.x;
     if (tid < n2) tw[tid] = X.twB[tid];
