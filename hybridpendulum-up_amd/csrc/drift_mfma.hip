@@ -1721,24 +1721,30 @@ __global__ __launch_bounds__(256) void k_slab_dft_mfma(const cT<T>* in, cT<T>* o
     const int NN = N * N;
     const int k1 = 16 * k1t + lr;
     const cT<T> p1 = pk[k1];
-    for (int sl = blockIdx.x; sl < nslab; sl += gridDim.x) {
+    // the slab's four elements per thread (clamped in-range loads); the next
+    // slab's are loaded while this one is transformed
+    cT<T> x[4];
+    auto load = [&](int sl) {
         const cT<T>* Fb = in + (int64_t)sl * NN;
-        cT<T>* Fo = out + (int64_t)sl * NN;
-        cT<T> x[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {  // clamped in-range loads, zero outside
-            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-            const bool ok = a2 < N && a1 < N;
-            const cT<T> v = Fb[ok ? a2 * N + a1 : 0];
-            x[i] = ok ? cmul<T>(v, pk[a1]) : czero<T>();
-        }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-            pa_re[a2 * kPad + a1] = x[i].x;
-            pa_im[a2 * kPad + a1] = x[i].y;
+            x[i] = Fb[a2 < N && a1 < N ? a2 * N + a1 : 0];
+        }
+    };
+    if ((int)blockIdx.x < nslab) load(blockIdx.x);
+    for (int sl = blockIdx.x; sl < nslab; sl += gridDim.x) {
+        cT<T>* Fo = out + (int64_t)sl * NN;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {  // pre-chirp along k1, zero outside the slab
+            const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
+            const cT<T> v = a2 < N && a1 < N ? cmul<T>(x[i], pk[a1]) : czero<T>();
+            pa_re[a2 * kPad + a1] = v.x;
+            pa_im[a2 * kPad + a1] = v.y;
         }
         __syncthreads();  // plane A complete; plane B of the previous slab consumed
+        // another slab: no overlap with this slab's output, also in place
+        if (sl + (int)gridDim.x < nslab) load(sl + gridDim.x);
         cT<T> R[4];
         gm.t1(pa_re, pa_im, R);
 #pragma unroll

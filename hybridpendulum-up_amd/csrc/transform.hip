@@ -55,10 +55,14 @@ template <class T> __device__ __forceinline__ cT<T> mirror_coef(cT<T> z, int d) 
 // The inverse beta stage therefore writes H on the same half as the forward
 // (rows mn >= (M^2 - 1)/2), and the (alpha, gamma) stages read the other half
 // as conjugates.  Value at (mi, ni); Gx: G at (mn = 0, j, x).
+// One load at the selected row and a sign select, so that a caller's loads
+// stay branch-free and issue together.
 template <class T> __device__ __forceinline__ cT<T> get_g_half(const cT<T>* Gx, int B, int mi, int ni) {
     const int M = 2 * B - 1, mn = mi + M * ni;
     const int64_t rs = (int64_t)2 * B * kGX;
-    return mn >= (M * M - 1) / 2 ? Gx[mn * rs] : cconj<T>(Gx[(int64_t)(M * M - 1 - mn) * rs]);
+    const bool stored = mn >= (M * M - 1) / 2;
+    const cT<T> v = Gx[(int64_t)(stored ? mn : M * M - 1 - mn) * rs];
+    return cmk<T>(v.x, stored ? v.y : -v.y);
 }
 
 // ---------------------------------------------------------- forward 2-D --
@@ -644,33 +648,51 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
 
 // The f tile of a forward (alpha, gamma) block, [xl][g][a] at row stride rp:
 // f(alpha_a, beta_j, gamma_g, x0 + xl) (times the grid weight wR when given),
-// zero beyond nx.  Each thread issues all of its loads before the first LDS
-// store, so the tile costs one memory round trip instead of one per element
-// the thread stages.
+// zero beyond nx.  A row of 2B alpha values is contiguous in f, so the tile
+// is read as 16-byte (8-byte for 2B = 30 in FP32) pieces at 32-bit offsets from
+// the block's base, all issued before the first use: one memory round trip
+// per tile.  The weights are applied in a second pass; a weight branch inside
+// the load loop had made every load wait for its data (one round trip per
+// element).
 template <class T, int XC, int NT, int n2>
 __device__ __forceinline__ void stage_f_tile(const T* __restrict__ f, const T* __restrict__ wR, int wstride, int j,
                                              int64_t x0, int64_t nx, T* sf, int rp) {
-    constexpr int TOT = XC * n2 * n2, NL = (TOT + NT - 1) / NT;
-    const int64_t n2c = (int64_t)n2 * n2 * n2;
+    constexpr int VE = n2 % (16 / (int)sizeof(T)) == 0 ? 16 / (int)sizeof(T) : 8 / (int)sizeof(T);
+    static_assert(VE >= 1 && n2 % VE == 0, "rows of whole pieces");
+    typedef T VecT __attribute__((ext_vector_type(VE)));
+    constexpr int RV = n2 / VE, TOT = XC * n2 * RV, NL = (TOT + NT - 1) / NT;
+    const char* fb = reinterpret_cast<const char*>(f + (int64_t)n2 * j + (int64_t)n2 * n2 * n2 * x0);
+    const int xlmax = (int)(nx - 1 - x0 < XC - 1 ? nx - 1 - x0 : XC - 1);  // last omega point of the tile
     const int tid = threadIdx.x;
-    T v[NL];
+    VecT v[NL];
 #pragma unroll
     for (int u = 0; u < NL; u++) {
         const int i = tid + u * NT;
         const int ic = i < TOT ? i : TOT - 1;
-        const int xl = ic / (n2 * n2), g = (ic / n2) % n2, a = ic % n2;
-        const int64_t x = x0 + xl;
-        const int64_t iR = a + (int64_t)n2 * (j + (int64_t)n2 * g);
-        const T fv = f[iR + n2c * (x < nx ? x : nx - 1)];
-        v[u] = (i < TOT && x < nx) ? fv : T(0);
-        if (wR) v[u] *= wR[iR * wstride];  // transform of the product f(R, x) w(R)
+        const int xl = ic / (n2 * RV), g = (ic / RV) % n2, pc = ic % RV;
+        const int xc = xl < xlmax ? xl : xlmax;
+        const unsigned off = (unsigned)((pc * VE + n2 * n2 * g + n2 * n2 * n2 * xc) * (int)sizeof(T));
+        v[u] = *reinterpret_cast<const VecT*>(fb + off);
+    }
+    if (wR) {  // transform of the product f(R, x) w(R)
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int i = tid + u * NT;
+            const int ic = i < TOT ? i : TOT - 1;
+            const int g = (ic / RV) % n2, pc = ic % RV;
+#pragma unroll
+            for (int e = 0; e < VE; e++)
+                v[u][e] *= wR[(pc * VE + e + (int64_t)n2 * (j + (int64_t)n2 * g)) * wstride];
+        }
     }
 #pragma unroll
     for (int u = 0; u < NL; u++) {
         const int i = tid + u * NT;
         if (i < TOT) {
-            const int xl = i / (n2 * n2), g = (i / n2) % n2, a = i % n2;
-            sf[(xl * n2 + g) * rp + a] = v[u];
+            const int xl = i / (n2 * RV), g = (i / RV) % n2, pc = i % RV;
+            T* dst = sf + (xl * n2 + g) * rp + pc * VE;
+#pragma unroll
+            for (int e = 0; e < VE; e++) dst[e] = xl <= xlmax ? v[u][e] : T(0);
         }
     }
 }
