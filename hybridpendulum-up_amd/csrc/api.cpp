@@ -44,7 +44,7 @@ template <class T> std::vector<T> cast_vec(const std::vector<double>& v) { retur
 
 // Transform tables from a Wigner-d table d (M,M,B,2B) and weights w (2B).
 template <class T> struct XformTables {
-    DevBuf Wf, Wi, twB, twMB, dftN, idftN, twNf, twNi, bsF, bsI;
+    DevBuf Wf, Wi, twB, twMB, dftN, idftN, twNf, twNi, bsF, bsI, hrows;
     XformDev<T> view{};
     void build(int B, int dim, int N, const double* d, const double* w) {
         const int M = 2 * B - 1, n2 = 2 * B;
@@ -104,6 +104,17 @@ template <class T> struct XformTables {
             }
             bsF = upload(bf); bsI = upload(bi);
         }
+        // the half rows of the omega DFTs: (l, m, n) with n > 0, or n = 0 and
+        // m >= 0, each with the row of (l, -m, -n) (bit 30: (-1)^(m-n) = -1)
+        std::vector<int32_t> hr;
+        for (int l = 0; l < B; l++)
+            for (int n = 0; n <= l; n++)
+                for (int m = (n == 0 ? 0 : -l); m <= l; m++)
+                    hr.insert(hr.end(), {(int32_t)ridx_B(B, l, m, n),
+                                         (int32_t)ridx_B(B, l, -m, -n) | (((m - n) & 1) ? (1 << 30) : 0)});
+        hrows = upload(hr);
+        view.hrows = hrows.as<int32_t>();
+        view.nhalf = (int64_t)hr.size() / 2;
         view.B = B; view.dim = dim; view.N = Nn;
         view.nx = ipow64(Nn, dim); view.nrc = nrc_of(B);
         view.Wf = Wf.as<T>(); view.Wi = Wi.as<T>(); view.twB = twB.as<cT<T>>();
@@ -333,18 +344,32 @@ template <class T> struct PlanT : PlanBase {
         const int M = 2 * model.B - 1;
         ensure(G, sizeof(cT<T>) * (size_t)g_elems(model.B, nx));
         launch_fwd_ag<T>(xf.view, f, wR, wstride, G.p, st);
-        launch_beta_fwd<T>(xf.view, G.p, F, st);
-        launch_omega_dft<T>(xf.view, F, nullptr, false, st);
+        if (half_omega()) {  // the omega DFTs on half of the rows write the other half
+            launch_beta_fwd<T>(xf.view, G.p, F, st, false);
+            launch_omega_half<T>(xf.view, F, F, false, st);
+        } else {
+            launch_beta_fwd<T>(xf.view, G.p, F, st);
+            launch_omega_dft<T>(xf.view, F, nullptr, false, st);
+        }
         HPUP_HIP(hipGetLastError());
     }
     void backward(const void* F, void* f, hipStream_t st) override {
         const int M = 2 * model.B - 1;
         ensure(G, sizeof(cT<T>) * (size_t)g_elems(model.B, nx));
         ensure(tmp, cbytes());
-        launch_omega_dft_to<T>(xf.view, F, tmp.p, true, st);
-        launch_beta_inv<T>(xf.view, tmp.p, G.p, st);
+        if (half_omega()) {  // the Hermitian part on half of the rows
+            launch_omega_half<T>(xf.view, F, tmp.p, true, st);
+            launch_beta_inv<T>(xf.view, tmp.p, G.p, st, false);
+        } else {
+            launch_omega_dft_to<T>(xf.view, F, tmp.p, true, st);
+            launch_beta_inv<T>(xf.view, tmp.p, G.p, st);
+        }
         launch_inv_ag<T>(xf.view, G.p, f, st);
         HPUP_HIP(hipGetLastError());
+    }
+    // omega DFTs of the transforms on half of the rows (chirp MFMA kernels)
+    bool half_omega() const {
+        return (model.dim == 2 || model.dim == 3) && model.N <= 32 && model.B <= 32 && xf.view.bsF && xf.view.hrows;
     }
     // MFMA kernels cover omega grids up to 32 points per axis, and up to 64 for
     // the reduced model (dim 2); larger grids use the LDS-tiled FMA kernels

@@ -130,6 +130,10 @@ template <class T> struct XformDev {
     const cT<T>* twNi;  // e^{+2 pi i t/N}
     const cT<T>* bsF;   // chirp-Toeplitz tables of the forward / inverse omega DFT (N <= 32),
     const cT<T>* bsI;   // 96 entries each (launch_slab_dft_mfma)
+    // the rows (l, m, n) with n > 0, or n = 0 and m >= 0: pairs (row, row of
+    // (l, -m, -n) | bit 30 when (-1)^(m-n) = -1), nhalf of them
+    const int32_t* hrows;
+    int64_t nhalf;
 };
 
 // Elements of the angular/beta intermediate G: (2B-1)^2 2B rows of the Omega
@@ -141,10 +145,14 @@ inline int64_t g_elems(int B, int64_t nx) {
 template <class T>
 // wR (nullable): transform the product f(R, x) wR[R * wstride] (grid weight)
 void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstride, void* G, hipStream_t st);
+// mirror = false: only the rows with n > 0, or n = 0 and m >= 0 are written
+// (for launch_omega_half, which writes the others); bandlimit <= 32
 template <class T>
-void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t st);
+void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t st, bool mirror = true);
+// sym = false: F's rows already hold the Hermitian part (launch_omega_half's
+// inverse), so only the rows (l, m, n) of the stored half are read
 template <class T>
-void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t st);
+void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t st, bool sym = true);
 template <class T>
 void launch_inv_ag(const XformDev<T>& X, const void* G, void* f, hipStream_t st);
 // in-place omega DFT of the compact rows (forward sign -1, inverse +1 with 1/nx)
@@ -153,6 +161,14 @@ void launch_omega_dft(const XformDev<T>& X, void* F, void* tmp, bool inverse, hi
 // the same from in to out (out = in allowed)
 template <class T>
 void launch_omega_dft_to(const XformDev<T>& X, const void* in, void* out, bool inverse, hipStream_t st);
+// Omega DFTs of the coefficients of a real grid on the half rows only (N <= 32).
+// Forward (in place): the input holds the mirror relation F_{-m,-n} = (-1)^(m-n)
+// conj F_mn on its half rows' partners implicitly (only the half rows are read)
+// and every row is written.  Inverse (out of place): out's half rows get the
+// inverse DFT of the Hermitian part (F_mn(k) + (-1)^(m-n) conj F_{-m,-n}(-k)) / 2;
+// the other rows of out are not written.
+template <class T>
+void launch_omega_half(const XformDev<T>& X, const void* in, void* out, bool inverse, hipStream_t st);
 
 // layout conversion full (M,M,B,nx) <-> compact [r][x]
 template <class T>

@@ -1701,11 +1701,43 @@ __device__ __forceinline__ void fill_toeplitz(CircTab<T>& t, const cT<T>* bs, in
     }
 }
 
-// 2-D DFT over (k1, k2) of every N x N slab of in, times `scale`, into out
-// (in place when they are equal: a slab is read whole before it is written).
-template <class T>
+// Omega DFTs of the coefficients of a real grid.  Before the omega DFT the
+// coefficients satisfy F^l_{-m,-n}(x) = s conj F^l_mn(x), s = (-1)^(m-n)
+// (the forward beta stage writes that mirror), so after it
+// F^l_{-m,-n}(k) = s conj F^l_mn(-k): the forward DFTs need only the rows
+// with n > 0, or n = 0 and m >= 0 (the half list hr: pairs (row, mirror row
+// | bit 30 when s = -1)), and the last pass writes the mirror rows.  The
+// inverse transform's real output depends on the coefficients only through
+// their Hermitian part (F^l_mn(k) + s conj F^l_{-m,-n}(-k)) / 2, which the
+// first inverse pass forms on the half rows, so the rest of the inverse runs
+// on those rows only.
+enum OmegaRows {
+    kRowsAll = 0,     // every row
+    kRowsHalf = 1,    // the half rows
+    kRowsMirror = 2,  // the half rows, output written to the mirror rows too
+    kRowsSym = 3      // the half rows, input taken as the Hermitian part
+};
+
+struct HalfRow {
+    int r, rm;  // row and mirror row
+    bool neg;   // s = -1
+};
+__device__ __forceinline__ HalfRow half_row(const int32_t* hr, int64_t i) {
+    const int32_t b = hr[2 * i + 1];
+    return {hr[2 * i], b & 0x3fffffff, ((b >> 30) & 1) != 0};
+}
+// s conj z
+template <class T> __device__ __forceinline__ cT<T> sconj(cT<T> z, bool neg) {
+    return neg ? cmk<T>(-z.x, z.y) : cmk<T>(z.x, -z.y);
+}
+
+// 2-D DFT over (k1, k2) of N x N slabs of in, times `scale`, into out (in
+// place when they are equal: a slab is read whole before it is written, and
+// the mirror rows a kRowsMirror pass writes are read by no other slab).  Slab
+// sl is row sl / nz at k3 = sl % nz (rows from the half list unless kRowsAll).
+template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_slab_dft_mfma(const cT<T>* in, cT<T>* out, const cT<T>* __restrict__ bs,
-                                                       int N, T scale, int nslab) {
+                                                       int N, T scale, int nslab, int nz, const int32_t* __restrict__ hr) {
     __shared__ T pa_re[32 * kPad], pa_im[32 * kPad], pb_re[32 * kPad], pb_im[32 * kPad];
     __shared__ CircTab<T> tab;
     __shared__ cT<T> pk[32];
@@ -1721,24 +1753,55 @@ __global__ __launch_bounds__(256) void k_slab_dft_mfma(const cT<T>* in, cT<T>* o
     const int NN = N * N;
     const int k1 = 16 * k1t + lr;
     const cT<T> p1 = pk[k1];
-    // the slab's four elements per thread (clamped in-range loads); the next
-    // slab's are loaded while this one is transformed
-    cT<T> x[4];
+    // slab sl: its base, and for the mirror modes the mirror slab's (at -k3)
+    auto slab = [&](int sl, int64_t& b, int64_t& bm, bool& neg, bool& self) {
+        if (MODE == kRowsAll) {
+            b = (int64_t)sl * NN;
+            bm = b;
+            neg = false;
+            self = true;
+            return;
+        }
+        const int i = sl / nz, k3 = sl - i * nz;
+        const HalfRow h = half_row(hr, i);
+        b = ((int64_t)h.r * nz + k3) * NN;
+        bm = ((int64_t)h.rm * nz + (k3 == 0 ? 0 : nz - k3)) * NN;
+        neg = h.neg;
+        self = h.rm == h.r;
+    };
+    // the slab's four elements per thread (clamped in-range loads; with the
+    // mirror slab's element at (-k2, -k1) for kRowsSym); the next slab's are
+    // loaded while this one is transformed
+    cT<T> x[4], y[4];
     auto load = [&](int sl) {
-        const cT<T>* Fb = in + (int64_t)sl * NN;
+        int64_t b, bm;
+        bool ng, sf;
+        slab(sl, b, bm, ng, sf);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-            x[i] = Fb[a2 < N && a1 < N ? a2 * N + a1 : 0];
+            const bool ok = a2 < N && a1 < N;
+            x[i] = in[b + (ok ? a2 * N + a1 : 0)];
+            if (MODE == kRowsSym) {
+                const int m2 = a2 == 0 ? 0 : N - a2, m1 = a1 == 0 ? 0 : N - a1;
+                y[i] = in[bm + (ok ? m2 * N + m1 : 0)];
+            }
         }
     };
     if ((int)blockIdx.x < nslab) load(blockIdx.x);
     for (int sl = blockIdx.x; sl < nslab; sl += gridDim.x) {
-        cT<T>* Fo = out + (int64_t)sl * NN;
+        int64_t b, bm;
+        bool neg, self;
+        slab(sl, b, bm, neg, self);
 #pragma unroll
         for (int i = 0; i < 4; i++) {  // pre-chirp along k1, zero outside the slab
             const int idx = tid + 256 * i, a2 = idx >> 5, a1 = idx & 31;
-            const cT<T> v = a2 < N && a1 < N ? cmul<T>(x[i], pk[a1]) : czero<T>();
+            cT<T> u = x[i];
+            if (MODE == kRowsSym) {
+                const cT<T> c = sconj<T>(y[i], neg);
+                u = cmk<T>(T(0.5) * (u.x + c.x), T(0.5) * (u.y + c.y));
+            }
+            const cT<T> v = a2 < N && a1 < N ? cmul<T>(u, pk[a1]) : czero<T>();
             pa_re[a2 * kPad + a1] = v.x;
             pa_im[a2 * kPad + a1] = v.y;
         }
@@ -1750,16 +1813,21 @@ __global__ __launch_bounds__(256) void k_slab_dft_mfma(const cT<T>* in, cT<T>* o
 #pragma unroll
         for (int v = 0; v < 4; v++) {  // along k1 done; pre-chirp along k2 (p = 0 in the padding)
             const int k2 = 16 * k2t + GemmF<T, false>::orow(lq, v);
-            const cT<T> y = cmul<T>(cmul<T>(R[v], p1), pk[k2]);
-            pb_re[k2 * kPad + k1] = y.x;
-            pb_im[k2 * kPad + k1] = y.y;
+            const cT<T> yv = cmul<T>(cmul<T>(R[v], p1), pk[k2]);
+            pb_re[k2 * kPad + k1] = yv.x;
+            pb_im[k2 * kPad + k1] = yv.y;
         }
         __syncthreads();  // plane B complete; plane A consumed
         gm.t2(pb_re, pb_im, R);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int k2 = 16 * k2t + GemmF<T, false>::orow(lq, v);
-            if (k1 < N && k2 < N) Fo[k2 * N + k1] = cscale<T>(cmul<T>(R[v], pk[k2]), scale);
+            if (k1 < N && k2 < N) {
+                const cT<T> z = cscale<T>(cmul<T>(R[v], pk[k2]), scale);
+                out[b + k2 * N + k1] = z;
+                if (MODE == kRowsMirror && !self)
+                    out[bm + (k2 == 0 ? 0 : N - k2) * N + (k1 == 0 ? 0 : N - k1)] = sconj<T>(z, neg);
+            }
         }
     }
 }
@@ -1767,28 +1835,53 @@ __global__ __launch_bounds__(256) void k_slab_dft_mfma(const cT<T>* in, cT<T>* o
 // DFT along the outermost axis: lines of N points at stride `inner`, one per
 // (line block L, column j); 64-column chunks as the B operand of the Toeplitz
 // GEMM (the k_axis3_mfma layout), times `scale`.  Out of place or in place.
-template <class T>
+// Line block blockIdx.y is a row (from the half list unless kRowsAll); the
+// mirror of column j = k2 N + k1 is (-k2) N + (-k1) (3-D rows, inner = N^2).
+template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_axis_dft_mfma(const cT<T>* in, cT<T>* out, const cT<T>* __restrict__ bs,
-                                                       int N, int64_t inner, T scale) {
+                                                       int N, int64_t inner, T scale, const int32_t* __restrict__ hr) {
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
     constexpr int CP = 65;
     __shared__ T sre[32 * CP], sim[32 * CP];
     __shared__ CircTab<T> tab;
     __shared__ cT<T> pk[32];
-    const int64_t L = blockIdx.y;
+    int64_t L = blockIdx.y, Lm = L;
+    bool neg = false;
+    if (MODE != kRowsAll) {
+        const HalfRow h = half_row(hr, blockIdx.y);
+        L = h.r;
+        Lm = h.rm;
+        neg = h.neg;
+    }
     const int64_t j0 = (int64_t)blockIdx.x * 64;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
+    // mirror column of j (clamped in range)
+    auto mirror_col = [&](int64_t j) {
+        const int jj = (int)(j < inner ? j : inner - 1), c2 = jj / N, c1 = jj - c2 * N;
+        return (int64_t)(c2 == 0 ? 0 : N - c2) * N + (c1 == 0 ? 0 : N - c1);
+    };
     // the chunk's eight elements per thread (row a = w + 4 i, column lane):
     // unconditional clamped loads, all issued before the first use
-    cT<T> x[8];
+    cT<T> x[8], y[8];
+    {
+        const int64_t j = j0 + lane, jm = MODE == kRowsSym ? mirror_col(j) : 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int a = w + 4 * i;
-        const int64_t j = j0 + lane;
-        const bool ok = a < N && j < inner;
-        x[i] = in[ok ? (L * N + a) * inner + j : 0];
-        if (!ok) x[i] = czero<T>();
+        for (int i = 0; i < 8; i++) {
+            const int a = w + 4 * i;
+            const bool ok = a < N && j < inner;
+            x[i] = in[ok ? (L * N + a) * inner + j : 0];
+            if (MODE == kRowsSym) y[i] = in[ok ? (Lm * N + (a == 0 ? 0 : N - a)) * inner + jm : 0];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int a = w + 4 * i;
+            if (MODE == kRowsSym) {
+                const cT<T> c = sconj<T>(y[i], neg);
+                x[i] = cmk<T>(T(0.5) * (x[i].x + c.x), T(0.5) * (x[i].y + c.y));
+            }
+            if (!(a < N && j < inner)) x[i] = czero<T>();
+        }
     }
     fill_toeplitz<T>(tab, bs, tid);
     if (tid < 32) pk[tid] = bs[64 + tid];
@@ -1825,13 +1918,17 @@ __global__ __launch_bounds__(256) void k_axis_dft_mfma(const cT<T>* in, cT<T>* o
     }
     const int64_t j = j0 + 16 * w + lr;
     if (j >= inner) return;
+    const int64_t jm = MODE == kRowsMirror ? mirror_col(j) : 0;
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int k = 16 * h + MF::orow(lq, v);
-            if (k < N)
-                out[(L * N + k) * inner + j] = cscale<T>(cmul<T>(cmk<T>(acc[h][v], acc[h + 2][v]), pk[k]), scale);
+            if (k < N) {
+                const cT<T> z = cscale<T>(cmul<T>(cmk<T>(acc[h][v], acc[h + 2][v]), pk[k]), scale);
+                out[(L * N + k) * inner + j] = z;
+                if (MODE == kRowsMirror && Lm != L) out[(Lm * N + (k == 0 ? 0 : N - k)) * inner + jm] = sconj<T>(z, neg);
+            }
         }
 }
 
@@ -1841,9 +1938,8 @@ void launch_slab_dft_mfma(const void* in, void* out, int64_t nslab, int N, const
     HPUP_REQUIRE(N <= 32, HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT needs N <= 32");
     HPUP_REQUIRE(nslab < ((int64_t)1 << 31), HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT: too many slabs");
     const unsigned grid = (unsigned)std::min<int64_t>(nslab, 4096);
-    hipLaunchKernelGGL(k_slab_dft_mfma<T>, dim3(grid), dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out,
-                       (const cT<T>*)bs, N, scale,
-                       (int)nslab);
+    hipLaunchKernelGGL((k_slab_dft_mfma<T, kRowsAll>), dim3(grid), dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out,
+                       (const cT<T>*)bs, N, scale, (int)nslab, 1, nullptr);
 }
 
 template <class T>
@@ -1851,10 +1947,57 @@ void launch_axis_dft_mfma(const void* in, void* out, int N, int64_t inner, int64
                           hipStream_t st) {
     HPUP_REQUIRE(N <= 32, HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT needs N <= 32");
     dim3 grid((unsigned)((inner + 63) / 64), (unsigned)nlines);
-    hipLaunchKernelGGL(k_axis_dft_mfma<T>, grid, dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out,
-                       (const cT<T>*)bs, N, inner, scale);
+    hipLaunchKernelGGL((k_axis_dft_mfma<T, kRowsAll>), grid, dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out,
+                       (const cT<T>*)bs, N, inner, scale, nullptr);
 }
 
+template <class T, int MODE>
+static void slab_half(const XformDev<T>& X, const void* in, void* out, const cT<T>* bs, T scale, hipStream_t st) {
+    const int N = X.N, nz = X.dim == 3 ? N : 1;
+    const int64_t nslab = X.nhalf * nz;
+    HPUP_REQUIRE(nslab < ((int64_t)1 << 31), HPUP_ERR_UNSUPPORTED, "chirp MFMA DFT: too many slabs");
+    const unsigned grid = (unsigned)std::min<int64_t>(nslab, 4096);
+    hipLaunchKernelGGL((k_slab_dft_mfma<T, MODE>), dim3(grid), dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out, bs,
+                       N, scale, (int)nslab, nz, X.hrows);
+}
+template <class T, int MODE>
+static void axis_half(const XformDev<T>& X, const void* in, void* out, const cT<T>* bs, T scale, hipStream_t st) {
+    const int N = X.N;
+    const int64_t inner = (int64_t)N * N;
+    HPUP_REQUIRE(X.nhalf <= 65535, HPUP_ERR_UNSUPPORTED, "axis DFT: more than 65535 rows");
+    dim3 grid((unsigned)((inner + 63) / 64), (unsigned)X.nhalf);
+    hipLaunchKernelGGL((k_axis_dft_mfma<T, MODE>), grid, dim3(256), 0, st, (const cT<T>*)in, (cT<T>*)out, bs, N,
+                       inner, scale, X.hrows);
+}
+
+template <class T>
+void launch_omega_half(const XformDev<T>& X, const void* in, void* out, bool inverse, hipStream_t st) {
+    HPUP_REQUIRE((X.dim == 2 || X.dim == 3) && X.N <= 32 && X.bsF && X.hrows, HPUP_ERR_UNSUPPORTED,
+                 "half-row omega DFT: dim 2 or 3, N <= 32");
+    const int N = X.N;
+    const T s2 = inverse ? T(1) / T(N * N) : T(1), s3 = inverse ? T(1) / T(N) : T(1);
+    const cT<T>* bs = inverse ? X.bsI : X.bsF;
+    if (!inverse) {
+        HPUP_REQUIRE(in == out, HPUP_ERR_ARG, "forward half-row omega DFT runs in place");
+        if (X.dim == 3) {
+            slab_half<T, kRowsHalf>(X, in, out, bs, s2, st);
+            axis_half<T, kRowsMirror>(X, out, out, bs, s3, st);
+        } else {
+            slab_half<T, kRowsMirror>(X, in, out, bs, s2, st);
+        }
+        return;
+    }
+    HPUP_REQUIRE(in != out, HPUP_ERR_ARG, "inverse half-row omega DFT runs out of place");
+    if (X.dim == 3) {
+        axis_half<T, kRowsSym>(X, in, out, bs, s3, st);
+        slab_half<T, kRowsHalf>(X, out, out, bs, s2, st);
+    } else {
+        slab_half<T, kRowsSym>(X, in, out, bs, s2, st);
+    }
+}
+
+template void launch_omega_half<float>(const XformDev<float>&, const void*, void*, bool, hipStream_t);
+template void launch_omega_half<double>(const XformDev<double>&, const void*, void*, bool, hipStream_t);
 template void launch_slab_dft_mfma<float>(const void*, void*, int64_t, int, const void*, float, hipStream_t);
 template void launch_slab_dft_mfma<double>(const void*, void*, int64_t, int, const void*, double, hipStream_t);
 template void launch_axis_dft_mfma<float>(const void*, void*, int, int64_t, int64_t, const void*, float, hipStream_t);

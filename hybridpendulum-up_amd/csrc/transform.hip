@@ -167,7 +167,9 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
 // waves takes one (m, n) and one kGX-block of omega points (each wave a
 // 16-point column tile), whose G chunk [j][x % kGX] is contiguous; rows l <
 // max(|m|, |n|) and the padding carry zero weights and are not stored.
-template <class T, int RT, int KS>
+// MIRROR: also write the rows (l, -m, -n) (otherwise the half-row omega DFT
+// writes them after its own transform)
+template <class T, int RT, int KS, bool MIRROR>
 __global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T>* __restrict__ G,
                                                        cT<T>* __restrict__ F) {
     using MF = Mfma<T>;
@@ -213,14 +215,16 @@ __global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T
             if (l >= lmin && l < B) {
                 const cT<T> z = cmk<T>(re[rt][v], im[rt][v]);
                 F[d_ridx(B, l, m, n) * nx + x] = z;
-                if (mn != (M * M - 1) / 2) F[d_ridx(B, l, -m, -n) * nx + x] = mirror_coef<T>(z, m - n);
+                if (MIRROR && mn != (M * M - 1) / 2) F[d_ridx(B, l, -m, -n) * nx + x] = mirror_coef<T>(z, m - n);
             }
         }
 }
 
 // Inverse: G[mn][j][x] = sum_l Wi[mn][j][l] F^l_mn(x) -- A = Wi (rows j, RT
 // tiles; K = l in KS steps), B = the coefficient rows of (m, n).
-template <class T, int RT, int KS>
+// SYM: form the Hermitian part from the rows (l, m, n) and (l, -m, -n)
+// (otherwise the half-row omega DFT has formed it on the rows (l, m, n))
+template <class T, int RT, int KS, bool SYM>
 __global__ __launch_bounds__(256) void k_beta_inv_mfma(XformDev<T> X, const cT<T>* __restrict__ F,
                                                        cT<T>* __restrict__ G) {
     using MF = Mfma<T>;
@@ -242,8 +246,12 @@ __global__ __launch_bounds__(256) void k_beta_inv_mfma(XformDev<T> X, const cT<T
         const int l = 4 * s + lq;
         const bool lok = l >= lmin && l < B;
         const cT<T> v = F[d_ridx(B, lok ? l : lmax, m, n) * nx + xc];
-        const cT<T> u = mirror_coef<T>(F[d_ridx(B, lok ? l : lmax, -m, -n) * nx + xc], m - n);
-        f[s] = (lok && xok) ? cmk<T>(T(0.5) * (v.x + u.x), T(0.5) * (v.y + u.y)) : czero<T>();
+        if (SYM) {
+            const cT<T> u = mirror_coef<T>(F[d_ridx(B, lok ? l : lmax, -m, -n) * nx + xc], m - n);
+            f[s] = (lok && xok) ? cmk<T>(T(0.5) * (v.x + u.x), T(0.5) * (v.y + u.y)) : czero<T>();
+        } else {
+            f[s] = (lok && xok) ? v : czero<T>();
+        }
     }
     const T* W = X.Wi + (size_t)mn * n2 * B;
     acc_t re[RT], im[RT];
@@ -1232,24 +1240,31 @@ void launch_fwd_ag(const XformDev<T>& X, const void* f, const void* wR, int wstr
 // MFMA beta stages for B <= 32 (row tiles and K steps fixed per bandlimit
 // class); larger bandlimits keep the FMA kernels
 template <class T, int RT, int KS>
-static void beta_fwd_mfma(const XformDev<T>& X, const void* G, void* F, hipStream_t st) {
+static void beta_fwd_mfma(const XformDev<T>& X, const void* G, void* F, bool mirror, hipStream_t st) {
     const int M = 2 * X.B - 1;
-    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)((M * M + 1) / 2));  // half of (m, n), mirrored
-    hipLaunchKernelGGL((k_beta_fwd_mfma<T, RT, KS>), grid, dim3(256), 0, st, X, (const cT<T>*)G, (cT<T>*)F);
+    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)((M * M + 1) / 2));  // half of (m, n)
+    if (mirror)
+        hipLaunchKernelGGL((k_beta_fwd_mfma<T, RT, KS, true>), grid, dim3(256), 0, st, X, (const cT<T>*)G, (cT<T>*)F);
+    else
+        hipLaunchKernelGGL((k_beta_fwd_mfma<T, RT, KS, false>), grid, dim3(256), 0, st, X, (const cT<T>*)G, (cT<T>*)F);
 }
 template <class T, int RT, int KS>
-static void beta_inv_mfma(const XformDev<T>& X, const void* F, void* G, hipStream_t st) {
+static void beta_inv_mfma(const XformDev<T>& X, const void* F, void* G, bool sym, hipStream_t st) {
     const int M = 2 * X.B - 1;
     dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)((M * M + 1) / 2));  // the stored half of H
-    hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS>), grid, dim3(256), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
+    if (sym)
+        hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS, true>), grid, dim3(256), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
+    else
+        hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS, false>), grid, dim3(256), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
 }
 
 template <class T>
-void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t st) {
+void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t st, bool mirror) {
     static_assert(kGX == 64, "one workgroup of four 16-point tiles per G block");
-    if (X.B <= 8) return beta_fwd_mfma<T, 1, 4>(X, G, F, st);     // l < 16, j < 16
-    if (X.B <= 16) return beta_fwd_mfma<T, 1, 8>(X, G, F, st);    // l < 16, j < 32
-    if (X.B <= 32) return beta_fwd_mfma<T, 2, 16>(X, G, F, st);   // l < 32, j < 64
+    if (X.B <= 8) return beta_fwd_mfma<T, 1, 4>(X, G, F, mirror, st);     // l < 16, j < 16
+    if (X.B <= 16) return beta_fwd_mfma<T, 1, 8>(X, G, F, mirror, st);    // l < 16, j < 32
+    if (X.B <= 32) return beta_fwd_mfma<T, 2, 16>(X, G, F, mirror, st);   // l < 32, j < 64
+    HPUP_REQUIRE(mirror, HPUP_ERR_UNSUPPORTED, "beta stage without mirror rows: bandlimit <= 32");
     const int M = 2 * X.B - 1;
     dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)((M * M + 1) / 2));  // half of (m, n), mirrored
     hipLaunchKernelGGL(k_beta_fwd<T>, grid, dim3(256), sizeof(T) * X.B * 2 * X.B, st, X, (const cT<T>*)G,
@@ -1257,10 +1272,11 @@ void launch_beta_fwd(const XformDev<T>& X, const void* G, void* F, hipStream_t s
 }
 
 template <class T>
-void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t st) {
-    if (X.B <= 8) return beta_inv_mfma<T, 1, 2>(X, F, G, st);     // j < 16, l < 8
-    if (X.B <= 16) return beta_inv_mfma<T, 2, 4>(X, F, G, st);    // j < 32, l < 16
-    if (X.B <= 32) return beta_inv_mfma<T, 4, 8>(X, F, G, st);    // j < 64, l < 32
+void launch_beta_inv(const XformDev<T>& X, const void* F, void* G, hipStream_t st, bool sym) {
+    if (X.B <= 8) return beta_inv_mfma<T, 1, 2>(X, F, G, sym, st);     // j < 16, l < 8
+    if (X.B <= 16) return beta_inv_mfma<T, 2, 4>(X, F, G, sym, st);    // j < 32, l < 16
+    if (X.B <= 32) return beta_inv_mfma<T, 4, 8>(X, F, G, sym, st);    // j < 64, l < 32
+    HPUP_REQUIRE(sym, HPUP_ERR_UNSUPPORTED, "beta stage on Hermitian rows: bandlimit <= 32");
     const int M = 2 * X.B - 1;
     dim3 grid((unsigned)((X.nx + 255) / 256), (unsigned)((M * M + 1) / 2));  // the stored half of H
     hipLaunchKernelGGL(k_beta_inv<T>, grid, dim3(256), sizeof(T) * X.B * 2 * X.B, st, X, (const cT<T>*)F,
@@ -1369,8 +1385,8 @@ void launch_compact_to_full(int B, int64_t nx, const void* compact, void* full, 
 
 #define HPUP_INST(T)                                                                                  \
     template void launch_fwd_ag<T>(const XformDev<T>&, const void*, const void*, int, void*, hipStream_t); \
-    template void launch_beta_fwd<T>(const XformDev<T>&, const void*, void*, hipStream_t);           \
-    template void launch_beta_inv<T>(const XformDev<T>&, const void*, void*, hipStream_t);           \
+    template void launch_beta_fwd<T>(const XformDev<T>&, const void*, void*, hipStream_t, bool);           \
+    template void launch_beta_inv<T>(const XformDev<T>&, const void*, void*, hipStream_t, bool);           \
     template void launch_inv_ag<T>(const XformDev<T>&, const void*, void*, hipStream_t);             \
     template void launch_omega_dft<T>(const XformDev<T>&, void*, void*, bool, hipStream_t);          \
     template void launch_omega_dft_to<T>(const XformDev<T>&, const void*, void*, bool, hipStream_t); \
