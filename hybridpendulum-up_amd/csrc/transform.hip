@@ -705,6 +705,26 @@ __device__ __forceinline__ void stage_f_tile(const T* __restrict__ f, const T* _
     }
 }
 
+// The output tile of an inverse (alpha, gamma) block from LDS ([xl][g][a], row
+// stride n2) to f(alpha_a, beta_j, gamma_g, x0 + xl): a row of 2B reals is
+// contiguous in f, so it is written in 16-byte pieces (8-byte for 2B = 30 in
+// FP32), one store per piece instead of one per value.
+template <class T, int XC, int NT, int n2>
+__device__ __forceinline__ void store_f_rows(const T* so, T* __restrict__ f, int j, int64_t x0, int64_t nx) {
+    constexpr int VE = n2 % (16 / (int)sizeof(T)) == 0 ? 16 / (int)sizeof(T) : 8 / (int)sizeof(T);
+    static_assert(VE >= 1 && n2 % VE == 0, "rows of whole pieces");
+    typedef T VecT __attribute__((ext_vector_type(VE)));
+    constexpr int RV = n2 / VE;
+    char* fb = reinterpret_cast<char*>(f + (int64_t)n2 * j + (int64_t)n2 * n2 * n2 * x0);
+    for (int e = threadIdx.x; e < XC * n2 * RV; e += NT) {
+        const int xl = e / (n2 * RV), g = (e / RV) % n2, pc = e % RV;
+        if (x0 + xl < nx) {
+            const unsigned off = (unsigned)((pc * VE + n2 * n2 * g + n2 * n2 * n2 * xl) * (int)sizeof(T));
+            *reinterpret_cast<VecT*>(fb + off) = *reinterpret_cast<const VecT*>(so + (xl * n2 + g) * n2 + pc * VE);
+        }
+    }
+}
+
 // ------------------------------------------- (alpha, gamma) stages, 2B = 30 --
 //
 // At 2B = 30 the two angular DFTs of each (beta_j, omega) tile are length-30
@@ -834,24 +854,29 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft(XformDev<T> X, const cT<T>* _
         for (int g = 0; g < n2; g++) sH[m * SH + g * XC + xl] = v[g];
     }
     __syncthreads();
-    for (int task = threadIdx.x; task < n2 * XC; task += NT) {
-        const int g = task / XC, xl = task % XC;
-        const int64_t x = x0 + xl;
-        cT<T> v[n2];
+    // one column (g, xl) per thread; the real rows go through LDS ([xl][g][a],
+    // aliasing sH) so that f is written in whole-row pieces
+    static_assert(n2 * XC <= NT, "one column per thread");
+    const int task = threadIdx.x;
+    const bool act = task < n2 * XC;
+    const int g = act ? task / XC : 0, xl = act ? task % XC : 0;
+    cT<T> v[n2];
 #pragma unroll
-        for (int mi = 0; mi < M; mi++) {
-            const int m = mi - lmax;
-            const cT<T> h = sH[(m >= 0 ? m : -m) * SH + g * XC + xl];
-            v[(mi - lmax + n2) % n2] = m >= 0 ? h : cconj<T>(h);
-        }
-        v[B] = czero<T>();
-        fftn<T, -1, n2>(v);
-        if (x < nx) {
-            T* fo = f + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x));
-#pragma unroll
-            for (int a = 0; a < n2; a++) fo[a] = v[a].x;
-        }
+    for (int mi = 0; mi < M; mi++) {
+        const int m = mi - lmax;
+        const cT<T> h = sH[(m >= 0 ? m : -m) * SH + g * XC + xl];
+        v[(mi - lmax + n2) % n2] = m >= 0 ? h : cconj<T>(h);
     }
+    v[B] = czero<T>();
+    fftn<T, -1, n2>(v);
+    __syncthreads();  // so overwrites sH
+    T* so = reinterpret_cast<T*>(smem_raw);
+    if (act) {
+#pragma unroll
+        for (int a = 0; a < n2; a++) so[(xl * n2 + g) * n2 + a] = v[a].x;
+    }
+    __syncthreads();
+    store_f_rows<T, XC, NT, n2>(so, f, j, x0, nx);
 }
 
 // Forward (alpha, gamma) stage at 2B = 30: H[m][g] = sum_a f[a][g] e^{+i m alpha_a},
@@ -1110,11 +1135,7 @@ __global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* _
         for (int k = 0; k < 15; k++) orow[k] = y[k].x;
     }
     __syncthreads();
-    for (int e = tid; e < XC * n2 * n2; e += NT) {
-        const int xl = e / (n2 * n2), g = (e / n2) % n2, a = e % n2;
-        const int64_t x = x0 + xl;
-        if (x < nx) f[a + (int64_t)n2 * (j + (int64_t)n2 * (g + (int64_t)n2 * x))] = so[(xl * n2 + g) * n2 + a];
-    }
+    store_f_rows<T, XC, NT, n2>(so, f, j, x0, nx);
 }
 
 // ---------------------------------------------------------- launchers -----
@@ -1160,7 +1181,8 @@ static void inv_ag_fft(const XformDev<T>& X, const void* G, void* f, hipStream_t
     constexpr int n2 = 2 * B, M = 2 * B - 1;
     const int nxc = (int)((X.nx + XC - 1) / XC);
     HPUP_REQUIRE((int64_t)nxc * n2 < (1ll << 31), HPUP_ERR_UNSUPPORTED, "omega grid too large");
-    const size_t sh = sizeof(cT<T>) * B * fft_sh<T>(XC, n2);  // rows m >= 0
+    // rows m >= 0 of H, then the real output tile
+    const size_t sh = std::max(sizeof(cT<T>) * B * fft_sh<T>(XC, n2), sizeof(T) * XC * n2 * n2);
     allow_lds((const void*)k_inv_ag_fft<T, B, XC, NT>, sh);
     hipLaunchKernelGGL((k_inv_ag_fft<T, B, XC, NT>), dim3((unsigned)(nxc * n2)), dim3(NT), sh, st, X,
                        (const cT<T>*)G, (T*)f, nxc);
