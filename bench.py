@@ -384,6 +384,54 @@ def row_timings(plan, F, reps=3):
             "get_stat_ms": ts, "get_stat_fit_ok": bool(st["fit_ok"])}
 
 
+def transforms_b30_row(reps=3):
+    """The transforms of the large hybrid workload (B = 30, 20^3 omega grid,
+    FP32 plan), HIP events on the plan's stream."""
+    import torch
+    p = hybrid_plan("fp32")
+    F = hybrid_initial(p)
+    f = p.backward(F)
+    p.forward(f, F)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tb = tf = 0.0
+    for _ in range(reps):
+        ev[0].record()
+        p.backward(F, f)
+        ev[1].record()
+        p.forward(f, F)
+        ev[2].record()
+        torch.cuda.synchronize()
+        tb += ev[0].elapsed_time(ev[1]) / reps
+        tf += ev[1].elapsed_time(ev[2]) / reps
+    return {"transform_backward_ms": tb, "transform_forward_ms": tf, "coefficients": p.ncoef, "grid_points": p.ngrid}
+
+
+def profile_rows(cfg, names):
+    """--rows-only: the named rows alone (a short program to run under a
+    profiler)."""
+    import torch
+    rows = {}
+    for name in names:
+        if name == "transforms":
+            p64 = make_plan(cfg, "fp64", "RK4")
+            F0 = initial_coeffs(p64, cfg)
+            p32 = make_plan(cfg, "fp32", "RK4")
+            for tag, pl, Fr in (("fp64", p64, F0), ("fp32", p32, F0.to(torch.complex64))):
+                rows[tag] = row_timings(pl, Fr)
+                log("rows %s: backward %.2f ms, forward %.2f ms" % (tag, rows[tag]["transform_backward"]["ms"],
+                                                                      rows[tag]["transform_forward"]["ms"]))
+            del p64, p32, F0
+        elif name == "transforms_b30":
+            rows[name] = transforms_b30_row()
+            log("rows %s: backward %.2f ms, forward %.2f ms" % (name, rows[name]["transform_backward_ms"],
+                                                                  rows[name]["transform_forward_ms"]))
+        else:
+            raise SystemExit("bench.py: unknown row %r for --rows-only" % name)
+        torch.cuda.empty_cache()
+    return rows
+
+
 def _omp_threads():
     env = os.environ.get("OMP_NUM_THREADS", "")
     return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
@@ -471,6 +519,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rows", action="store_true", help="skip the transform and get_stat timings")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--rows-only", default="",
+                    help="profiling mode: run only these rows (comma list of 'transforms' -- the B=15, 30^3 "
+                         "FP64 and FP32 transforms and get_stat -- and 'transforms_b30' -- the B=30, 20^3 FP32 "
+                         "transforms of the large hybrid workload), print them as one JSON line and exit")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="at the end, run a plain copy kernel (torch.neg, 4-byte elements) over this many GiB "
                          "three times: the known-size reference for scaling HBM counters")
@@ -482,6 +534,10 @@ def main():
     import torch
     cfg = dict(CFG)
     K, W = args.steps, args.warmup
+    if args.rows_only:
+        print(json.dumps({"rows": profile_rows(cfg, args.rows_only.split(","))}), flush=True)
+        copy_probe(args.copy_probe_gib)
+        return
 
     stages = {"euler": 1, "midpoint": 2, "RK2": 2, "RK4": 4}[args.method]
     log("building FP64 plan and initial coefficients")
@@ -661,15 +717,22 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    copy_probe(args.copy_probe_gib)
 
-    if args.copy_probe_gib > 0:
-        n = int(args.copy_probe_gib * 2 ** 30 / 4)
-        x = torch.ones(n, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            torch.neg(x, out=y)
-        torch.cuda.synchronize()
-        log("copy probe: %d bytes read and %d written per launch" % (4 * n, 4 * n))
+
+def copy_probe(gib):
+    """A plain copy kernel (torch.neg, 4-byte elements) over `gib` GiB, three
+    launches: the known-size reference for scaling HBM counters."""
+    import torch
+    if gib <= 0:
+        return
+    n = int(gib * 2 ** 30 / 4)
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    log("copy probe: %d bytes read and %d written per launch" % (4 * n, 4 * n))
 
 
 if __name__ == "__main__":
