@@ -136,11 +136,16 @@ template <class T> struct XformDev {
     int64_t nhalf;
 };
 
-// Elements of the angular/beta intermediate G: (2B-1)^2 2B rows of the Omega
-// points rounded up to whole blocks of kGX (its layout is Omega-blocked)
+// Elements of the angular/beta intermediate: the forward's G, (2B-1)^2 2B rows
+// of the Omega points rounded up to whole blocks of kGX (its layout is
+// Omega-blocked), or the inverse's H, pairs of Omega points by 2B by the
+// stored half of (m, n) padded to 8 rows, whichever is larger
 constexpr int kGX = 64;
 inline int64_t g_elems(int B, int64_t nx) {
-    return (int64_t)(2 * B - 1) * (2 * B - 1) * 2 * B * ((nx + kGX - 1) / kGX) * kGX;
+    const int64_t M = 2 * B - 1, n2 = 2 * B;
+    const int64_t g = M * M * n2 * ((nx + kGX - 1) / kGX) * kGX;
+    const int64_t h = (nx + 1) / 2 * 2 * n2 * (((M * M + 1) / 2 + 7) / 8 * 8);
+    return g > h ? g : h;
 }
 template <class T>
 // wR (nullable): transform the product f(R, x) wR[R * wstride] (grid weight)

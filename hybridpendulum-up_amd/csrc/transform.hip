@@ -54,14 +54,31 @@ template <class T> __device__ __forceinline__ cT<T> mirror_coef(cT<T> z, int d) 
 // omega point): Re sum_mn G[m][n] e^{-i(m a + n g)} = sum_mn H[m][n] e^{-i(m a + n g)}.
 // The inverse beta stage therefore writes H on the same half as the forward
 // (rows mn >= (M^2 - 1)/2), and the (alpha, gamma) stages read the other half
-// as conjugates.  Value at (mi, ni); Gx: G at (mn = 0, j, x).
-// One load at the selected row and a sign select, so that a caller's loads
-// stay branch-free and issue together.
-template <class T> __device__ __forceinline__ cT<T> get_g_half(const cT<T>* Gx, int B, int mi, int ni) {
-    const int M = 2 * B - 1, mn = mi + M * ni;
-    const int64_t rs = (int64_t)2 * B * kGX;
-    const bool stored = mn >= (M * M - 1) / 2;
-    const cT<T> v = Gx[(int64_t)(stored ? mn : M * M - 1 - mn) * rs];
+// as conjugates.
+//
+// The inverse intermediate H (the Hermitian part on the stored half of (m, n),
+// written by the inverse beta stage) in the order of its readers: for each
+// pair of omega points and each beta_j the half's rows contiguous,
+// [x / 2][j][mh][x % 2], mh = mn - (M^2 - 1)/2 < h_rows(B) (padded to a
+// multiple of 8, so that a 128-byte line holds 8 rows of a pair in FP32).  An
+// (alpha, gamma) block of the inverse then reads whole lines (in the forward
+// order, [(m, n)][j][x % 64], it used 16 bytes of each 128-byte line), and a
+// beta workgroup writes whole lines by taking 8 consecutive rows.
+__host__ __device__ __forceinline__ int h_rows(int B) {
+    const int M = 2 * B - 1;
+    return ((M * M + 1) / 2 + 7) / 8 * 8;
+}
+// H at (mh = 0, j, x); row mh is at [2 mh]
+__device__ __forceinline__ int64_t h_base(int B, int j, int64_t x) {
+    return (((x >> 1) * (2 * B) + j) * (int64_t)h_rows(B)) * 2 + (x & 1);
+}
+// Value at (mi, ni) (mn = mi + M ni) from Hx = H + h_base(B, j, x): the
+// unstored half is read as the conjugate of the row (-m, -n), with one load
+// at the selected row and a sign select
+template <class T> __device__ __forceinline__ cT<T> get_h(const cT<T>* Hx, int B, int mi, int ni) {
+    const int M = 2 * B - 1, mn = mi + M * ni, h0 = (M * M - 1) / 2;
+    const bool stored = mn >= h0;
+    const cT<T> v = Hx[2 * (stored ? mn - h0 : h0 - mn)];
     return cmk<T>(v.x, stored ? v.y : -v.y);
 }
 
@@ -220,22 +237,29 @@ __global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T
         }
 }
 
-// Inverse: G[mn][j][x] = sum_l Wi[mn][j][l] F^l_mn(x) -- A = Wi (rows j, RT
-// tiles; K = l in KS steps), B = the coefficient rows of (m, n).
+// Inverse: H[mn][j][x] = sum_l Wi[mn][j][l] Fs^l_mn(x) -- A = Wi (rows j, RT
+// tiles; K = l in KS steps), B = the coefficient rows of (m, n) (Fs: their
+// Hermitian part).  A workgroup of eight waves takes eight consecutive rows
+// mh of the stored half and 16 omega points, wave w the row 8 blockIdx.y + w,
+// so that together they write whole lines of H (h_base).
 // SYM: form the Hermitian part from the rows (l, m, n) and (l, -m, -n)
 // (otherwise the half-row omega DFT has formed it on the rows (l, m, n))
 template <class T, int RT, int KS, bool SYM>
-__global__ __launch_bounds__(256) void k_beta_inv_mfma(XformDev<T> X, const cT<T>* __restrict__ F,
+__global__ __launch_bounds__(512) void k_beta_inv_mfma(XformDev<T> X, const cT<T>* __restrict__ F,
                                                        cT<T>* __restrict__ G) {
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
-    const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
-    const int mn = (M * M - 1) / 2 + blockIdx.y;  // the stored half of H
+    // one 16-row tile of j at a time, [j % 16][x % 16][w], stored as whole lines
+    __shared__ cT<T> tile[16 * 16 * 8];
+    const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1, h0 = (M * M - 1) / 2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    // the padding rows (mh > h0) compute on the clamped row and store nothing
+    const int mh = 8 * blockIdx.y + w, mhc = mh <= h0 ? mh : h0;
+    const int mn = h0 + mhc;
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
     const int64_t nx = X.nx;
-    const int64_t x = (int64_t)blockIdx.x * kGX + 16 * w + lr;
+    const int64_t x = (int64_t)blockIdx.x * 16 + lr;
     const bool xok = x < nx;
     const int64_t xc = xok ? x : nx - 1;
     // (F^l_mn + (-1)^(m-n) conj F^l_{-m,-n}) / 2: with Wi[-m,-n] = (-1)^(m-n) Wi[m,n]
@@ -268,15 +292,43 @@ __global__ __launch_bounds__(256) void k_beta_inv_mfma(XformDev<T> X, const cT<T
             im[rt] = MF::fma(a, f[s].y, im[rt]);
         }
     }
-    if (!xok) return;
-    cT<T>* Gp = G + g_off(B, mn, 0, x);
+    // H lines: (j, pair of omega points) -> the 8 rows of the workgroup, both
+    // points of the pair per row (FP32: one 16-byte piece per row, FP64: one
+    // per row and point); consecutive threads take consecutive pieces of a line
+    constexpr int PR = sizeof(T) == 4 ? 1 : 2;  // pieces per (row, pair)
+    const int64_t x0 = (int64_t)blockIdx.x * 16;
+    const int mrows = h0 + 1 - 8 * (int)blockIdx.y;  // rows of this workgroup that exist
 #pragma unroll
-    for (int rt = 0; rt < RT; rt++)
+    for (int rt = 0; rt < RT; rt++) {
+        if (16 * rt >= n2) break;
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int j = 16 * rt + MF::orow(lq, v);
-            if (j < n2) Gp[(int64_t)j * kGX] = cmk<T>(re[rt][v], im[rt][v]);
+        for (int v = 0; v < 4; v++) tile[(MF::orow(lq, v) * 16 + lr) * 8 + w] = cmk<T>(re[rt][v], im[rt][v]);
+        __syncthreads();
+#pragma unroll
+        for (int p = threadIdx.x; p < 16 * 8 * 8 * PR; p += 512) {
+            const int r8 = (p / PR) % 8, xp = (p / (PR * 8)) % 8, jl = p / (PR * 64);
+            const int j = 16 * rt + jl;
+            const int64_t xa = x0 + 2 * xp;
+            if (j < n2 && r8 < mrows && xa < nx) {
+                cT<T>* dst = G + h_base(B, j, xa) + 2 * (8 * (int)blockIdx.y + r8);
+                if (PR == 1) {
+                    const cT<T> a = tile[(jl * 16 + 2 * xp) * 8 + r8], b = tile[(jl * 16 + 2 * xp + 1) * 8 + r8];
+                    if (xa + 1 < nx) {
+                        typedef T V4 __attribute__((ext_vector_type(4)));
+                        V4 q;
+                        q.x = a.x; q.y = a.y; q.z = b.x; q.w = b.y;
+                        *reinterpret_cast<V4*>(dst) = q;
+                    } else {
+                        dst[0] = a;
+                    }
+                } else {
+                    const int xo = p % 2;
+                    if (xa + xo < nx) dst[xo] = tile[(jl * 16 + 2 * xp + xo) * 8 + r8];
+                }
+            }
         }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------- inverse -------
@@ -286,14 +338,14 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
     extern __shared__ char smem_raw[];
     T* sW = reinterpret_cast<T*>(smem_raw);
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1;
-    const int mn = (M * M - 1) / 2 + blockIdx.y;  // the stored half of H (see get_g_half)
+    const int mn = (M * M - 1) / 2 + blockIdx.y;  // the stored half of H (see get_h)
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
     for (int i = threadIdx.x; i < B * n2; i += blockDim.x) sW[i] = X.Wi[(size_t)mn * B * n2 + i];
     __syncthreads();
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= X.nx) return;
-    cT<T>* Gp = G + g_off(B, mn, 0, x);
+    const int mh = mn - (M * M - 1) / 2;
     for (int j0 = 0; j0 < n2; j0 += 8) {
         cT<T> acc[8];
 #pragma unroll
@@ -313,7 +365,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             if (j0 + t >= n2) break;
-            Gp[(int64_t)(j0 + t) * kGX] = acc[t];
+            G[h_base(B, j0 + t, x) + 2 * mh] = acc[t];
         }
     }
 }
@@ -344,9 +396,9 @@ __global__ __launch_bounds__(256) void k_inv_ag(XformDev<T> X, const cT<T>* __re
 #pragma unroll
         for (int k = 0; k < GC; k++) acc[k] = czero<T>();
         if (x < nx) {
-            const cT<T>* Gx = G + g_off(B, 0, j, x);
+            const cT<T>* Hx = G + h_base(B, j, x);
             for (int ni = 0; ni < M; ni++) {
-                const cT<T> gv = get_g_half<T>(Gx, B, mi, ni);
+                const cT<T> gv = get_h<T>(Hx, B, mi, ni);
                 const int n = ni - lmax;
                 const int dn = ((n % n2) + n2) % n2;  // (n g) mod n2 advances by dn per gamma
                 int t = (int)(((int64_t)dn * g0) % n2);
@@ -613,11 +665,11 @@ __global__ __launch_bounds__(256) void k_inv_ag_r(XformDev<T> X, const cT<T>* __
         const int64_t x = x0 + xl;
         const bool ld = act && x < nx;
         // clamped, branch-free loads; invalid entries are zeroed after the load
-        const cT<T>* Gx = G + g_off(B, 0, j, min<int64_t>(x, nx - 1));
+        const cT<T>* Hx = G + h_base(B, j, min<int64_t>(x, nx - 1));
         cT<T> gr[kAgMax];
 #pragma unroll
         for (int ni = 0; ni < kAgMax; ni++) {
-            const cT<T> v = get_g_half<T>(Gx, B, min(mi, M - 1), min(ni, M - 1));
+            const cT<T> v = get_h<T>(Hx, B, min(mi, M - 1), min(ni, M - 1));
             gr[ni] = (ld && ni < M) ? v : czero<T>();
         }
         for (int g = part; g < n2; g += S1) {
@@ -834,18 +886,18 @@ __global__ __launch_bounds__(NT) void k_inv_ag_fft(XformDev<T> X, const cT<T>* _
     int64_t x0;
     ag_block<T, XC>(nxc, n2, j, x0);
     const int64_t nx = X.nx;
-    // the Hermitian part of G (get_g_half): H[-m][g] = conj H[m][g], so only
+    // the Hermitian part of G (get_h): H[-m][g] = conj H[m][g], so only
     // the rows m >= 0 are transformed over n
     for (int task = threadIdx.x; task < B * XC; task += NT) {
         const int m = task / XC, xl = task % XC;
         const int64_t x = x0 + xl;
         const bool ld = x < nx;
         // clamped, branch-free loads; a column beyond nx is zeroed after the load
-        const cT<T>* Gx = G + g_off(B, 0, j, min<int64_t>(x, nx - 1));
+        const cT<T>* Hx = G + h_base(B, j, min<int64_t>(x, nx - 1));
         cT<T> v[n2];
 #pragma unroll
         for (int ni = 0; ni < M; ni++) {
-            const cT<T> g = get_g_half<T>(Gx, B, lmax + m, ni);
+            const cT<T> g = get_h<T>(Hx, B, lmax + m, ni);
             v[(ni - lmax + n2) % n2] = ld ? g : czero<T>();
         }
         v[B] = czero<T>();
@@ -1091,7 +1143,7 @@ __global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* _
     if (tid < n2) tw[tid] = X.twB[tid];
     const int q = tid & 3, P = rev2(q);
     cT<T> y[15];
-    // stage 1 on the rows m >= 0 of the Hermitian part of G (get_g_half); the
+    // stage 1 on the rows m >= 0 of the Hermitian part of G (get_h); the
     // rows m < 0 of H are the conjugates
     const bool act1 = tid < T1;
     {
@@ -1100,11 +1152,11 @@ __global__ __launch_bounds__(NT) void k_inv_ag_q60(XformDev<T> X, const cT<T>* _
         const int64_t x = x0 + xl;
         const bool ld = x < nx;
         // clamped loads; a column beyond nx is zeroed after the load
-        const cT<T>* Gx = G + g_off(B, 0, j, min<int64_t>(x, nx - 1));
+        const cT<T>* Hx = G + h_base(B, j, min<int64_t>(x, nx - 1));
 #pragma unroll
         for (int t = 0; t < 15; t++) {
             const int ni = (4 * t + q + 29) % n2;  // 59 at the Nyquist bin K = 30
-            const cT<T> g = get_g_half<T>(Gx, B, lmax + m, ni < M ? ni : 0);
+            const cT<T> g = get_h<T>(Hx, B, lmax + m, ni < M ? ni : 0);
             y[t] = (ld && ni < M) ? g : czero<T>();
         }
     }
@@ -1272,12 +1324,12 @@ static void beta_fwd_mfma(const XformDev<T>& X, const void* G, void* F, bool mir
 }
 template <class T, int RT, int KS>
 static void beta_inv_mfma(const XformDev<T>& X, const void* F, void* G, bool sym, hipStream_t st) {
-    const int M = 2 * X.B - 1;
-    dim3 grid((unsigned)((X.nx + kGX - 1) / kGX), (unsigned)((M * M + 1) / 2));  // the stored half of H
+    // 16 omega points by 8 rows of the stored half of H per workgroup
+    dim3 grid((unsigned)((X.nx + 15) / 16), (unsigned)(h_rows(X.B) / 8));
     if (sym)
-        hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS, true>), grid, dim3(256), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
+        hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS, true>), grid, dim3(512), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
     else
-        hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS, false>), grid, dim3(256), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
+        hipLaunchKernelGGL((k_beta_inv_mfma<T, RT, KS, false>), grid, dim3(512), 0, st, X, (const cT<T>*)F, (cT<T>*)G);
 }
 
 template <class T>
