@@ -48,15 +48,21 @@ template <class T> struct XformTables {
     XformDev<T> view{};
     void build(int B, int dim, int N, const double* d, const double* w) {
         const int M = 2 * B - 1, n2 = 2 * B;
-        std::vector<double> wf((size_t)M * M * B * n2, 0.0), wi((size_t)M * M * B * n2, 0.0);
+        // row strides of the weight tables: whole K groups of the MFMA beta
+        // kernels' reduction (zero beyond the data), the row length beyond B = 32
+        const int wfs = B <= 8 ? 16 : B <= 16 ? 32 : B <= 32 ? 64 : n2;
+        const int wis = B <= 8 ? 8 : B <= 16 ? 16 : B <= 32 ? 32 : B;
+        std::vector<double> wf((size_t)M * M * B * wfs, 0.0), wi((size_t)M * M * n2 * wis, 0.0);
         for (int mn = 0; mn < M * M; mn++)
             for (int l = 0; l < B; l++)
                 for (int j = 0; j < n2; j++) {
                     const int mi = mn % M, ni = mn / M;
                     const double dv = d[mi + (size_t)M * (ni + (size_t)M * (l + (size_t)B * j))];
-                    wf[((size_t)mn * B + l) * n2 + j] = w[j] * dv;
-                    wi[((size_t)mn * n2 + j) * B + l] = (2.0 * l + 1.0) * dv;
+                    wf[((size_t)mn * B + l) * wfs + j] = w[j] * dv;
+                    wi[((size_t)mn * n2 + j) * wis + l] = (2.0 * l + 1.0) * dv;
                 }
+        view.wfs = wfs;
+        view.wis = wis;
         Wf = upload(cast_vec<T>(wf));
         Wi = upload(cast_vec<T>(wi));
         std::vector<T> tb(2 * n2), dn, idn, tf, ti;

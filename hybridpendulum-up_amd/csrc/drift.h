@@ -119,8 +119,9 @@ void launch_drift_slab(const DriftDev<T>& D, const void* Fin, const void* Z3, co
 template <class T> struct XformDev {
     int B, dim, N;
     int64_t nx, nrc;
-    const T* Wf;        // forward beta weights  [mn][l][j] = w_j d^l_mn(beta_j)
-    const T* Wi;        // inverse beta weights  [mn][j][l] = (2l+1) d^l_mn(beta_j)
+    const T* Wf;        // forward beta weights  [mn][l][j] = w_j d^l_mn(beta_j), rows of wfs
+    const T* Wi;        // inverse beta weights  [mn][j][l] = (2l+1) d^l_mn(beta_j), rows of wis
+    int wfs, wis;       // row strides (zero-padded to whole reduction groups)
     const cT<T>* twB;   // e^{i pi t / B}, t in [0, 2B)
     const cT<T>* twMB;  // [32][32] e^{i pi (r - (B-1)) t / B} for r < 2B-1, t < 2B, zero elsewhere, then its
                         // transpose [t][r] (B <= 16)

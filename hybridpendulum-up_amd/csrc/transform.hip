@@ -8,7 +8,9 @@
 //   2. k_beta_fwd per (m,n): F^l_mn(x) = sum_j w_j d^l_mn(beta_j) G[(m,n)][j][x]
 //   3. omega DFT of every compact row (slab kernel on axes 1,2; axis kernel on 3)
 // Inverse: the same stages reversed (1/nx on omega, unnormalized over alpha,
-// gamma, real part), Nyquist bins of alpha and gamma implicitly zero.
+// gamma, real part), Nyquist bins of alpha and gamma implicitly zero; the
+// intermediate (H, the Hermitian part) in the order of its (alpha, gamma)
+// readers (see h_base).
 #include <algorithm>
 
 #include "drift.h"
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
     const int mn = (M * M - 1) / 2 + blockIdx.y;  // the half of G the forward stages fill
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
-    for (int i = threadIdx.x; i < B * n2; i += blockDim.x) sW[i] = X.Wf[(size_t)mn * B * n2 + i];
+    for (int i = threadIdx.x; i < B * n2; i += blockDim.x) sW[i] = X.Wf[((size_t)mn * B + i / n2) * X.wfs + i % n2];
     __syncthreads();
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= X.nx) return;
@@ -177,6 +179,16 @@ __global__ __launch_bounds__(256) void k_beta_fwd(XformDev<T> X, const cT<T>* __
     }
 }
 
+// Reduction order of the beta GEMMs: the K = 4 steps in groups of KG = min(KS,
+// 4); lane group lq takes index (s / KG) 4 KG + KG lq + s % KG at step s, so
+// that its KG consecutive steps read consecutive weights of a row (the tables'
+// rows are zero-padded to 4 KS entries, whole groups)
+template <int KS> struct KGroup { static constexpr int n = KS < 4 ? KS : 4; };
+template <int KS> __device__ __forceinline__ int kstep_index(int s, int lq) {
+    constexpr int KG = KGroup<KS>::n;
+    return (s / KG) * 4 * KG + KG * lq + s % KG;
+}
+
 // The beta stages as MFMA GEMMs per (m, n) block over the omega points.
 // Forward: F^l_mn(x) = sum_j Wf[mn][l][j] G[mn][j][x] -- A = Wf (rows l, RT
 // tiles of 16; K = j in KS steps of 4), B = G (K = j, columns x), one real
@@ -200,14 +212,16 @@ __global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T
     const int64_t x = (int64_t)blockIdx.x * kGX + 16 * w + lr;  // this lane's column
     const bool xok = x < nx;
     const cT<T>* Gp = G + g_off(B, mn, 0, xok ? x : nx - 1);
+    // reduction order: step s takes j = kstep_index(s, lq), so that a lane's
+    // weights of KG consecutive steps are consecutive in a row (one vector load)
     cT<T> g[KS];
 #pragma unroll
     for (int s = 0; s < KS; s++) {
-        const int j = 4 * s + lq;
+        const int j = kstep_index<KS>(s, lq);
         const cT<T> v = Gp[(int64_t)(j < n2 ? j : 0) * kGX];
         g[s] = (j < n2 && xok) ? v : czero<T>();
     }
-    const T* W = X.Wf + (size_t)mn * B * n2;
+    const T* W = X.Wf + (size_t)mn * B * X.wfs;
     acc_t re[RT], im[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; rt++) {
@@ -215,12 +229,19 @@ __global__ __launch_bounds__(256) void k_beta_fwd_mfma(XformDev<T> X, const cT<T
         im[rt] = MF::zero();
         const int l = 16 * rt + lr;
         const bool lok = l >= lmin && l < B;
+        const T* Wl = W + (size_t)(l < B ? l : lmax) * X.wfs;
 #pragma unroll
-        for (int s = 0; s < KS; s++) {
-            const int j = 4 * s + lq;
-            const T a = (lok && j < n2) ? W[(size_t)l * n2 + j] : T(0);
-            re[rt] = MF::fma(a, g[s].x, re[rt]);
-            im[rt] = MF::fma(a, g[s].y, im[rt]);
+        for (int c = 0; c < KS / KGroup<KS>::n; c++) {
+            constexpr int KG = KGroup<KS>::n;
+            typedef T VecT __attribute__((ext_vector_type(KG)));
+            const VecT wv = *reinterpret_cast<const VecT*>(Wl + 4 * KG * c + KG * lq);
+#pragma unroll
+            for (int t = 0; t < KG; t++) {
+                const int s = KG * c + t;
+                const T a = lok ? wv[t] : T(0);  // zero beyond n2 in the table
+                re[rt] = MF::fma(a, g[s].x, re[rt]);
+                im[rt] = MF::fma(a, g[s].y, im[rt]);
+            }
         }
     }
     if (!xok) return;
@@ -277,17 +298,19 @@ __global__ __launch_bounds__(512) void k_beta_inv_mfma(XformDev<T> X, const cT<T
             f[s] = (lok && xok) ? v : czero<T>();
         }
     }
-    const T* W = X.Wi + (size_t)mn * n2 * B;
+    const T* W = X.Wi + (size_t)mn * n2 * X.wis;
     acc_t re[RT], im[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; rt++) {
         re[rt] = MF::zero();
         im[rt] = MF::zero();
         const int j = 16 * rt + lr;
+        // (the grouped vector-load order of the forward stage measured 10%
+        // slower here in FP64)
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int l = 4 * s + lq;
-            const T a = (j < n2 && l >= lmin && l < B) ? W[(size_t)j * B + l] : T(0);
+            const T a = (j < n2 && l >= lmin && l < B) ? W[(size_t)j * X.wis + l] : T(0);
             re[rt] = MF::fma(a, f[s].x, re[rt]);
             im[rt] = MF::fma(a, f[s].y, im[rt]);
         }
@@ -341,7 +364,7 @@ __global__ __launch_bounds__(256) void k_beta_inv(XformDev<T> X, const cT<T>* __
     const int mn = (M * M - 1) / 2 + blockIdx.y;  // the stored half of H (see get_h)
     const int m = mn % M - lmax, n = mn / M - lmax;
     const int lmin = max(abs(m), abs(n));
-    for (int i = threadIdx.x; i < B * n2; i += blockDim.x) sW[i] = X.Wi[(size_t)mn * B * n2 + i];
+    for (int i = threadIdx.x; i < B * n2; i += blockDim.x) sW[i] = X.Wi[((size_t)mn * n2 + i / B) * X.wis + i % B];
     __syncthreads();
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= X.nx) return;
