@@ -270,7 +270,9 @@ __global__ __launch_bounds__(512) void k_beta_inv_mfma(XformDev<T> X, const cT<T
                                                        cT<T>* __restrict__ G) {
     using MF = Mfma<T>;
     using acc_t = typename MF::acc;
-    // one 16-row tile of j at a time, [j % 16][x % 16][w], stored as whole lines
+    // one 16-row tile of j at a time, [j % 16][x % 16][w ^ ((x % 16) / 2)], stored
+    // as whole lines (the swizzle spreads the 16 omega points of a wave's write
+    // over distinct banks; a line's reads stay within one 128-byte row group)
     __shared__ cT<T> tile[16 * 16 * 8];
     const int B = X.B, n2 = 2 * B, lmax = B - 1, M = 2 * B - 1, h0 = (M * M - 1) / 2;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
@@ -325,7 +327,8 @@ __global__ __launch_bounds__(512) void k_beta_inv_mfma(XformDev<T> X, const cT<T
     for (int rt = 0; rt < RT; rt++) {
         if (16 * rt >= n2) break;
 #pragma unroll
-        for (int v = 0; v < 4; v++) tile[(MF::orow(lq, v) * 16 + lr) * 8 + w] = cmk<T>(re[rt][v], im[rt][v]);
+        for (int v = 0; v < 4; v++)
+            tile[(MF::orow(lq, v) * 16 + lr) * 8 + (w ^ (lr >> 1))] = cmk<T>(re[rt][v], im[rt][v]);
         __syncthreads();
 #pragma unroll
         for (int p = threadIdx.x; p < 16 * 8 * 8 * PR; p += 512) {
@@ -335,7 +338,8 @@ __global__ __launch_bounds__(512) void k_beta_inv_mfma(XformDev<T> X, const cT<T
             if (j < n2 && r8 < mrows && xa < nx) {
                 cT<T>* dst = G + h_base(B, j, xa) + 2 * (8 * (int)blockIdx.y + r8);
                 if (PR == 1) {
-                    const cT<T> a = tile[(jl * 16 + 2 * xp) * 8 + r8], b = tile[(jl * 16 + 2 * xp + 1) * 8 + r8];
+                    const cT<T> a = tile[(jl * 16 + 2 * xp) * 8 + (r8 ^ xp)],
+                                b = tile[(jl * 16 + 2 * xp + 1) * 8 + (r8 ^ xp)];
                     if (xa + 1 < nx) {
                         typedef T V4 __attribute__((ext_vector_type(4)));
                         V4 q;
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(512) void k_beta_inv_mfma(XformDev<T> X, const cT<T
                     }
                 } else {
                     const int xo = p % 2;
-                    if (xa + xo < nx) dst[xo] = tile[(jl * 16 + 2 * xp + xo) * 8 + r8];
+                    if (xa + xo < nx) dst[xo] = tile[(jl * 16 + 2 * xp + xo) * 8 + (r8 ^ xp)];
                 }
             }
         }
